@@ -1,0 +1,242 @@
+/*
+ * rtpg.h — C ABI of the MI355X (gfx950) path-tracing and path-guiding core.
+ *
+ * Drop-in boundary for the tracer/integrator path of RTLib's OptiX test
+ * application.  The reference has no C ABI: its surface is C++ classes and
+ * templates.  Each entry point below names the reference interface it replaces
+ * (paths relative to the reference repository root; OPX7 = RTLib/Src/Ext/OPX7,
+ * Test = OPX7/Test, Utils = OPX7/Include/RTLib/Ext/OPX7/Utils).
+ *
+ * Conventions
+ *   - Every function returns 0 (RTPG_OK) on success or a negative status; no
+ *     C++ exception crosses the ABI.  rtpg_last_error() returns a thread-local
+ *     description of the last failure.
+ *   - Host arrays are owned by the caller and only read/written during the call.
+ *     Device memory is owned by the library.
+ *   - All work is ordered on the context's HIP stream (set with
+ *     rtpg_context_set_stream, default: the context's own stream).  Download
+ *     functions synchronise the stream.  One host thread per context.
+ *   - Handles are destroyed with their *_destroy function; destroying a context
+ *     does not destroy guides created on it (destroy those first).
+ */
+#ifndef RTPG_H
+#define RTPG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTPG_OK 0
+#define RTPG_ERR_INVALID (-1)
+#define RTPG_ERR_HIP (-2)
+#define RTPG_ERR_STATE (-3)
+#define RTPG_ERR_LIMIT (-4)
+
+/* Render flags.  Bits 1..64 keep the meaning of the reference's ParamFlag
+ * (Test/cuda/SimpleKernel.h:149-159). */
+#define RTPG_FLAG_NEE 1u
+#define RTPG_FLAG_RIS 2u
+#define RTPG_FLAG_USE_GRID 4u
+#define RTPG_FLAG_USE_TREE 8u
+#define RTPG_FLAG_GUIDE_SAMPLE 16u /* reference PARAM_FLAG_BUILD: sample from the guide */
+#define RTPG_FLAG_FINAL 32u        /* no recording */
+#define RTPG_FLAG_LOCATE 64u       /* insert hash-grid cells */
+#define RTPG_FLAG_MIS 128u
+#define RTPG_FLAG_RR 256u
+#define RTPG_FLAG_USE_MLP 512u
+
+/* Material types (per triangle). */
+#define RTPG_MAT_LAMBERT 0u
+#define RTPG_MAT_GGX 1u   /* param = roughness (alpha) */
+#define RTPG_MAT_PHONG 2u /* reference "PHONG": param ignored, specular + shininess from spec[] */
+#define RTPG_MAT_GLASS 3u /* param = index of refraction */
+
+typedef struct rtpg_context_t* rtpg_context;
+typedef struct rtpg_sdtree_t* rtpg_sdtree;
+typedef struct rtpg_sdtree_ctrl_t* rtpg_sdtree_ctrl;
+typedef struct rtpg_hashgrid_t* rtpg_hashgrid;
+typedef struct rtpg_mqtree_t* rtpg_mqtree;
+typedef struct rtpg_mqtree_ctrl_t* rtpg_mqtree_ctrl;
+
+const char* rtpg_last_error(void);
+const char* rtpg_version(void);
+
+/* --- context ---------------------------------------------------------------
+ * Replaces OPX7Context/CUDAContext creation (OPX7/Src/OPX7Context.cpp:274-292,
+ * Ext/CUDA/Src/CUDAContext.cpp) and CUDAStream::Synchronize. */
+int rtpg_context_create(int device, rtpg_context* out);
+int rtpg_context_destroy(rtpg_context ctx);
+int rtpg_context_set_stream(rtpg_context ctx, void* hip_stream);
+int rtpg_synchronize(rtpg_context ctx);
+
+/* --- scene ------------------------------------------------------------------
+ * Replaces geometry upload + GAS/IAS build (Test/Include/RTLibExtOPX7Test.h:
+ * 153-296, 515-656; OPX7/Src/OPX7Natives.cpp:195-244), the hitgroup material
+ * records (RTLibExtOPX7Test.h:1602-1638) and the mesh-light list upload
+ * (RTLibExtOPX7Test.h:700-725).
+ *   verts:    3*nv floats;  idx: 3*nt vertex indices
+ *   albedo:   4*nt floats (rgb, param) per triangle
+ *   mat_type: nt values (RTPG_MAT_*)
+ *   emission: 3*nt floats (0 = not emissive; emissive triangles form the light list)
+ *   spec:     4*nt floats (specular rgb, shininess) or NULL
+ * light_mode 0: lights picked uniformly per triangle; 1: pick a light mesh
+ * uniformly, then a triangle in it (reference MeshLightList::Sample,
+ * SimpleKernel.h:81-120), with light_mesh[nt] giving each triangle's mesh id. */
+int rtpg_scene_upload(rtpg_context ctx, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* albedo, const uint32_t* mat_type, const float* emission, const float* spec,
+                      uint32_t light_mode, const uint32_t* light_mesh);
+/* Replace per-triangle albedo (rgb, 3*nt) in original triangle order. */
+int rtpg_scene_set_albedo(rtpg_context ctx, const float* albedo_rgb);
+/* info: [0] BVH nodes, [1] BVH4 depth, [2] stack bound, [3] max leaf size,
+ *       [4] lights, [5] triangles.  aabb: world box (lo xyz, hi xyz). */
+int rtpg_scene_info(rtpg_context ctx, uint32_t info[6], float aabb[6]);
+
+/* --- camera / frame ---------------------------------------------------------
+ * Pinhole camera as RayGenData (SimpleKernel.h:191-199) with u, v, w from
+ * Camera::GetUVW (Core/Include/RTLib/Core/Camera.h:78-92). */
+int rtpg_camera_set(rtpg_context ctx, const float eye[3], const float u[3], const float v[3], const float w[3]);
+int rtpg_frame_resize(rtpg_context ctx, uint32_t width, uint32_t height);
+int rtpg_accum_clear(rtpg_context ctx);
+/* out: 3*W*H floats (sum of samples, row 0 = bottom), spp: samples accumulated. */
+int rtpg_accum_download(rtpg_context ctx, float* out, uint32_t* spp);
+
+/* --- render -----------------------------------------------------------------
+ * Replaces the Params H2D copy + TracerData::Launch -> optixLaunch
+ * (RTLibExtOPX7TestApplication.cpp:2191-2198, RTLibExtOPX7Test.h:1345-1348)
+ * and the raygen/closest-hit programs (Test/cuda/SimpleTrace.cu,
+ * SimpleGuide.cu, SimpleGuide2.cu). */
+typedef struct rtpg_render_params {
+  uint32_t flags;          /* RTPG_FLAG_* */
+  uint32_t spp;            /* samples per pixel in this call */
+  uint32_t max_depth;      /* path vertices (reference MaxDepth) */
+  uint32_t rr_depth;       /* Russian roulette from this depth (with RTPG_FLAG_RR) */
+  uint32_t num_candidates; /* RIS candidates (reference Ris.NumCandidates) */
+  uint32_t sample_offset;  /* index of the first sample, for the RNG streams */
+  uint64_t seed;
+  float bg[3];             /* background radiance (reference MissData.BgColor) */
+  float fraction;          /* guide-vs-BSDF mixture (reference SdTree.Fraction) */
+  rtpg_sdtree sdtree;      /* guide for RTPG_FLAG_USE_TREE, else NULL */
+  rtpg_hashgrid hashgrid;  /* spatial hash for RTPG_FLAG_USE_GRID */
+  rtpg_mqtree mqtree;      /* direction quadtrees for RTPG_FLAG_USE_GRID */
+  uint32_t max_paths;      /* paths in flight per chunk (0 = default 4M) */
+} rtpg_render_params;
+int rtpg_render(rtpg_context ctx, const rtpg_render_params* params);
+/* Last render's counters: [0] paths, [1] radiance rays, [2] shadow rays. */
+int rtpg_render_stats(rtpg_context ctx, uint64_t stats[3]);
+
+/* --- ray queries (the reference's TraceRadiance / TraceOccluded,
+ * SimpleKernel.h:347-366) ----------------------------------------------------
+ * rays: 8*n floats (ox, oy, oz, tmin, dx, dy, dz, tmax).
+ * hits: 4*n floats (t, u, v, primitive id bits; t = inf on a miss). */
+int rtpg_trace_closest(rtpg_context ctx, const float* rays, uint32_t n, float* hits);
+int rtpg_trace_occluded(rtpg_context ctx, const float* rays, uint32_t n, uint32_t* occluded);
+
+/* --- SD-tree ----------------------------------------------------------------
+ * Replaces RTSTreeWrapperT (Utils/OPX7UtilsPathGuiding.h:1333-1767). */
+int rtpg_sdtree_create(rtpg_context ctx, const float aabb_min[3], const float aabb_max[3], uint32_t max_dtree_depth,
+                       float fraction, rtpg_sdtree* out);
+int rtpg_sdtree_destroy(rtpg_sdtree tree);
+/* RTSTreeWrapperT::Clear (:1634-1636): one leaf, empty DTrees. */
+int rtpg_sdtree_clear(rtpg_sdtree tree);
+/* Spatial split threshold: mode 0 = reference size_t(sqrt(2^iter*spp/4)*c)
+ * (:1645-1656, c = 4000); mode 1 = c*sqrt(2^iter). */
+int rtpg_sdtree_set_split(rtpg_sdtree tree, int mode, float c, int max_mb, float subdiv_th);
+/* RTSTreeWrapperT::Reset(iter, spp) (:1645-1656): refine the spatial tree from
+ * the recorded weights, then rebuild every building DTree from its sampling tree. */
+int rtpg_sdtree_reset(rtpg_sdtree tree, int iter, int samples_per_pass);
+/* RTSTreeWrapperT::Build (:1657-1735): sums, areas, sampling := building. */
+int rtpg_sdtree_build(rtpg_sdtree tree);
+/* Device record of n samples (DTreeT::RecordIrradiance, :419-427) at
+ * positions pos[3n], canonical directions canon[2n]. */
+int rtpg_sdtree_record(rtpg_sdtree tree, const float* pos, const float* canon, const float* irradiance,
+                       const float* stat_weight, uint32_t n);
+/* STree lookup (STreeT::GetDTreeWrapper, :639-648): DTree index per position. */
+int rtpg_sdtree_lookup(rtpg_sdtree tree, const float* pos, uint32_t n, uint32_t* dtree);
+/* Draw canonical directions from the sampling trees (DTreeNodeT::Sample,
+ * :228-277); stream i of PCG32(seed) per sample. */
+int rtpg_sdtree_sample(rtpg_sdtree tree, const uint32_t* dtree, uint32_t n, uint64_t seed, float* canon);
+/* Sphere density of canonical directions (DTreeT::Pdf, :484-489). */
+int rtpg_sdtree_pdf(rtpg_sdtree tree, const uint32_t* dtree, const float* canon, uint32_t n, float* pdf);
+/* Sizes: [0] STree nodes, [1] DTrees, [2] building nodes, [3] sampling nodes. */
+int rtpg_sdtree_sizes(rtpg_sdtree tree, uint32_t sizes[4]);
+/* Flattened export/import (reference Upload/Download, :1353-1615).  Any
+ * pointer may be NULL.  stree[s0]; per DTree (n = s1): b_cnt, s_cnt (u32),
+ * scal[6*n] = b_sw, b_sum, b_area, s_sw, s_sum, s_area; pools: b_sums[4*s2],
+ * b_child[4*s2] (u16), s_sums[4*s3], s_child[4*s3] (u16).  Node pools are
+ * concatenated per DTree in DTree order. */
+int rtpg_sdtree_export(rtpg_sdtree tree, uint32_t* stree, uint32_t* b_cnt, uint32_t* s_cnt, float* scal,
+                       float* b_sums, uint16_t* b_child, float* s_sums, uint16_t* s_child);
+int rtpg_sdtree_import(rtpg_sdtree tree, uint32_t n_stree, const uint32_t* stree, uint32_t n_dtrees,
+                       const uint32_t* b_cnt, const uint32_t* s_cnt, const float* scal, const float* b_sums,
+                       const uint16_t* b_child, const float* s_sums, const uint16_t* s_child);
+
+/* --- SD-tree pass controller ------------------------------------------------
+ * Replaces RTSTreeControllerT (Utils/OPX7UtilsPathGuiding.h:1769-1920).
+ * state: 0 Record, 1 RecordAndSample, 2 Sample. */
+int rtpg_sdtree_ctrl_create(rtpg_sdtree tree, uint32_t sample_for_budget, uint32_t iteration_for_built,
+                            float ratio_for_budget, uint32_t sample_per_launch, rtpg_sdtree_ctrl* out);
+int rtpg_sdtree_ctrl_destroy(rtpg_sdtree_ctrl c);
+int rtpg_sdtree_ctrl_start(rtpg_sdtree_ctrl c);
+int rtpg_sdtree_ctrl_beg_trace(rtpg_sdtree_ctrl c);
+/* samples: samples traced since BegTrace (0 = sample_per_launch). */
+int rtpg_sdtree_ctrl_end_trace(rtpg_sdtree_ctrl c, uint32_t samples);
+/* q: [0] state, [1] iteration, [2] samples in the current pass so far,
+ *    [3] samples of the current pass, [4] executing (0/1), [5] samples done. */
+int rtpg_sdtree_ctrl_query(rtpg_sdtree_ctrl c, uint32_t q[6]);
+
+/* --- hash grid ---------------------------------------------------------------
+ * Replaces DoubleBufferedHashGrid3T / DoubleBufferedHashGrid3Buffer
+ * (Utils/OPX7UtilsGrid.h:157-353, Test/Include/RTLibExtOPX7Test.h:1472-1570).
+ * key_mode 0: the reference's pcg1d-chained bucket and linear-index checksum;
+ * key_mode 1: bucket = pcg1d(key) % (slots/32), checksum = key + 1 with key the
+ * 30-bit Morton code of the 10-bit cell coordinates (bounds <= 1024 per axis). */
+int rtpg_hashgrid_create(rtpg_context ctx, const float aabb_min[3], const float aabb_max[3], const uint32_t bounds[3],
+                         uint32_t slots, uint32_t key_mode, rtpg_hashgrid* out);
+int rtpg_hashgrid_destroy(rtpg_hashgrid g);
+int rtpg_hashgrid_clear(rtpg_hashgrid g);
+/* DoubleBufferedHashGrid3Buffer::Update (:1544-1558): previous := current. */
+int rtpg_hashgrid_update(rtpg_hashgrid g);
+/* insert = 0: FindFromPrv (read-only); insert = 1: FindFromCur (atomicCAS insert).
+ * slot = 0xFFFFFFFF when not found / the 32-slot block is full. */
+int rtpg_hashgrid_find(rtpg_hashgrid g, const float* pos, uint32_t n, int insert, uint32_t* slot);
+/* which = 0: current table, 1: previous table. */
+int rtpg_hashgrid_download(rtpg_hashgrid g, int which, uint32_t* checksums);
+
+/* --- Morton direction quadtrees ------------------------------------------
+ * Replaces RTMortonQuadTreeWrapperT (Utils/OPX7UtilsMorton.h:590-764) and the
+ * device MortonQuadTreeT (:253-449).  Per slot a full pyramid of
+ * (4^(level+1)-1)/3 floats; which = 0 building, 1 sampling. */
+int rtpg_mqtree_create(rtpg_context ctx, uint32_t slots, uint32_t level, float fraction, rtpg_mqtree* out);
+int rtpg_mqtree_destroy(rtpg_mqtree m);
+int rtpg_mqtree_allocate(rtpg_mqtree m);  /* zero both pyramids (:608-639) */
+int rtpg_mqtree_update(rtpg_mqtree m);    /* swap, zero the new building pyramid (:640-651) */
+int rtpg_mqtree_build(rtpg_mqtree m);     /* mortonBuildKernel on the sampling pyramid (OPX7UtilsMortonBuild.cu:2-16) */
+int rtpg_mqtree_record(rtpg_mqtree m, const uint32_t* slot, const float* canon, const float* value, uint32_t n);
+/* out: 3*n floats (u, v, sphere pdf) */
+int rtpg_mqtree_sample(rtpg_mqtree m, const uint32_t* slot, uint32_t n, uint64_t seed, float* out);
+int rtpg_mqtree_pdf(rtpg_mqtree m, const uint32_t* slot, const float* canon, uint32_t n, float* pdf);
+int rtpg_mqtree_download(rtpg_mqtree m, int which, float* weights);
+int rtpg_mqtree_upload(rtpg_mqtree m, int which, const float* weights);
+/* info: [0] slots, [1] level, [2] floats per slot */
+int rtpg_mqtree_info(rtpg_mqtree m, uint32_t info[3]);
+
+/* --- hash-grid pass controller ---------------------------------------------
+ * Replaces RTMortonQuadTreeControllerT (Utils/OPX7UtilsMorton.h:766-993); at
+ * the end of each pass it also snapshots the hash grid (previous := current),
+ * as the application does (RTLibExtOPX7TestApplication.cpp:2208-2222).
+ * state: 0 Record, 1 RecordAndSample, 2 Sample, 3 Locate. */
+int rtpg_mqtree_ctrl_create(rtpg_mqtree tree, rtpg_hashgrid grid, uint32_t sample_for_budget,
+                            uint32_t iteration_for_built, uint32_t iteration_for_locate, float ratio_for_budget,
+                            uint32_t sample_per_launch, rtpg_mqtree_ctrl* out);
+int rtpg_mqtree_ctrl_destroy(rtpg_mqtree_ctrl c);
+int rtpg_mqtree_ctrl_start(rtpg_mqtree_ctrl c);
+int rtpg_mqtree_ctrl_beg_trace(rtpg_mqtree_ctrl c);
+int rtpg_mqtree_ctrl_end_trace(rtpg_mqtree_ctrl c, uint32_t samples);
+int rtpg_mqtree_ctrl_query(rtpg_mqtree_ctrl c, uint32_t q[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTPG_H */

@@ -1,0 +1,1498 @@
+/*
+ * oracle.c — CPU oracle of the rtpg path (TEST INFRASTRUCTURE ONLY; see oracle.h).
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, so products and sums are
+ * rounded one operation at a time, as the HIP kernels do where bit-exactness
+ * is claimed).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI_F 3.14159265358979323846f
+#define INV_PI_F 0.31830988618379067154f
+#define TWO_PI_F 6.28318530717958647692f
+#define INV_2PI_F 0.15915494309189533577f
+#define LEAF_BIT 0x80000000u
+#define INVALID 0xFFFFFFFFu
+
+/* ======================================================================
+ * random numbers
+ * ==================================================================== */
+typedef struct {
+  uint64_t state, inc;
+} pcg32;
+
+static uint32_t pcg32_next(pcg32* r) {
+  uint64_t old = r->state;
+  r->state = old * 6364136223846793005ULL + r->inc;
+  uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+  uint32_t rot = (uint32_t)(old >> 59u);
+  return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+}
+
+/* pcg32_srandom_r (O'Neill, pcg-c-basic) */
+static pcg32 pcg32_seed(uint64_t initstate, uint64_t seq) {
+  pcg32 r;
+  r.state = 0u;
+  r.inc = (seq << 1u) | 1u;
+  pcg32_next(&r);
+  r.state += initstate;
+  pcg32_next(&r);
+  return r;
+}
+
+static uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+static pcg32 path_rng(uint64_t seed, uint32_t pixel, uint32_t sample) {
+  return pcg32_seed(mix64(seed ^ mix64((uint64_t)sample)), (uint64_t)pixel);
+}
+
+/* to_uniform_float, Math/Math.h:682-695 */
+float orc_to_uniform_float(uint32_t v) {
+  uint32_t bits = (v >> 9) | 0x3F800000u;
+  float f;
+  memcpy(&f, &bits, 4);
+  return f - 1.0f;
+}
+
+static float rnd(pcg32* r) { return orc_to_uniform_float(pcg32_next(r)); }
+
+void orc_rng_stream(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t n, uint32_t* out) {
+  pcg32 r = path_rng(seed, pixel, sample);
+  for (uint32_t i = 0; i < n; ++i) out[i] = pcg32_next(&r);
+}
+
+void orc_pcg32_raw(uint64_t initstate, uint64_t seq, uint32_t n, uint32_t* out) {
+  pcg32 r = pcg32_seed(initstate, seq);
+  for (uint32_t i = 0; i < n; ++i) out[i] = pcg32_next(&r);
+}
+
+void orc_rng_floats(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t n, float* out) {
+  pcg32 r = path_rng(seed, pixel, sample);
+  for (uint32_t i = 0; i < n; ++i) out[i] = rnd(&r);
+}
+
+/* Xorshift32::next, Math/Random.h:44-50 */
+uint32_t orc_xorshift32(uint32_t* state) {
+  uint32_t y = *state;
+  y = y ^ (y << 13);
+  y = y ^ (y >> 17);
+  return *state = y ^ (y << 5);
+}
+
+/* pcg1d, Math/Hash.h:13-18 */
+uint32_t orc_pcg1d(uint32_t input) {
+  uint32_t state = input * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+/* xxhash, Math/Hash.h:50-66 */
+uint32_t orc_xxhash32(uint32_t data, uint32_t seed) {
+  const uint32_t P2 = 2246822519U, P3 = 3266489917U, P4 = 668265263U, P5 = 374761393U;
+  uint32_t h = data * P3;
+  h += seed + P5 + 4U;
+  h = (h << 17) | (h >> 15);
+  h *= P4;
+  h ^= h >> 15;
+  h *= P2;
+  h ^= h >> 13;
+  h *= P3;
+  h ^= h >> 16;
+  return h;
+}
+
+/* ======================================================================
+ * Morton codes: bit-by-bit interleave (the product uses magic-number
+ * spreads; this loop form is the definition).
+ * ==================================================================== */
+uint32_t orc_morton2(uint32_t x, uint32_t y) {
+  uint32_t c = 0;
+  for (int b = 0; b < 16; ++b) {
+    c |= ((x >> b) & 1u) << (2 * b);
+    c |= ((y >> b) & 1u) << (2 * b + 1);
+  }
+  return c;
+}
+
+uint32_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t c = 0;
+  for (int b = 0; b < 10; ++b) {
+    c |= ((x >> b) & 1u) << (3 * b);
+    c |= ((y >> b) & 1u) << (3 * b + 1);
+    c |= ((z >> b) & 1u) << (3 * b + 2);
+  }
+  return c;
+}
+
+static void morton2_decode(uint32_t c, uint32_t* x, uint32_t* y) {
+  uint32_t xx = 0, yy = 0;
+  for (int b = 0; b < 16; ++b) {
+    xx |= ((c >> (2 * b)) & 1u) << b;
+    yy |= ((c >> (2 * b + 1)) & 1u) << b;
+  }
+  *x = xx;
+  *y = yy;
+}
+
+/* ======================================================================
+ * vectors
+ * ==================================================================== */
+typedef struct {
+  float x, y, z;
+} v3;
+static v3 V(float x, float y, float z) {
+  v3 r = {x, y, z};
+  return r;
+}
+static v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 mul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static v3 scl(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static v3 divs(v3 a, float s) { return V(a.x / s, a.y / s, a.z / s); }
+static v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 cross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static float len(v3 a) { return sqrtf(dot(a, a)); }
+static v3 nrm(v3 a) { return scl(a, 1.0f / sqrtf(dot(a, a))); }
+static float avg3(v3 a) { return (a.x + a.y + a.z) / 3.0f; }
+static float max3(v3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+static int finite3(v3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+static v3 ld3(const float* p) { return V(p[0], p[1], p[2]); }
+
+/* canonical maps, Math/VectorFunction.h:5698-5720 */
+static void dir_to_canonical(v3 d, float* px, float* py) {
+  if (!finite3(d)) {
+    *px = 0.0f;
+    *py = 0.0f;
+    return;
+  }
+  float z = fminf(fmaxf(d.z, -1.0f), 1.0f);
+  float phi = atan2f(d.y, d.x);
+  while (phi < 0.0f) phi += TWO_PI_F;
+  *px = (z + 1.0f) / 2.0f;
+  *py = phi * INV_2PI_F;
+}
+static v3 canonical_to_dir(float px, float py) {
+  float ct = 2.0f * px - 1.0f;
+  float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+  float phi = TWO_PI_F * py;
+  return V(st * cosf(phi), st * sinf(phi), ct);
+}
+void orc_dir_to_canonical(const float* d, float* uv) { dir_to_canonical(ld3(d), &uv[0], &uv[1]); }
+void orc_canonical_to_dir(const float* uv, float* d) {
+  v3 r = canonical_to_dir(uv[0], uv[1]);
+  d[0] = r.x;
+  d[1] = r.y;
+  d[2] = r.z;
+}
+
+/* ONB, Math/VectorFunction.h:5666-5697 */
+typedef struct {
+  v3 t, b, n;
+} onb;
+static onb make_onb(v3 normal) {
+  onb o;
+  o.n = nrm(normal);
+  if (fabsf(o.n.x) > fabsf(o.n.z)) {
+    o.b = V(-o.n.y, o.n.x, 0.0f);
+  } else {
+    o.b = V(0.0f, -o.n.z, o.n.y);
+  }
+  o.b = nrm(o.b);
+  o.t = nrm(cross(o.n, o.b));
+  return o;
+}
+static v3 onb_local(const onb* o, v3 d) { return add(add(scl(o->t, d.x), scl(o->b, d.y)), scl(o->n, d.z)); }
+
+/* ======================================================================
+ * hash grid, Utils/OPX7UtilsGrid.h:157-353
+ * ==================================================================== */
+void orc_grid_cell_hash(const orc_grid* g, const float* pos, uint32_t* cell3, uint32_t* bucket, uint32_t* checksum) {
+  uint32_t c[3];
+  for (int k = 0; k < 3; ++k) {
+    float f = (pos[k] - g->origin[k]) / g->size[k];
+    float gk = (float)g->bounds[k] * f;
+    /* CUDA's float->uint conversion saturates: negative -> 0 */
+    uint32_t ci = gk > 0.0f ? (gk < 4294967040.0f ? (uint32_t)gk : 0xFFFFFFFFu) : 0u;
+    c[k] = ci < g->bounds[k] - 1u ? ci : g->bounds[k] - 1u;
+    cell3[k] = c[k];
+  }
+  uint32_t nb = g->slots / 32u;
+  if (g->key_mode == 1u) {
+    uint32_t key = orc_morton3(c[0], c[1], c[2]);
+    *bucket = orc_pcg1d(key) % nb;
+    *checksum = key + 1u;
+  } else {
+    *bucket = orc_pcg1d(g->slots + c[2] + orc_pcg1d(c[1] + orc_pcg1d(c[0]))) % nb;
+    *checksum = 1u + g->bounds[0] * g->bounds[1] * c[2] + g->bounds[0] * c[1] + c[0];
+  }
+}
+
+/* GetPrvCellIndex :298-319 (read-only) / GetCurCellIndex :266-297 (host branch) */
+void orc_grid_find(const orc_grid* g, uint32_t* cur, const uint32_t* prv, const float* pos, uint32_t n, int insert,
+                   uint32_t* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t c3[3], b, cs;
+    orc_grid_cell_hash(g, pos + 3 * i, c3, &b, &cs);
+    uint32_t res = INVALID;
+    for (uint32_t k = 0; k < 32u; ++k) {
+      uint32_t idx = 32u * b + k;
+      if (insert) {
+        uint32_t prev = cur[idx];
+        if (prev == 0u) cur[idx] = cs;
+        if (prev == 0u || prev == cs) {
+          res = idx;
+          break;
+        }
+      } else {
+        uint32_t c = prv[idx];
+        if (c == 0u) break;
+        if (c == cs) {
+          res = idx;
+          break;
+        }
+      }
+    }
+    out[i] = res;
+  }
+}
+
+/* ======================================================================
+ * Morton quadtree, Utils/OPX7UtilsMorton.h:253-449, OPX7UtilsMortonBuild.cu
+ * ==================================================================== */
+static uint32_t level_offset(uint32_t level) { return ((1u << (2u * level)) - 1u) / 3u; }
+
+static uint32_t canon_bin(float c, uint32_t nn) {
+  uint32_t x = (uint32_t)(c * (float)nn);
+  return x < nn ? x : nn - 1u;
+}
+
+/* MortonQuadTreeT::Record, leaf-only branch :262-285 */
+void orc_mq_record(float* w, uint32_t per_cell, uint32_t level, const uint32_t* slot, const float* canon,
+                   const float* val, uint32_t n) {
+  uint32_t nn = 1u << level;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (slot[i] == INVALID) continue;
+    uint32_t code = orc_morton2(canon_bin(canon[2 * i], nn), canon_bin(canon[2 * i + 1], nn));
+    w[(size_t)per_cell * slot[i] + level_offset(level) + code] += val[i];
+  }
+}
+
+/* mortonBuildKernel :2-16, the serial loop with exact offsets (4^k-1)/3 */
+void orc_mq_build(float* w, uint32_t level, uint32_t per_cell, uint32_t cells) {
+  for (uint32_t c = 0; c < cells; ++c) {
+    float* b = w + (size_t)per_cell * c;
+    for (uint32_t i = 0; i < level; ++i) {
+      uint32_t src = level_offset(level - i);
+      uint32_t dst = level_offset(level - 1 - i);
+      uint32_t ncode = 1u << (2u * (level - i));
+      for (uint32_t code = 0; code < ncode; ++code) b[dst + (code >> 2)] += b[src + code];
+    }
+  }
+}
+
+/* MortonQuadTreeT::SampleAndPdf :286-344 (pdf times 1/(4 pi), wrapper :486-500) */
+void orc_mq_sample(const float* w, uint32_t per_cell, uint32_t level, const uint32_t* slot, uint32_t n,
+                   uint64_t seed, float* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    pcg32 r = pcg32_seed(seed, i);
+    const float* b = w + (size_t)per_cell * slot[i];
+    float total = b[0];
+    if (total <= 0.0f) {
+      out[3 * i] = rnd(&r);
+      out[3 * i + 1] = rnd(&r);
+      out[3 * i + 2] = 1.0f * 0.25f * INV_PI_F;
+      continue;
+    }
+    uint32_t size = 4, offset = 1, code = 0;
+    for (uint32_t l = 0; l < level; ++l) {
+      float wv[4];
+      for (int k = 0; k < 4; ++k) wv[k] = b[offset + (code << 2) + (uint32_t)k];
+      uint32_t local = 0;
+      float partial = wv[0] + wv[2];
+      float boundary = partial / total;
+      float sample = rnd(&r);
+      if (sample < boundary) {
+        sample /= boundary;
+        boundary = wv[0] / partial;
+      } else {
+        partial = wv[1] + wv[3];
+        sample = (sample - boundary) / (1.0f - boundary);
+        boundary = wv[1] / partial;
+        local |= 1u;
+      }
+      if (sample >= boundary) local |= 2u;
+      total = wv[local];
+      code = (code << 2) + local;
+      offset += size;
+      size *= 4u;
+    }
+    uint32_t xi, yi;
+    morton2_decode(code, &xi, &yi);
+    float nn = (float)(1u << level);
+    float a = rnd(&r);
+    float c = rnd(&r);
+    out[3 * i] = ((float)xi + a) / nn;
+    out[3 * i + 1] = ((float)yi + c) / nn;
+    out[3 * i + 2] = total / b[0] * (nn * nn) * 0.25f * INV_PI_F;
+  }
+}
+
+static float mq_pdf1(const float* b, uint32_t level, float cu, float cv) {
+  if (b[0] <= 0.0f) return 1.0f * 0.25f * INV_PI_F;
+  uint32_t nn = 1u << level;
+  uint32_t code = orc_morton2(canon_bin(cu, nn), canon_bin(cv, nn));
+  return b[level_offset(level) + code] / b[0] * (float)(nn * nn) * 0.25f * INV_PI_F;
+}
+
+/* MortonQuadTreeT::Pdf :401-414 */
+void orc_mq_pdf(const float* w, uint32_t per_cell, uint32_t level, const uint32_t* slot, const float* canon,
+                uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; ++i)
+    out[i] = mq_pdf1(w + (size_t)per_cell * slot[i], level, canon[2 * i], canon[2 * i + 1]);
+}
+
+/* ======================================================================
+ * SD-tree, Utils/OPX7UtilsPathGuiding.h
+ * ==================================================================== */
+static int child_idx(float* px, float* py) { /* DTreeNodeT::GetChildIdx :69-82 */
+  int r = 0;
+  if (*px < 0.5f) {
+    *px *= 2.0f;
+  } else {
+    *px = 2.0f * *px - 1.0f;
+    r |= 1;
+  }
+  if (*py < 0.5f) {
+    *py *= 2.0f;
+  } else {
+    *py = 2.0f * *py - 1.0f;
+    r |= 2;
+  }
+  return r;
+}
+
+/* DTreeNodeT::GetArea :342-355 (recursive) */
+static float dnode_area(const float* sums, const uint16_t* child, uint32_t k) {
+  float result = 0.0f;
+  for (int i = 0; i < 4; ++i) {
+    if (sums[4 * k + i] > 0.0f) {
+      uint16_t c = child[4 * k + i];
+      if (c == 0) {
+        result += 1.0f / 4.0f;
+      } else {
+        result += dnode_area(sums, child, c) / 4.0f;
+      }
+    }
+  }
+  return result;
+}
+
+/* DTreeNodeT::Build :356-369 (recursive post-order) */
+static void dnode_build(float* sums, const uint16_t* child, uint32_t k) {
+  for (int i = 0; i < 4; ++i) {
+    uint16_t c = child[4 * k + i];
+    if (c == 0) continue;
+    dnode_build(sums, child, c);
+    float sum = 0.0f;
+    for (int j = 0; j < 4; ++j) sum += sums[4 * c + j];
+    sums[4 * k + i] = sum;
+  }
+}
+
+/* RTDTreeT::Build :948-957, per DTree of the concatenated pool */
+void orc_dtree_build(uint32_t n_dtrees, const uint32_t* cnt, float* sums, const uint16_t* child, float* out_sum,
+                     float* out_area) {
+  size_t base = 0;
+  for (uint32_t j = 0; j < n_dtrees; ++j) {
+    float* s = sums + 4 * base;
+    const uint16_t* c = child + 4 * base;
+    dnode_build(s, c, 0);
+    out_area[j] = dnode_area(s, c, 0);
+    float sum = 0.0f;
+    for (int i = 0; i < 4; ++i) sum += s[i];
+    out_sum[j] = sum;
+    base += cnt[j];
+  }
+}
+
+/* RTDTreeT::Reset :888-947 — DFS with an explicit LIFO stack; a source leaf
+ * continues as a node of the new tree whose four sums are a quarter of the slot. */
+uint32_t orc_dtree_reset(const float* src_sums, const uint16_t* src_child, float total, uint32_t max_depth,
+                         float th, uint16_t* out_child, uint32_t* depth_out) {
+  enum { CAP = 65536 };
+  typedef struct {
+    uint32_t dst, src, depth, self;
+    float val;
+  } ent;
+  ent* st = (ent*)malloc(sizeof(ent) * 4 * CAP);
+  size_t sp = 0;
+  uint32_t count = 1, maxd = 0;
+  memset(out_child, 0, 4 * sizeof(uint16_t));
+  st[sp++] = (ent){0, 0, 1, 0, 0.0f};
+  int stop = 0;
+  while (sp > 0 && !stop) {
+    ent e = st[--sp];
+    if (e.depth > maxd) maxd = e.depth;
+    for (int i = 0; i < 4; ++i) {
+      float si = e.self ? e.val : src_sums[4 * e.src + i];
+      double fraction = total > 0.0f ? (double)(si / total) : pow(0.25, (double)e.depth);
+      if (e.depth < max_depth && fraction > (double)th) {
+        uint16_t sch = e.self ? 0 : src_child[4 * e.src + i];
+        ent ne;
+        ne.dst = count;
+        ne.depth = e.depth + 1;
+        if (!e.self && sch != 0) {
+          ne.src = sch;
+          ne.self = 0;
+          ne.val = 0.0f;
+        } else {
+          ne.src = 0;
+          ne.self = 1;
+          ne.val = si / 4.0f;
+        }
+        st[sp++] = ne;
+        out_child[4 * e.dst + i] = (uint16_t)count;
+        memset(out_child + 4 * (size_t)count, 0, 4 * sizeof(uint16_t));
+        count++;
+        if (count > 65535u) {
+          stop = 1;
+          break;
+        }
+      }
+    }
+  }
+  free(st);
+  *depth_out = maxd;
+  return count;
+}
+
+uint64_t orc_split_threshold(int mode, float c, int iter, int spp) {
+  if (mode == 0) return (uint64_t)(sqrt(pow(2.0, iter) * spp / 4.0f) * (double)c);
+  return (uint64_t)((double)c * sqrt(pow(2.0, iter)));
+}
+
+/* RTSTreeT::Refine :1275-1311 with ShallSplit :1270-1274 and SubDivide :1231-1247 */
+uint32_t orc_stree_refine(const uint32_t* stree, uint32_t n_nodes, const float* b_sw, const uint32_t* b_cnt,
+                          const uint32_t* s_cnt, uint64_t th, int max_mb, uint32_t* out, uint32_t cap, uint32_t* map,
+                          uint32_t* n_dtrees_out) {
+  typedef struct {
+    int leaf;
+    uint32_t child;
+    float sw;
+    int64_t src;
+  } hn;
+  if (cap < n_nodes) return 0;
+  hn* h = (hn*)malloc(sizeof(hn) * cap);
+  uint32_t n = n_nodes;
+  for (uint32_t i = 0; i < n; ++i) {
+    h[i].leaf = (stree[i] & LEAF_BIT) != 0;
+    h[i].child = h[i].leaf ? 0 : stree[i];
+    h[i].src = h[i].leaf ? (int64_t)(stree[i] & ~LEAF_BIT) : -1;
+    h[i].sw = h[i].leaf ? b_sw[h[i].src] : 0.0f;
+  }
+  int refine = 1;
+  if (max_mb >= 0) {
+    size_t fp = 0;
+    for (uint32_t i = 0; i < n; ++i)
+      fp += h[i].leaf ? 24u * ((size_t)b_cnt[h[i].src] + s_cnt[h[i].src]) + 48u : 96u;
+    if (fp / 1000000 >= (size_t)max_mb) refine = 0;
+  }
+  if (refine) {
+    size_t scap = 2 * (size_t)cap + 2, sp = 0;
+    uint32_t* st = (uint32_t*)malloc(sizeof(uint32_t) * scap);
+    st[sp++] = 0;
+    while (sp > 0) {
+      uint32_t idx = st[--sp];
+      if (h[idx].leaf && (uint64_t)n < 0xFFFFFFFFull - 1 && h[idx].sw > (float)th) {
+        if (n + 2 > cap) {
+          free(st);
+          free(h);
+          return 0;
+        }
+        for (int i = 0; i < 2; ++i) {
+          h[n + i].leaf = 1;
+          h[n + i].child = 0;
+          h[n + i].sw = h[idx].sw / 2.0f;
+          h[n + i].src = h[idx].src;
+        }
+        h[idx].leaf = 0;
+        h[idx].child = n;
+        h[idx].src = -1;
+        n += 2;
+      }
+      if (!h[idx].leaf) {
+        st[sp++] = h[idx].child;
+        st[sp++] = h[idx].child + 1;
+      }
+    }
+    free(st);
+  }
+  uint32_t nd = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (h[i].leaf) {
+      out[i] = LEAF_BIT | nd;
+      map[nd++] = (uint32_t)h[i].src;
+    } else {
+      out[i] = h[i].child;
+    }
+  }
+  *n_dtrees_out = nd;
+  free(h);
+  return n;
+}
+
+/* STreeT::GetDTreeWrapper :639-648, descent :561-567 (child index clamped) */
+static uint32_t stree_lookup1(const uint32_t* stree, const float* lo, float size, v3 p) {
+  float q[3] = {(p.x - lo[0]) / size, (p.y - lo[1]) / size, (p.z - lo[2]) / size};
+  uint32_t w = stree[0];
+  int axis = 0;
+  while (!(w & LEAF_BIT)) {
+    float pa = q[axis] * 2.0f;
+    int idx = (int)pa;
+    idx = idx < 0 ? 0 : (idx > 1 ? 1 : idx);
+    q[axis] = pa - (float)idx;
+    w = stree[w + (uint32_t)idx];
+    axis = axis == 2 ? 0 : axis + 1;
+  }
+  return w & ~LEAF_BIT;
+}
+
+void orc_stree_lookup(const uint32_t* stree, const float* aabb_min, float size, const float* pos, uint32_t n,
+                      uint32_t* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = stree_lookup1(stree, aabb_min, size, ld3(pos + 3 * i));
+}
+
+static void atomic_addf(float* p, float v) {
+  uint32_t* u = (uint32_t*)p;
+  uint32_t old = __atomic_load_n(u, __ATOMIC_RELAXED), nw;
+  for (;;) {
+    float f;
+    memcpy(&f, &old, 4);
+    f += v;
+    memcpy(&nw, &f, 4);
+    if (__atomic_compare_exchange_n(u, &old, nw, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
+  }
+}
+
+/* DTreeT::Impl<Nearest>::RecordIrradiance :419-427 + DTreeNodeT::Record :113-125 */
+static void dtree_record1(float* b_sw, float* b_sums, const uint16_t* b_child, uint32_t base, uint32_t cell,
+                          float cu, float cv, float irr, float sw) {
+  if (!(isfinite(sw) && sw > 0.0f)) return;
+  atomic_addf(&b_sw[cell], sw);
+  if (!(isfinite(irr) && irr > 0.0f)) return;
+  uint32_t node = 0;
+  int idx = child_idx(&cu, &cv);
+  for (;;) {
+    uint16_t c = b_child[4 * ((size_t)base + node) + idx];
+    if (c == 0) break;
+    node = c;
+    idx = child_idx(&cu, &cv);
+  }
+  atomic_addf(&b_sums[4 * ((size_t)base + node) + idx], irr * sw);
+}
+
+void orc_sdtree_record(const uint32_t* stree, const float* aabb_min, float size, const uint32_t* b_base,
+                       float* b_sw, float* b_sums, const uint16_t* b_child, const float* pos, const float* canon,
+                       const float* irr, const float* sw, uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t cell = stree_lookup1(stree, aabb_min, size, ld3(pos + 3 * i));
+    dtree_record1(b_sw, b_sums, b_child, b_base[cell], cell, canon[2 * i], canon[2 * i + 1], irr[i], sw[i]);
+  }
+}
+
+static float dtree_mean(float sum, float sw) { /* DTreeT::GetMean :444-448 */
+  if (sw <= 0.0f) return 0.0f;
+  float factor = INV_2PI_F / (2.0f * sw);
+  return factor * sum;
+}
+
+/* DTreeNodeT::Sample :228-277 + clamp of DTreeT::Sample :477-482 */
+static void dtree_sample1(const float* sums, const uint16_t* child, float sum, float sw, pcg32* r, float* ox,
+                          float* oy) {
+  if (dtree_mean(sum, sw) <= 0.0f) {
+    *ox = rnd(r);
+    *oy = rnd(r);
+    return;
+  }
+  uint32_t node = 0;
+  float rx = 0.0f, ry = 0.0f, size = 1.0f;
+  for (;;) {
+    const float* s = sums + 4 * node;
+    int idx = 0;
+    float partial = s[0] + s[2];
+    float total = s[0] + s[1] + s[2] + s[3];
+    float boundary = partial / total;
+    float ox0 = 0.0f, oy0 = 0.0f;
+    float sample = rnd(r);
+    if (sample < boundary) {
+      sample = sample / boundary;
+      boundary = s[0] / partial;
+    } else {
+      partial = total - partial;
+      ox0 = 0.5f;
+      sample = (sample - boundary) / (1.0f - boundary);
+      boundary = s[1] / partial;
+      idx |= 1;
+    }
+    if (sample >= boundary) {
+      oy0 = 0.5f;
+      idx |= 2;
+    }
+    rx = rx + size * ox0;
+    ry = ry + size * oy0;
+    size *= 0.5f;
+    uint16_t c = child[4 * node + idx];
+    if (c == 0 || s[idx] <= 0.0f) {
+      float a = rnd(r);
+      float b = rnd(r);
+      rx = rx + size * a;
+      ry = ry + size * b;
+      break;
+    }
+    node = c;
+  }
+  *ox = fminf(fmaxf(rx, 0.0f), 1.0f);
+  *oy = fminf(fmaxf(ry, 0.0f), 1.0f);
+}
+
+/* DTreeNodeT::Pdf :278-316 times 1/(4 pi) (DTreeT::Pdf :484-489) */
+static float dtree_pdf1(const float* sums, const uint16_t* child, float sum, float sw, float px, float py) {
+  if (dtree_mean(sum, sw) <= 0.0f) return 1.0f * INV_2PI_F / 2.0f;
+  float result = 1.0f;
+  uint32_t node = 0;
+  int idx = child_idx(&px, &py);
+  for (;;) {
+    const float* s = sums + 4 * node;
+    float total = s[0] + s[1] + s[2] + s[3];
+    if (s[idx] <= 0.0f) {
+      result = 0.0f;
+      break;
+    }
+    result *= 4.0f * s[idx] / total;
+    uint16_t c = child[4 * node + idx];
+    if (c == 0) break;
+    node = c;
+    idx = child_idx(&px, &py);
+  }
+  return result * INV_2PI_F / 2.0f;
+}
+
+void orc_sdtree_sample(const uint32_t* s_base, const float* s_sum, const float* s_sw, const float* s_sums,
+                       const uint16_t* s_child, const uint32_t* cells, uint32_t n, uint64_t seed, float* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    pcg32 r = pcg32_seed(seed, i);
+    uint32_t c = cells[i];
+    dtree_sample1(s_sums + 4 * (size_t)s_base[c], s_child + 4 * (size_t)s_base[c], s_sum[c], s_sw[c], &r,
+                  &out[2 * i], &out[2 * i + 1]);
+  }
+}
+
+void orc_sdtree_pdf(const uint32_t* s_base, const float* s_sum, const float* s_sw, const float* s_sums,
+                    const uint16_t* s_child, const uint32_t* cells, const float* canon, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t c = cells[i];
+    out[i] = dtree_pdf1(s_sums + 4 * (size_t)s_base[c], s_child + 4 * (size_t)s_base[c], s_sum[c], s_sw[c],
+                        canon[2 * i], canon[2 * i + 1]);
+  }
+}
+
+/* ======================================================================
+ * geometry: Moller-Trumbore, same operation order as the HIP kernel
+ * ==================================================================== */
+int orc_isect_tri(const float* o, const float* d, const float* a, const float* e1, const float* e2, float tmin,
+                  float* tbest, float* ou, float* ov) {
+  float px = d[1] * e2[2] - d[2] * e2[1];
+  float py = d[2] * e2[0] - d[0] * e2[2];
+  float pz = d[0] * e2[1] - d[1] * e2[0];
+  float det = e1[0] * px + e1[1] * py + e1[2] * pz;
+  if (det == 0.0f) return 0;
+  float inv = 1.0f / det;
+  float sx = o[0] - a[0], sy = o[1] - a[1], sz = o[2] - a[2];
+  float u = (sx * px + sy * py + sz * pz) * inv;
+  if (!(u >= 0.0f && u <= 1.0f)) return 0;
+  float qx = sy * e1[2] - sz * e1[1];
+  float qy = sz * e1[0] - sx * e1[2];
+  float qz = sx * e1[1] - sy * e1[0];
+  float v = (d[0] * qx + d[1] * qy + d[2] * qz) * inv;
+  if (!(v >= 0.0f && u + v <= 1.0f)) return 0;
+  float t = (e2[0] * qx + e2[1] * qy + e2[2] * qz) * inv;
+  if (!(t > tmin && t < *tbest)) return 0;
+  *tbest = t;
+  *ou = u;
+  *ov = v;
+  return 1;
+}
+
+void orc_brute_closest(const float* verts, const uint32_t* idx, uint32_t nt, const float* rays, uint32_t n,
+                       float* hits) {
+  for (uint32_t r = 0; r < n; ++r) {
+    const float* ray = rays + 8 * r;
+    float tb = ray[7], u = 0, v = 0;
+    uint32_t best = INVALID;
+    for (uint32_t t = 0; t < nt; ++t) {
+      const float* a = verts + 3 * idx[3 * t];
+      const float* b = verts + 3 * idx[3 * t + 1];
+      const float* c = verts + 3 * idx[3 * t + 2];
+      float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      if (orc_isect_tri(ray, ray + 4, a, e1, e2, ray[3], &tb, &u, &v)) best = t;
+    }
+    hits[4 * r] = best == INVALID ? INFINITY : tb;
+    hits[4 * r + 1] = u;
+    hits[4 * r + 2] = v;
+    memcpy(&hits[4 * r + 3], &best, 4);
+  }
+}
+
+/* ======================================================================
+ * scene + binary BVH (oracle-owned acceleration, binned SAH)
+ * ==================================================================== */
+typedef struct {
+  float lo[3], hi[3];
+  int32_t left;   /* >= 0 inner (right = left + 1); < 0 leaf */
+  uint32_t first, count;
+} bnode;
+
+struct orc_scene {
+  uint32_t nt;
+  float* tri;      /* 9 floats per triangle: v0, e1, e2 (original order) */
+  float* albedo;   /* 4 per tri */
+  float* spec;     /* 4 per tri or NULL */
+  uint32_t* mat;
+  int32_t* light_of;
+  uint32_t nl;
+  float* light;    /* per light: v0(3) e1(3) e2(3) area, pdf, Le(3) = 15 floats */
+  float* cdf;
+  bnode* nodes;
+  uint32_t nn;
+  uint32_t* prim;  /* leaf order -> triangle */
+};
+
+typedef struct {
+  float lo[3], hi[3], c[3];
+} pref;
+
+static float box_area(const float* lo, const float* hi) {
+  float d0 = fmaxf(hi[0] - lo[0], 0.0f), d1 = fmaxf(hi[1] - lo[1], 0.0f), d2 = fmaxf(hi[2] - lo[2], 0.0f);
+  return 2.0f * (d0 * d1 + d1 * d2 + d2 * d0);
+}
+
+static void bvh_build(orc_scene* s) {
+  uint32_t nt = s->nt;
+  pref* r = (pref*)malloc(sizeof(pref) * nt);
+  s->prim = (uint32_t*)malloc(sizeof(uint32_t) * nt);
+  for (uint32_t t = 0; t < nt; ++t) {
+    const float* q = s->tri + 9 * t;
+    for (int k = 0; k < 3; ++k) {
+      float a = q[k], b = q[k] + q[3 + k], c = q[k] + q[6 + k];
+      r[t].lo[k] = fminf(a, fminf(b, c));
+      r[t].hi[k] = fmaxf(a, fmaxf(b, c));
+      r[t].c[k] = 0.5f * (r[t].lo[k] + r[t].hi[k]);
+    }
+    s->prim[t] = t;
+  }
+  size_t cap = 2 * (size_t)nt + 1;
+  s->nodes = (bnode*)malloc(sizeof(bnode) * cap);
+  s->nn = 1;
+  s->nodes[0].first = 0;
+  s->nodes[0].count = nt;
+  uint32_t* st = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+  size_t sp = 0;
+  st[sp++] = 0;
+  enum { NB = 16 };
+  while (sp) {
+    uint32_t ni = st[--sp];
+    bnode* n = &s->nodes[ni];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = n->first; i < n->first + n->count; ++i) {
+      pref* p = &r[s->prim[i]];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(lo[k], p->lo[k]);
+        hi[k] = fmaxf(hi[k], p->hi[k]);
+        clo[k] = fminf(clo[k], p->c[k]);
+        chi[k] = fmaxf(chi[k], p->c[k]);
+      }
+    }
+    memcpy(n->lo, lo, sizeof lo);
+    memcpy(n->hi, hi, sizeof hi);
+    n->left = -1;
+    if (n->count <= 4) continue;
+    int best_axis = -1, best_bin = 0;
+    float best = INFINITY;
+    for (int ax = 0; ax < 3; ++ax) {
+      float ext = chi[ax] - clo[ax];
+      if (!(ext > 0.0f)) continue;
+      float blo[NB][3], bhi[NB][3];
+      uint32_t bc[NB] = {0};
+      for (int b = 0; b < NB; ++b)
+        for (int k = 0; k < 3; ++k) {
+          blo[b][k] = INFINITY;
+          bhi[b][k] = -INFINITY;
+        }
+      for (uint32_t i = n->first; i < n->first + n->count; ++i) {
+        pref* p = &r[s->prim[i]];
+        int b = (int)((p->c[ax] - clo[ax]) / ext * NB);
+        if (b >= NB) b = NB - 1;
+        bc[b]++;
+        for (int k = 0; k < 3; ++k) {
+          blo[b][k] = fminf(blo[b][k], p->lo[k]);
+          bhi[b][k] = fmaxf(bhi[b][k], p->hi[k]);
+        }
+      }
+      for (int sb = 0; sb < NB - 1; ++sb) {
+        float l0[3] = {INFINITY, INFINITY, INFINITY}, l1[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float r0[3] = {INFINITY, INFINITY, INFINITY}, r1[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t nl = 0, nr = 0;
+        for (int b = 0; b < NB; ++b) {
+          if (!bc[b]) continue;
+          float* x0 = b <= sb ? l0 : r0;
+          float* x1 = b <= sb ? l1 : r1;
+          for (int k = 0; k < 3; ++k) {
+            x0[k] = fminf(x0[k], blo[b][k]);
+            x1[k] = fmaxf(x1[k], bhi[b][k]);
+          }
+          if (b <= sb) nl += bc[b]; else nr += bc[b];
+        }
+        if (!nl || !nr) continue;
+        float cost = box_area(l0, l1) * nl + box_area(r0, r1) * nr;
+        if (cost < best) {
+          best = cost;
+          best_axis = ax;
+          best_bin = sb;
+        }
+      }
+    }
+    uint32_t mid;
+    if (best_axis < 0) {
+      mid = n->first + n->count / 2;
+    } else {
+      float ext = chi[best_axis] - clo[best_axis];
+      uint32_t i = n->first, j = n->first + n->count;
+      while (i < j) {
+        int b = (int)((r[s->prim[i]].c[best_axis] - clo[best_axis]) / ext * NB);
+        if (b >= NB) b = NB - 1;
+        if (b <= best_bin) {
+          ++i;
+        } else {
+          uint32_t tmp = s->prim[i];
+          s->prim[i] = s->prim[--j];
+          s->prim[j] = tmp;
+        }
+      }
+      mid = i;
+      if (mid == n->first || mid == n->first + n->count) mid = n->first + n->count / 2;
+    }
+    uint32_t l = s->nn;
+    s->nn += 2;
+    bnode* L = &s->nodes[l];
+    bnode* R = &s->nodes[l + 1];
+    n = &s->nodes[ni];
+    L->first = n->first;
+    L->count = mid - n->first;
+    R->first = mid;
+    R->count = n->first + n->count - mid;
+    n->left = (int32_t)l;
+    st[sp++] = l + 1;
+    st[sp++] = l;
+  }
+  free(st);
+  free(r);
+}
+
+orc_scene* orc_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* albedo,
+                            const uint32_t* mat, const float* emission, const float* spec) {
+  (void)nv;
+  orc_scene* s = (orc_scene*)calloc(1, sizeof(orc_scene));
+  s->nt = nt;
+  s->tri = (float*)malloc(sizeof(float) * 9 * nt);
+  s->albedo = (float*)malloc(sizeof(float) * 4 * nt);
+  memcpy(s->albedo, albedo, sizeof(float) * 4 * nt);
+  if (spec) {
+    s->spec = (float*)malloc(sizeof(float) * 4 * nt);
+    memcpy(s->spec, spec, sizeof(float) * 4 * nt);
+  }
+  s->mat = (uint32_t*)malloc(sizeof(uint32_t) * nt);
+  memcpy(s->mat, mat, sizeof(uint32_t) * nt);
+  s->light_of = (int32_t*)malloc(sizeof(int32_t) * nt);
+  for (uint32_t t = 0; t < nt; ++t) {
+    const float* a = verts + 3 * idx[3 * t];
+    const float* b = verts + 3 * idx[3 * t + 1];
+    const float* c = verts + 3 * idx[3 * t + 2];
+    float* q = s->tri + 9 * t;
+    for (int k = 0; k < 3; ++k) {
+      q[k] = a[k];
+      q[3 + k] = b[k] - a[k];
+      q[6 + k] = c[k] - a[k];
+    }
+    s->light_of[t] = -1;
+  }
+  /* emissive triangles with positive area, in triangle order, picked uniformly */
+  uint32_t nl = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const float* e = emission + 3 * t;
+    if (e[0] + e[1] + e[2] > 0.0f) {
+      const float* q = s->tri + 9 * t;
+      float area = 0.5f * len(cross(ld3(q + 3), ld3(q + 6)));
+      if (area > 0.0f) s->light_of[t] = (int32_t)nl++;
+    }
+  }
+  s->nl = nl;
+  s->light = (float*)malloc(sizeof(float) * 15 * (nl ? nl : 1));
+  s->cdf = (float*)malloc(sizeof(float) * (nl ? nl : 1));
+  double acc = 0.0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    int32_t k = s->light_of[t];
+    if (k < 0) continue;
+    const float* q = s->tri + 9 * t;
+    float* L = s->light + 15 * k;
+    memcpy(L, q, 9 * sizeof(float));
+    L[9] = 0.5f * len(cross(ld3(q + 3), ld3(q + 6)));
+    L[10] = 1.0f / (float)nl;
+    L[11] = emission[3 * t];
+    L[12] = emission[3 * t + 1];
+    L[13] = emission[3 * t + 2];
+    acc += L[10];
+    s->cdf[k] = (float)acc;
+  }
+  if (nl) s->cdf[nl - 1] = 1.0f;
+  bvh_build(s);
+  return s;
+}
+
+void orc_scene_destroy(orc_scene* s) {
+  if (!s) return;
+  free(s->tri);
+  free(s->albedo);
+  free(s->spec);
+  free(s->mat);
+  free(s->light_of);
+  free(s->light);
+  free(s->cdf);
+  free(s->nodes);
+  free(s->prim);
+  free(s);
+}
+
+static int box_hit(const bnode* n, v3 o, v3 inv, float tmin, float tmax) {
+  float t0 = tmin, t1 = tmax;
+  float ov[3] = {o.x, o.y, o.z}, iv[3] = {inv.x, inv.y, inv.z};
+  for (int k = 0; k < 3; ++k) {
+    float a = (n->lo[k] - ov[k]) * iv[k], b = (n->hi[k] - ov[k]) * iv[k];
+    if (a > b) {
+      float tt = a;
+      a = b;
+      b = tt;
+    }
+    t0 = fmaxf(t0, a);
+    t1 = fminf(t1, b * 1.00000024f);
+  }
+  return t0 <= t1;
+}
+
+/* closest (any = 0) or any hit (any = 1); returns triangle or -1 */
+static int64_t trace(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int any, float* t_out, uint64_t* rays) {
+  (*rays)++;
+  const float eps = 1e-20f;
+  v3 dd = V(fabsf(d.x) > eps ? d.x : copysignf(eps, d.x), fabsf(d.y) > eps ? d.y : copysignf(eps, d.y),
+            fabsf(d.z) > eps ? d.z : copysignf(eps, d.z));
+  v3 inv = V(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  float tb = tmax, u, v;
+  int64_t best = -1;
+  uint32_t st[128];
+  int sp = 0;
+  st[sp++] = 0;
+  float ov[3] = {o.x, o.y, o.z}, dv[3] = {d.x, d.y, d.z};
+  while (sp) {
+    const bnode* n = &s->nodes[st[--sp]];
+    if (!box_hit(n, o, inv, tmin, tb)) continue;
+    if (n->left < 0) {
+      for (uint32_t i = n->first; i < n->first + n->count; ++i) {
+        uint32_t t = s->prim[i];
+        const float* q = s->tri + 9 * t;
+        if (orc_isect_tri(ov, dv, q, q + 3, q + 6, tmin, &tb, &u, &v)) {
+          best = t;
+          if (any) {
+            *t_out = tb;
+            return best;
+          }
+        }
+      }
+    } else if (sp + 2 <= 128) {
+      st[sp++] = (uint32_t)n->left + 1;
+      st[sp++] = (uint32_t)n->left;
+    }
+  }
+  *t_out = tb;
+  return best;
+}
+
+/* ======================================================================
+ * integrator — the same estimator, random-number order and guide logic as the
+ * wavefront kernels (NEE with one light sample or RIS, MIS by the power
+ * heuristic, Russian roulette, SD-tree guided mixture and record).
+ * ==================================================================== */
+enum { F_NEE = 1, F_RIS = 2, F_USE_TREE = 8, F_GUIDE = 16, F_FINAL = 32, F_MIS = 128, F_RR = 256 };
+enum { M_LAMBERT = 0, M_GGX = 1, M_PHONG = 2, M_GLASS = 3 };
+
+static float ggx_d(float c, float a2) {
+  float c2 = c * c;
+  float den = c2 * (a2 - 1.0f) + 1.0f;
+  return a2 / (PI_F * den * den);
+}
+static float ggx_g1(float cv, float a2) {
+  float c = fabsf(cv);
+  float c2 = c * c;
+  return 2.0f * c / (c + sqrtf(a2 + (1.0f - a2) * c2));
+}
+static v3 schlick3(v3 f0, float c) {
+  float m = 1.0f - fminf(fmaxf(c, 0.0f), 1.0f);
+  float m5 = m * m * m * m * m;
+  return add(f0, scl(sub(V(1, 1, 1), f0), m5));
+}
+static float phong_lobe(v3 dir, v3 refl, float shin) {
+  float rc = fmaxf(dot(refl, dir), 0.0f);
+  float p = (shin != 0.0f) ? powf(rc, shin) : (rc == 0.0f ? 1.0f : 0.0f);
+  return (shin + 1.0f) * p * INV_2PI_F;
+}
+
+typedef struct {
+  v3 f;
+  float pdf;
+} beval;
+
+static beval bsdf_eval(uint32_t type, v3 albedo, float param, v3 spec, v3 n, v3 wo, v3 wi) {
+  beval e;
+  float ci = dot(n, wi);
+  if (type == M_GGX) {
+    float co = dot(n, wo);
+    if (ci <= 0.0f || co <= 0.0f) {
+      e.f = V(0, 0, 0);
+      e.pdf = 0.0f;
+      return e;
+    }
+    float a = fmaxf(param, 1e-3f), a2 = a * a;
+    v3 h = nrm(add(wi, wo));
+    float ch = dot(n, h);
+    float D = ggx_d(ch, a2);
+    float G = ggx_g1(ci, a2) * ggx_g1(co, a2);
+    float oh = fmaxf(dot(wo, h), 1e-8f);
+    v3 fr = schlick3(albedo, oh);
+    e.f = scl(fr, D * G / (4.0f * ci * co));
+    e.pdf = D * ch / (4.0f * oh);
+    return e;
+  }
+  if (type == M_PHONG) {
+    v3 refl = nrm(add(scl(wo, -1.0f), scl(n, 2.0f * dot(n, wo))));
+    float ad = 1.0e-10f + avg3(albedo), as = 1.0e-10f + avg3(spec);
+    float sel = ad / (ad + as);
+    float lobe = phong_lobe(wi, refl, param);
+    e.f = add(scl(albedo, INV_PI_F), scl(spec, ((param + 2.0f) / (param + 1.0f)) * lobe));
+    e.pdf = sel * fmaxf(ci * INV_PI_F, 0.0f) + (1.0f - sel) * lobe;
+    if (ci <= 0.0f) e.f = V(0, 0, 0);
+    return e;
+  }
+  if (ci <= 0.0f) {
+    e.f = V(0, 0, 0);
+    e.pdf = 0.0f;
+    return e;
+  }
+  e.f = scl(albedo, INV_PI_F);
+  e.pdf = ci * INV_PI_F;
+  return e;
+}
+
+static v3 cosine_local(float u1, float u2) {
+  float r = sqrtf(u1), z = sqrtf(1.0f - u1), t = 2.0f * u2 * PI_F;
+  return V(r * cosf(t), r * sinf(t), z);
+}
+
+static v3 bsdf_sample(uint32_t type, v3 albedo, float param, v3 spec, v3 n, v3 wo, pcg32* r) {
+  if (type == M_GGX) {
+    float u1 = rnd(r), u2 = rnd(r);
+    float a = fmaxf(param, 1e-3f), a2 = a * a;
+    float ch = sqrtf((1.0f - u1) / (1.0f + (a2 - 1.0f) * u1));
+    float sh = sqrtf(fmaxf(1.0f - ch * ch, 0.0f));
+    float phi = TWO_PI_F * u2;
+    onb o = make_onb(n);
+    v3 h = onb_local(&o, V(sh * cosf(phi), sh * sinf(phi), ch));
+    return sub(scl(h, 2.0f * dot(wo, h)), wo);
+  }
+  if (type == M_PHONG) {
+    float us = rnd(r), u1 = rnd(r), u2 = rnd(r);
+    float ad = 1.0e-10f + avg3(albedo), as = 1.0e-10f + avg3(spec);
+    float sel = ad / (ad + as);
+    if (us < sel) {
+      onb o = make_onb(n);
+      return onb_local(&o, cosine_local(u1, u2));
+    }
+    v3 refl = nrm(add(scl(wo, -1.0f), scl(n, 2.0f * dot(n, wo))));
+    float ct = powf(u1, 1.0f / (param + 1.0f));
+    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+    float phi = TWO_PI_F * u2;
+    onb o = make_onb(refl);
+    return onb_local(&o, V(st * cosf(phi), st * sinf(phi), ct));
+  }
+  float u1 = rnd(r), u2 = rnd(r);
+  onb o = make_onb(n);
+  return onb_local(&o, cosine_local(u1, u2));
+}
+
+static float power_h(float a, float b) {
+  float a2 = a * a, b2 = b * b;
+  return (a2 + b2) > 0.0f ? a2 / (a2 + b2) : 0.0f;
+}
+
+static uint32_t pick_light(const orc_scene* s, float u) {
+  uint32_t lo = 0, hi = s->nl - 1u;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (s->cdf[mid] > u) hi = mid; else lo = mid + 1u;
+  }
+  return lo;
+}
+
+typedef struct {
+  v3 p, n, le;
+  float area, pdf;
+} lsample;
+
+static lsample sample_light(const orc_scene* s, pcg32* r) {
+  float ul = rnd(r), u1 = rnd(r), u2 = rnd(r);
+  const float* L = s->light + 15 * pick_light(s, ul);
+  float lo = fminf(u1, u2), hi = fmaxf(u1, u2);
+  lsample ls;
+  v3 v0 = ld3(L), e1 = ld3(L + 3), e2 = ld3(L + 6);
+  ls.p = add(add(v0, scl(e1, 1.0f - hi)), scl(e2, hi - lo));
+  ls.n = nrm(cross(e1, e2));
+  ls.le = ld3(L + 11);
+  ls.area = L[9];
+  ls.pdf = L[10];
+  return ls;
+}
+
+typedef struct {
+  v3 T;       /* throughput after the bounce */
+  float wo_pdf, cosv, cu, cv;
+  uint32_t cell, flags;
+} vtx;
+
+enum { VTX_VALID = 1, VTX_DELTA = 2, VTX_BAD_BSDF = 4 };
+
+static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_sdtree* g, uint32_t pix, uint32_t smp,
+                            v3* Lout) {
+  uint64_t rays = 0;
+  pcg32 r = path_rng(p->seed, pix, p->sample_offset + smp);
+  uint32_t x = pix % p->width, y = pix / p->width;
+  float jx = rnd(&r), jy = rnd(&r);
+  float dx = 2.0f * ((float)x + jx) / (float)p->width - 1.0f;
+  float dy = 2.0f * ((float)y + jy) / (float)p->height - 1.0f;
+  v3 o = ld3(p->eye);
+  v3 d = nrm(add(add(scl(ld3(p->u), dx), scl(ld3(p->v), dy)), ld3(p->w)));
+  v3 T = V(1, 1, 1), L = V(0, 0, 0);
+  float prev_pdf = 0.0f;
+  int prev_delta = 1;
+  const int nee = (p->flags & F_NEE) && s->nl > 0;
+  const int ris = nee && (p->flags & F_RIS);
+  const int mis = nee && !ris && (p->flags & F_MIS);
+  const int use_tree = g != NULL && (p->flags & F_USE_TREE);
+  const int record = use_tree && !(p->flags & F_FINAL);
+  const float alpha = p->fraction;
+  v3 cbuf[64];
+  vtx vb[64];
+  uint32_t nh = 0;
+  for (uint32_t depth = 0; depth < p->max_depth; ++depth) {
+    float t;
+    int64_t tri = trace(s, o, d, 1e-4f, 1e20f, 0, &t, &rays);
+    v3 c = V(0, 0, 0);
+    vtx vx;
+    memset(&vx, 0, sizeof vx);
+    vx.cell = INVALID;
+    int cont = 0;
+    v3 Tn = V(0, 0, 0), nd = V(0, 0, 0), pos = V(0, 0, 0);
+    float wo_pdf = 0.0f;
+    int next_delta = 0;
+    if (tri < 0) {
+      c = mul(T, ld3(p->bg));
+    } else {
+      const float* q = s->tri + 9 * tri;
+      v3 ng = nrm(cross(ld3(q + 3), ld3(q + 6)));
+      float dn = dot(d, ng);
+      int front = dn < 0.0f;
+      v3 n = front ? ng : neg(ng);
+      pos = add(o, scl(d, t));
+      v3 wo = neg(d);
+      if (s->light_of[tri] >= 0) {
+        if (front) {
+          const float* Lr = s->light + 15 * s->light_of[tri];
+          float w = 1.0f;
+          if (depth > 0 && !prev_delta && nee) {
+            if (mis) {
+              float pl = Lr[10] / Lr[9] * t * t / fabsf(dn);
+              w = power_h(prev_pdf, pl);
+            } else {
+              w = 0.0f;
+            }
+          }
+          c = scl(mul(T, ld3(Lr + 11)), w);
+        }
+      } else if (depth + 1u < p->max_depth) {
+        uint32_t type = s->mat[tri];
+        v3 albedo = ld3(s->albedo + 4 * tri);
+        float param = s->albedo[4 * tri + 3];
+        v3 spec = V(0, 0, 0);
+        if (type == M_PHONG && s->spec) {
+          spec = ld3(s->spec + 4 * tri);
+          param = s->spec[4 * tri + 3];
+        }
+        if (type == M_GLASS) {
+          float ior = param, ci = dot(ng, d), eta;
+          v3 rn;
+          if (ci < 0.0f) {
+            rn = ng;
+            eta = 1.0f / ior;
+            ci = -ci;
+          } else {
+            rn = neg(ng);
+            eta = ior;
+          }
+          float so2 = (1.0f - ci * ci) * eta * eta;
+          float f0 = (1.0f - eta) / (1.0f + eta);
+          f0 = f0 * f0;
+          float m = 1.0f - ci;
+          float fres = f0 + (1.0f - f0) * (m * m * m * m * m);
+          v3 refl = nrm(sub(d, scl(rn, 2.0f * dot(d, rn))));
+          if (rnd(&r) < fres || so2 > 1.0f) {
+            pos = add(pos, scl(rn, 1e-4f));
+            nd = refl;
+            Tn = mul(T, albedo);
+          } else {
+            pos = sub(pos, scl(rn, 1e-4f));
+            float si2 = fmaxf(1.0f - ci * ci, 0.0f);
+            float co = sqrtf(1.0f - so2);
+            if (si2 > 0.0f) {
+              v3 k = divs(add(d, scl(rn, ci)), sqrtf(si2));
+              nd = nrm(sub(scl(k, sqrtf(so2)), scl(rn, co)));
+            } else {
+              nd = d;
+            }
+            Tn = T;
+          }
+          wo_pdf = fabsf(dot(nd, rn));
+          vx.cosv = wo_pdf;
+          vx.flags = VTX_DELTA;
+          next_delta = 1;
+          cont = max3(Tn) > 0.0f;
+        } else {
+          uint32_t cell = INVALID;
+          if (use_tree) cell = stree_lookup1(g->stree, g->aabb_min, g->size, pos);
+          const int guided = use_tree && (p->flags & F_GUIDE) && cell != INVALID;
+          const float* gs = guided ? g->s_sums + 4 * (size_t)g->s_base[cell] : NULL;
+          const uint16_t* gc = guided ? g->s_child + 4 * (size_t)g->s_base[cell] : NULL;
+          if (nee) {
+            if (ris) {
+              float wsum = 0.0f, fa_y = 0.0f, dist_y = 0.0f;
+              uint32_t m = 0;
+              v3 f_y = V(0, 0, 0), dir_y = V(0, 0, 0);
+              for (uint32_t k = 0; k < p->num_candidates; ++k) {
+                lsample ls = sample_light(s, &r);
+                v3 dl = sub(ls.p, pos);
+                float dist = len(dl);
+                v3 dir = divs(dl, dist);
+                float ndl = dot(dir, n), lndl = -dot(dir, ls.n);
+                beval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
+                float gg = fmaxf(ndl, 0.0f) * fmaxf(lndl, 0.0f) / (dist * dist);
+                v3 f = scl(mul(be.f, ls.le), gg);
+                float fa = avg3(f);
+                float wi = fa * (ls.area / ls.pdf);
+                wsum += wi;
+                ++m;
+                float ur = rnd(&r);
+                if ((wi / wsum) >= ur) {
+                  f_y = f;
+                  fa_y = fa;
+                  dir_y = dir;
+                  dist_y = dist;
+                }
+              }
+              if (wsum > 0.0f && fa_y > 0.0f) {
+                float W = wsum / (fa_y * (float)m);
+                v3 con = scl(mul(T, f_y), W);
+                float tt;
+                if (max3(con) > 0.0f && trace(s, pos, dir_y, 1e-4f, dist_y - 1e-4f, 1, &tt, &rays) < 0) c = add(c, con);
+              }
+            } else {
+              lsample ls = sample_light(s, &r);
+              v3 dl = sub(ls.p, pos);
+              float dist2 = dot(dl, dl), dist = sqrtf(dist2);
+              v3 dir = divs(dl, dist);
+              float cl = -dot(dir, ls.n), cs = dot(dir, n);
+              if (cl > 0.0f && cs > 0.0f) {
+                beval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
+                float pl = ls.pdf / ls.area * dist2 / cl;
+                float w = 1.0f;
+                if (mis) {
+                  float bp = be.pdf;
+                  if (guided) {
+                    float cu, cv;
+                    dir_to_canonical(dir, &cu, &cv);
+                    float gp = dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], cu, cv);
+                    bp = (1.0f - alpha) * bp + alpha * gp;
+                  }
+                  w = power_h(pl, bp);
+                }
+                v3 con = scl(mul(mul(T, be.f), ls.le), cs * w / pl);
+                float tt;
+                if (max3(con) > 0.0f && trace(s, pos, dir, 1e-4f, dist - 1e-4f, 1, &tt, &rays) < 0) c = add(c, con);
+              }
+            }
+          }
+          v3 dir;
+          if (guided) {
+            float us = rnd(&r);
+            if (us < alpha) {
+              float gu, gv;
+              dtree_sample1(gs, gc, g->s_sum[cell], g->s_sw[cell], &r, &gu, &gv);
+              dir = nrm(canonical_to_dir(gu, gv));
+            } else {
+              dir = bsdf_sample(type, albedo, param, spec, n, wo, &r);
+            }
+          } else {
+            dir = bsdf_sample(type, albedo, param, spec, n, wo, &r);
+          }
+          beval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
+          wo_pdf = be.pdf;
+          if (guided) {
+            dir_to_canonical(dir, &vx.cu, &vx.cv);
+            float gp = dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], vx.cu, vx.cv);
+            wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
+          } else if (record && cell != INVALID) {
+            dir_to_canonical(dir, &vx.cu, &vx.cv);
+          }
+          float cd = dot(dir, n);
+          vx.cosv = cd;
+          Tn = (wo_pdf > 0.0f && cd > 0.0f) ? scl(mul(T, be.f), cd / wo_pdf) : V(0, 0, 0);
+          if ((p->flags & F_RR) && depth >= p->rr_depth) {
+            float qq = fminf(max3(Tn), 0.95f);
+            if (rnd(&r) >= qq) {
+              Tn = V(0, 0, 0);
+            } else {
+              Tn = divs(Tn, qq);
+            }
+          }
+          int ok = finite3(be.f) && be.f.x >= 0.0f && be.f.y >= 0.0f && be.f.z >= 0.0f;
+          vx.flags = ok ? 0u : VTX_BAD_BSDF;
+          vx.cell = cell;
+          nd = dir;
+          cont = max3(Tn) > 0.0f && isfinite(max3(Tn));
+        }
+      }
+    }
+    L = add(L, c);
+    cbuf[nh] = c;
+    vx.T = Tn;
+    vx.wo_pdf = wo_pdf;
+    if (cont && vx.cell != INVALID) vx.flags |= VTX_VALID;
+    vb[nh] = vx;
+    nh++;
+    if (!cont) break;
+    o = pos;
+    d = nd;
+    T = Tn;
+    prev_pdf = wo_pdf;
+    prev_delta = next_delta;
+  }
+  if (record) {
+    /* TraceVertexT::Commit :830-866 over vertices 0 .. nh-2 */
+    for (uint32_t dv = 0; dv + 1 < nh; ++dv) {
+      vtx* v = &vb[dv];
+      if (!(v->flags & VTX_VALID) || (v->flags & VTX_DELTA) || v->cell == INVALID) continue;
+      float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+      for (uint32_t k = dv + 1; k < nh; ++k) {
+        rx += cbuf[k].x;
+        ry += cbuf[k].y;
+        rz += cbuf[k].z;
+      }
+      int ok = isfinite(rx) && rx >= 0.0f && isfinite(ry) && ry >= 0.0f && isfinite(rz) && rz >= 0.0f;
+      if (v->wo_pdf <= 0.0f || !ok || (v->flags & VTX_BAD_BSDF)) continue;
+      float lx = v->T.x * v->wo_pdf > 1e-4f ? rx / v->T.x : 0.0f;
+      float ly = v->T.y * v->wo_pdf > 1e-4f ? ry / v->T.y : 0.0f;
+      float lz = v->T.z * v->wo_pdf > 1e-4f ? rz / v->T.z : 0.0f;
+      float cs = fabsf(v->cosv);
+      lx *= cs;
+      ly *= cs;
+      lz *= cs;
+      float value = ((lx + ly + lz) / 3.0f) / v->wo_pdf;
+      dtree_record1(g->b_sw, g->b_sums, g->b_child, g->b_base[v->cell], v->cell, v->cu, v->cv, value, 1.0f);
+    }
+  }
+  *Lout = L;
+  return rays;
+}
+
+typedef struct {
+  const orc_scene* s;
+  const orc_params* p;
+  const orc_sdtree* g;
+  float* accum;
+  uint32_t next;
+  uint64_t rays;
+  pthread_mutex_t mu;
+} job;
+
+static void* worker(void* arg) {
+  job* j = (job*)arg;
+  uint64_t rays = 0;
+  const uint32_t tile = 256;
+  for (;;) {
+    uint32_t b = __atomic_fetch_add(&j->next, tile, __ATOMIC_RELAXED);
+    uint32_t begin = j->p->px_begin + b;
+    if (begin >= j->p->px_end) break;
+    uint32_t end = begin + tile < j->p->px_end ? begin + tile : j->p->px_end;
+    for (uint32_t pix = begin; pix < end; ++pix) {
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+      for (uint32_t smp = 0; smp < j->p->spp; ++smp) {
+        v3 L;
+        rays += render_path(j->s, j->p, j->g, pix, smp, &L);
+        sx += L.x;
+        sy += L.y;
+        sz += L.z;
+      }
+      j->accum[3 * pix] += sx;
+      j->accum[3 * pix + 1] += sy;
+      j->accum[3 * pix + 2] += sz;
+    }
+  }
+  pthread_mutex_lock(&j->mu);
+  j->rays += rays;
+  pthread_mutex_unlock(&j->mu);
+  return NULL;
+}
+
+uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum) {
+  job j;
+  j.s = s;
+  j.p = p;
+  j.g = guide;
+  j.accum = accum;
+  j.next = 0;
+  j.rays = 0;
+  pthread_mutex_init(&j.mu, NULL);
+  uint32_t nth = p->threads ? p->threads : 1;
+  if (nth > 256) nth = 256;
+  pthread_t th[256];
+  for (uint32_t i = 0; i < nth; ++i) pthread_create(&th[i], NULL, worker, &j);
+  for (uint32_t i = 0; i < nth; ++i) pthread_join(th[i], NULL);
+  pthread_mutex_destroy(&j.mu);
+  return j.rays;
+}

@@ -1,0 +1,433 @@
+// C-ABI: context, scene upload, camera/frame, render and ray queries.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "../../include/rtpg.h"
+#include "context.h"
+
+namespace rtpg {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& s) { g_last_error = s; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw Error(RTPG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return RTPG_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return RTPG_ERR_INVALID;
+  } catch (...) {
+    set_last_error("unknown error");
+    return RTPG_ERR_INVALID;
+  }
+}
+
+static void require(bool c, const char* msg) {
+  if (!c) throw Error(RTPG_ERR_INVALID, msg);
+}
+
+}  // namespace rtpg
+
+using namespace rtpg;
+
+SceneView rtpg_context_t::scene_view() const {
+  SceneView v{};
+  v.nodes = nodes.p;
+  v.tris = tris.p;
+  v.shade = shade.p;
+  v.spec = has_spec ? spec.p : nullptr;
+  v.info = info.p;
+  v.lights = lights.p;
+  v.light_cdf = light_cdf.p;
+  v.n_lights = n_lights;
+  v.n_tris = n_tris;
+  v.n_nodes = n_nodes;
+  v.max_stack = max_stack;
+  return v;
+}
+
+extern "C" {
+
+const char* rtpg_last_error(void) { return g_last_error.c_str(); }
+const char* rtpg_version(void) { return "rtpg 0.1 (gfx950)"; }
+
+int rtpg_context_create(int device, rtpg_context* out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    int count = 0;
+    RTPG_HIP(hipGetDeviceCount(&count));
+    require(device >= 0 && device < count, "device index out of range");
+    RTPG_HIP(hipSetDevice(device));
+    auto* c = new rtpg_context_t();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      hip_check(e, "hipStreamCreateWithFlags");
+    }
+    c->stream = c->own_stream;
+    c->p_err.alloc(4);
+    c->p_err.zero(4, c->stream);
+    c->p_stats.alloc(4);
+    *out = c;
+  });
+}
+
+int rtpg_context_destroy(rtpg_context ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    hipStream_t own = ctx->own_stream;
+    delete ctx;
+    if (own) (void)hipStreamDestroy(own);
+  });
+}
+
+int rtpg_context_set_stream(rtpg_context ctx, void* hip_stream) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+  });
+}
+
+int rtpg_synchronize(rtpg_context ctx) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    RTPG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int rtpg_scene_upload(rtpg_context ctx, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* albedo, const uint32_t* mat_type, const float* emission, const float* spec,
+                      uint32_t light_mode, const uint32_t* light_mesh) {
+  return guarded([&] {
+    require(ctx && verts && idx && albedo && mat_type && emission, "NULL argument");
+    require(nt > 0 && nv > 0, "empty scene");
+    require(light_mode <= 1u, "light_mode must be 0 or 1");
+    require(light_mode == 0u || light_mesh != nullptr, "light_mode 1 needs light_mesh");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    BvhBuildResult b = build_bvh4(verts, nv, idx, nt, std::min(threads, 16));
+    if (b.max_stack > 16u + 112u) throw Error(RTPG_ERR_LIMIT, "BVH too deep for the traversal stack");
+    std::vector<float4> shade(nt), specv(nt);
+    std::vector<uint32_t> info(nt);
+    std::vector<uint32_t> slot_of(nt);
+    for (uint32_t s = 0; s < nt; ++s) slot_of[b.order[s]] = s;
+    // Light list in original triangle order.
+    std::vector<uint32_t> light_tris;
+    for (uint32_t t = 0; t < nt; ++t) {
+      const float* e = emission + 3 * t;
+      if (e[0] + e[1] + e[2] > 0.0f) {
+        const TriRec& tr = b.tris[slot_of[t]];
+        V3 e1{tr.e1[0], tr.e1[1], tr.e1[2]}, e2{tr.e2[0], tr.e2[1], tr.e2[2]};
+        float area = 0.5f * length(cross(e1, e2));
+        if (area > 0.0f) light_tris.push_back(t);
+      }
+    }
+    uint32_t nl = (uint32_t)light_tris.size();
+    std::vector<float> pdf(nl, 0.0f);
+    if (light_mode == 0u) {
+      for (uint32_t k = 0; k < nl; ++k) pdf[k] = 1.0f / (float)nl;
+    } else {
+      // Uniform mesh, then uniform triangle within the mesh.
+      std::vector<uint32_t> meshes;
+      for (uint32_t k = 0; k < nl; ++k) meshes.push_back(light_mesh[light_tris[k]]);
+      std::vector<uint32_t> uniq = meshes;
+      std::sort(uniq.begin(), uniq.end());
+      uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+      for (uint32_t k = 0; k < nl; ++k) {
+        uint32_t m = meshes[k];
+        uint32_t cnt = (uint32_t)std::count(meshes.begin(), meshes.end(), m);
+        pdf[k] = 1.0f / ((float)uniq.size() * (float)cnt);
+      }
+    }
+    std::vector<float4> lights(4 * (size_t)std::max(nl, 1u));
+    std::vector<float> cdf(std::max(nl, 1u), 1.0f);
+    double acc = 0.0;
+    for (uint32_t k = 0; k < nl; ++k) {
+      uint32_t t = light_tris[k];
+      uint32_t s = slot_of[t];
+      const TriRec& tr = b.tris[s];
+      V3 e1{tr.e1[0], tr.e1[1], tr.e1[2]}, e2{tr.e2[0], tr.e2[1], tr.e2[2]};
+      float area = 0.5f * length(cross(e1, e2));
+      lights[4 * k + 0] = make_float4(tr.v0[0], tr.v0[1], tr.v0[2], area);
+      lights[4 * k + 1] = make_float4(e1.x, e1.y, e1.z, pdf[k]);
+      lights[4 * k + 2] = make_float4(e2.x, e2.y, e2.z, bitsf(s));
+      lights[4 * k + 3] = make_float4(emission[3 * t], emission[3 * t + 1], emission[3 * t + 2], 0.0f);
+      acc += pdf[k];
+      cdf[k] = (float)acc;
+    }
+    if (nl) cdf[nl - 1] = 1.0f;
+    std::vector<int32_t> light_of(nt, -1);
+    for (uint32_t k = 0; k < nl; ++k) light_of[light_tris[k]] = (int32_t)k;
+    ctx->shade_host.assign(4 * (size_t)nt, 0.0f);
+    for (uint32_t s = 0; s < nt; ++s) {
+      uint32_t t = b.order[s];
+      require(mat_type[t] <= 3u, "unknown material type");
+      shade[s] = make_float4(albedo[4 * t], albedo[4 * t + 1], albedo[4 * t + 2], albedo[4 * t + 3]);
+      for (int k = 0; k < 4; ++k) ctx->shade_host[4 * (size_t)s + k] = albedo[4 * t + k];
+      specv[s] = spec ? make_float4(spec[4 * t], spec[4 * t + 1], spec[4 * t + 2], spec[4 * t + 3])
+                      : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      uint32_t lw = light_of[t] >= 0 ? (uint32_t)light_of[t] + 1u : 0u;
+      info[s] = (mat_type[t] & kTypeMask) | (lw << 4);
+    }
+    hipStream_t s = ctx->stream;
+    ctx->nodes.upload(b.nodes, s);
+    ctx->tris.upload(b.tris, s);
+    ctx->shade.upload(shade, s);
+    ctx->spec.upload(specv, s);
+    ctx->info.upload(info, s);
+    ctx->lights.upload(lights, s);
+    ctx->light_cdf.upload(cdf, s);
+    RTPG_HIP(hipStreamSynchronize(s));
+    ctx->has_spec = spec != nullptr;
+    ctx->order = b.order;
+    ctx->slot_of = slot_of;
+    ctx->n_tris = nt;
+    ctx->n_lights = nl;
+    ctx->n_nodes = (uint32_t)b.nodes.size();
+    ctx->bvh_depth = b.max_depth;
+    ctx->max_stack = b.max_stack;
+    ctx->max_leaf = b.max_leaf;
+    for (int k = 0; k < 3; ++k) {
+      ctx->world[k] = b.world_lo[k];
+      ctx->world[3 + k] = b.world_hi[k];
+    }
+  });
+}
+
+int rtpg_scene_set_albedo(rtpg_context ctx, const float* albedo_rgb) {
+  return guarded([&] {
+    require(ctx && albedo_rgb, "NULL argument");
+    require(ctx->n_tris > 0, "no scene");
+    std::vector<float4> shade(ctx->n_tris);
+    for (uint32_t s = 0; s < ctx->n_tris; ++s) {
+      uint32_t t = ctx->order[s];
+      float* h = &ctx->shade_host[4 * (size_t)s];
+      h[0] = albedo_rgb[3 * t];
+      h[1] = albedo_rgb[3 * t + 1];
+      h[2] = albedo_rgb[3 * t + 2];
+      shade[s] = make_float4(h[0], h[1], h[2], h[3]);
+    }
+    ctx->shade.upload(shade, ctx->stream);
+    RTPG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int rtpg_scene_info(rtpg_context ctx, uint32_t info[6], float aabb[6]) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    if (info) {
+      info[0] = ctx->n_nodes;
+      info[1] = ctx->bvh_depth;
+      info[2] = ctx->max_stack;
+      info[3] = ctx->max_leaf;
+      info[4] = ctx->n_lights;
+      info[5] = ctx->n_tris;
+    }
+    if (aabb) std::memcpy(aabb, ctx->world, sizeof(ctx->world));
+  });
+}
+
+int rtpg_camera_set(rtpg_context ctx, const float eye[3], const float u[3], const float v[3], const float w[3]) {
+  return guarded([&] {
+    require(ctx && eye && u && v && w, "NULL argument");
+    for (int k = 0; k < 3; ++k) {
+      ctx->cam.eye[k] = eye[k];
+      ctx->cam.u[k] = u[k];
+      ctx->cam.v[k] = v[k];
+      ctx->cam.w[k] = w[k];
+    }
+  });
+}
+
+int rtpg_frame_resize(rtpg_context ctx, uint32_t width, uint32_t height) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    require(width > 0 && height > 0 && (uint64_t)width * height < (1ull << 31), "bad frame size");
+    ctx->width = width;
+    ctx->height = height;
+    ctx->accum.alloc(3 * (size_t)width * height);
+    ctx->accum.zero(3 * (size_t)width * height, ctx->stream);
+    ctx->accum_spp = 0;
+  });
+}
+
+int rtpg_accum_clear(rtpg_context ctx) {
+  return guarded([&] {
+    require(ctx && ctx->accum.p, "frame not sized");
+    ctx->accum.zero(3 * (size_t)ctx->width * ctx->height, ctx->stream);
+    ctx->accum_spp = 0;
+  });
+}
+
+int rtpg_accum_download(rtpg_context ctx, float* out, uint32_t* spp) {
+  return guarded([&] {
+    require(ctx && ctx->accum.p, "frame not sized");
+    if (out) ctx->accum.download(out, 3 * (size_t)ctx->width * ctx->height, ctx->stream);
+    if (spp) *spp = ctx->accum_spp;
+  });
+}
+
+}  // extern "C"
+
+namespace rtpg {
+__global__ void k_stats_add(const uint32_t* count, const uint32_t* scount, uint32_t depth, uint32_t n,
+                            unsigned long long* stats);
+hipError_t launch_stats_add(const uint32_t* count, const uint32_t* scount, uint32_t depth, uint32_t n,
+                            unsigned long long* stats, hipStream_t s);
+SdTreeView sdtree_view_of(rtpg_sdtree_t* t);
+void guide_views(const rtpg_render_params* rp, rtpg_context_t* ctx, HashGridView& hg, MortonView& mq);
+}  // namespace rtpg
+
+static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool guide) {
+  c->p_ray_o.reserve(n);
+  c->p_ray_d.reserve(n);
+  c->p_T.reserve(n);
+  c->p_L.reserve(n);
+  c->p_hit.reserve(n);
+  c->p_rng.reserve(n);
+  c->p_nhit.reserve(n);
+  c->p_q0.reserve(n);
+  c->p_q1.reserve(n);
+  c->p_shadow.reserve(3 * (size_t)n);
+  c->p_count.reserve(2 * (size_t)(max_depth + 2));
+  if (c->max_stack > 16u) c->p_deep.reserve((size_t)(c->max_stack - 16u) * n);
+  if (guide) {
+    c->p_vA.reserve((size_t)max_depth * n);
+    c->p_vB.reserve((size_t)max_depth * n);
+    c->p_vC.reserve((size_t)max_depth * n);
+  }
+}
+
+extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
+  return guarded([&] {
+    require(ctx && prm, "NULL argument");
+    require(ctx->n_tris > 0, "no scene uploaded");
+    require(ctx->accum.p != nullptr, "frame not sized");
+    require(prm->max_depth >= 1 && prm->max_depth <= 64, "max_depth must be in [1, 64]");
+    require(!(prm->flags & RTPG_FLAG_RIS) || prm->num_candidates >= 1, "RIS needs num_candidates >= 1");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    const bool use_tree = (prm->flags & RTPG_FLAG_USE_TREE) != 0;
+    const bool use_grid = (prm->flags & RTPG_FLAG_USE_GRID) != 0;
+    require(!(use_tree && use_grid), "USE_TREE and USE_GRID are exclusive");
+    require(!use_tree || prm->sdtree != nullptr, "USE_TREE needs an sdtree");
+    require(!use_grid || (prm->hashgrid != nullptr && prm->mqtree != nullptr), "USE_GRID needs hashgrid + mqtree");
+    int guide = use_tree ? 1 : (use_grid ? 2 : 0);
+    uint32_t npix = ctx->width * ctx->height;
+    uint32_t max_paths = prm->max_paths ? prm->max_paths : (1u << 22);
+    uint32_t chunk_spp = std::max(1u, max_paths / npix);
+    chunk_spp = std::min(chunk_spp, std::max(prm->spp, 1u));
+    SceneView sc = ctx->scene_view();
+    SdTreeView sd{};
+    HashGridView hg{};
+    MortonView mq{};
+    if (guide == 1) sd = sdtree_view_of(prm->sdtree);
+    if (guide == 2) guide_views(prm, ctx, hg, mq);
+    hipStream_t s = ctx->stream;
+    ctx->p_stats.zero(4, s);
+    uint32_t done = 0;
+    while (done < prm->spp) {
+      uint32_t cs = std::min(chunk_spp, prm->spp - done);
+      uint32_t n = cs * npix;
+      ensure_arena(ctx, n, prm->max_depth, guide != 0);
+      PathSoA ps{};
+      ps.n = n;
+      ps.ray_o = ctx->p_ray_o.p;
+      ps.ray_d = ctx->p_ray_d.p;
+      ps.T = ctx->p_T.p;
+      ps.L = ctx->p_L.p;
+      ps.hit = ctx->p_hit.p;
+      ps.rng = ctx->p_rng.p;
+      ps.nhit = ctx->p_nhit.p;
+      ps.queue[0] = ctx->p_q0.p;
+      ps.queue[1] = ctx->p_q1.p;
+      ps.count = ctx->p_count.p;
+      ps.scount = ctx->p_count.p + (prm->max_depth + 2u);
+      ps.shadow = ctx->p_shadow.p;
+      ps.deep = ctx->max_stack > 16u ? ctx->p_deep.p : nullptr;
+      ps.err = ctx->p_err.p;
+      ps.vA = guide ? ctx->p_vA.p : nullptr;
+      ps.vB = guide ? ctx->p_vB.p : nullptr;
+      ps.vC = guide ? ctx->p_vC.p : nullptr;
+      RenderParams rp{};
+      rp.width = ctx->width;
+      rp.height = ctx->height;
+      rp.flags = prm->flags;
+      rp.max_depth = prm->max_depth;
+      rp.rr_depth = prm->rr_depth;
+      rp.num_candidates = prm->num_candidates;
+      rp.spp = cs;
+      rp.sample_offset = prm->sample_offset + done;
+      rp.seed = prm->seed;
+      rp.bg[0] = prm->bg[0];
+      rp.bg[1] = prm->bg[1];
+      rp.bg[2] = prm->bg[2];
+      rp.fraction = prm->fraction;
+      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, guide, ctx->accum.p, s));
+      RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));
+      done += cs;
+    }
+    ctx->accum_spp += prm->spp;
+    unsigned long long st[4];
+    ctx->p_stats.download(st, 4, s);
+    uint32_t err = 0;
+    ctx->p_err.download(&err, 1, s);
+    if (err) throw Error(RTPG_ERR_LIMIT, "traversal or guide stack bound exceeded");
+    ctx->last_stats[0] = st[0];
+    ctx->last_stats[1] = st[1];
+    ctx->last_stats[2] = st[2];
+  });
+}
+
+extern "C" int rtpg_render_stats(rtpg_context ctx, uint64_t stats[3]) {
+  return guarded([&] {
+    require(ctx && stats, "NULL argument");
+    for (int k = 0; k < 3; ++k) stats[k] = ctx->last_stats[k];
+  });
+}
+
+static int trace_common(rtpg_context ctx, const float* rays, uint32_t n, float* hits, uint32_t* occl, bool any) {
+  return guarded([&] {
+    require(ctx && rays && (hits || occl), "NULL argument");
+    require(ctx->n_tris > 0, "no scene uploaded");
+    if (n == 0) return;
+    RTPG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    DBuf<float4> d_rays, d_hits;
+    DBuf<uint32_t> d_occ, d_deep;
+    d_rays.upload(reinterpret_cast<const float4*>(rays), 2 * (size_t)n, s);
+    if (any) d_occ.alloc(n); else d_hits.alloc(n);
+    if (ctx->max_stack > 16u) d_deep.alloc((size_t)(ctx->max_stack - 16u) * n);
+    RTPG_HIP(launch_trace_rays(ctx->scene_view(), d_rays.p, n, d_hits.p, d_occ.p, any,
+                               ctx->max_stack > 16u ? d_deep.p : nullptr, ctx->p_err.p, s));
+    if (any) d_occ.download(occl, n, s); else d_hits.download(reinterpret_cast<float4*>(hits), n, s);
+    uint32_t err = 0;
+    ctx->p_err.download(&err, 1, s);
+    if (err) throw Error(RTPG_ERR_LIMIT, "traversal stack bound exceeded");
+  });
+}
+
+extern "C" int rtpg_trace_closest(rtpg_context ctx, const float* rays, uint32_t n, float* hits) {
+  return trace_common(ctx, rays, n, hits, nullptr, false);
+}
+
+extern "C" int rtpg_trace_occluded(rtpg_context ctx, const float* rays, uint32_t n, uint32_t* occluded) {
+  return trace_common(ctx, rays, n, nullptr, occluded, true);
+}

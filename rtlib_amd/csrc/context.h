@@ -1,0 +1,80 @@
+// Library-side state behind the C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "bvh.h"
+#include "host.h"
+#include "render.h"
+#include "sdtree.h"
+#include "views.h"
+
+struct rtpg_context_t {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  // scene
+  rtpg::DBuf<rtpg::BvhNode4> nodes;
+  rtpg::DBuf<rtpg::TriRec> tris;
+  rtpg::DBuf<float4> shade, spec, lights;
+  rtpg::DBuf<uint32_t> info;
+  rtpg::DBuf<float> light_cdf;
+  std::vector<uint32_t> order;       // leaf slot -> original triangle
+  std::vector<uint32_t> slot_of;     // original triangle -> leaf slot
+  std::vector<float> shade_host;     // leaf order, 4 per triangle
+  uint32_t n_tris = 0, n_lights = 0, n_nodes = 0;
+  uint32_t bvh_depth = 0, max_stack = 0, max_leaf = 0;
+  float world[6] = {0, 0, 0, 0, 0, 0};
+  bool has_spec = false;
+  // frame
+  uint32_t width = 0, height = 0;
+  rtpg::DBuf<float> accum;
+  uint32_t accum_spp = 0;
+  rtpg::Camera cam{};
+  // path arena
+  rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC;
+  rtpg::DBuf<uint2> p_rng;
+  rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err;
+  rtpg::DBuf<unsigned long long> p_stats;
+  uint64_t last_stats[3] = {0, 0, 0};
+
+  rtpg::SceneView scene_view() const;
+};
+
+struct rtpg_sdtree_t {
+  rtpg_context_t* ctx = nullptr;
+  float aabb_min[3] = {0, 0, 0};
+  float size = 1.0f;
+  uint32_t max_dtree_depth = 20;
+  float fraction = 0.3f;
+  int split_mode = 0;
+  float split_c = 4000.0f;
+  int max_mb = 2000;
+  float subdiv_th = 0.01f;
+  // Host mirror of the spatial tree topology: leaf flag, first child / DTree.
+  struct SNode {
+    bool leaf;
+    uint32_t child;
+    uint32_t dtree;
+  };
+  std::vector<SNode> snodes;
+  rtpg::DBuf<uint32_t> d_stree;
+  // device DTree arrays (owned)
+  struct Owned {
+    uint32_t n = 0, b_nodes = 0, s_nodes = 0;
+    rtpg::DBuf<uint32_t> b_base, b_cnt, s_base, s_cnt, s_depth, b_depth;
+    rtpg::DBuf<float> b_sw, b_sum, b_area, s_sw, s_sum, s_area;
+    rtpg::DBuf<float4> info, b_sums, s_sums;
+    rtpg::DBuf<uint2> b_child, s_child;
+    rtpg::DTreeArrays view();
+  } dt;
+  rtpg::DBuf<uint32_t> d_err;
+
+  rtpg::SdTreeView view();
+  void clear();
+  void build();
+  void reset(int iter, int spp);
+  void publish();
+};

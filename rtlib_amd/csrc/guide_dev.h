@@ -1,0 +1,272 @@
+// Device-side guide structures: SD-tree (Muller 2017) and hash grid + Morton
+// direction quadtrees.  Each function names the reference routine whose
+// semantics it keeps.
+#pragma once
+#include "common.h"
+#include "views.h"
+
+namespace rtpg {
+
+constexpr uint32_t kInvalidCell = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// ---------------------------------------------------------------- SD-tree
+// STreeT::GetDTreeWrapper (OPX7UtilsPathGuiding.h:639-648, descent :561-567).
+// The split axis of a node is its depth mod 3 (root x, SubDivide :1240).
+// Unlike the reference the child index is clamped to {0,1}, so a point on the
+// upper face of the box cannot index past the pair.
+__device__ __forceinline__ uint32_t stree_lookup(const SdTreeView& g, V3 p) {
+  float qx = (p.x - g.aabb_min[0]) / g.aabb_size;
+  float qy = (p.y - g.aabb_min[1]) / g.aabb_size;
+  float qz = (p.z - g.aabb_min[2]) / g.aabb_size;
+  uint32_t w = g.stree[0];
+  uint32_t axis = 0;
+  while (!(w & kLeafBit)) {
+    float& q = axis == 0 ? qx : (axis == 1 ? qy : qz);
+    float pa = q * 2.0f;
+    int idx = (int)pa;
+    idx = idx < 0 ? 0 : (idx > 1 ? 1 : idx);
+    q = pa - (float)idx;
+    w = g.stree[w + (uint32_t)idx];
+    axis = axis == 2 ? 0 : axis + 1;
+  }
+  return w & ~kLeafBit;
+}
+
+RTPG_HD uint32_t dnode_child(uint2 c, int idx) {
+  uint32_t w = idx < 2 ? c.x : c.y;
+  return (w >> ((idx & 1) * 16)) & 0xFFFFu;
+}
+
+RTPG_HD float f4c(float4 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+
+// DTreeNodeT::GetChildIdx (OPX7UtilsPathGuiding.h:69-82): bit0 = x >= .5, bit1 = y >= .5.
+RTPG_HD int dtree_child_idx(float& px, float& py) {
+  int r = 0;
+  if (px < 0.5f) {
+    px *= 2.0f;
+  } else {
+    px = 2.0f * px - 1.0f;
+    r |= 1;
+  }
+  if (py < 0.5f) {
+    py *= 2.0f;
+  } else {
+    py = 2.0f * py - 1.0f;
+    r |= 2;
+  }
+  return r;
+}
+
+// DTreeT::GetMean on the sampling tree (:444-448).
+RTPG_HD float dtree_mean(float sum, float sw) {
+  if (sw <= 0.0f) return 0.0f;
+  float factor = kInv2Pi / (2.0f * sw);
+  return factor * sum;
+}
+
+// DTreeNodeT::Pdf (:278-316) times 1/(4 pi) as in DTreeT::Pdf (:484-489):
+// density on the sphere of the sampling tree of DTree `cell`.
+__device__ __forceinline__ float dtree_pdf(const SdTreeView& g, uint32_t cell, float px, float py) {
+#pragma clang fp contract(off)
+  float4 inf = g.info[cell];
+  if (dtree_mean(inf.y, inf.z) <= 0.0f) return 1.0f * kInv2Pi / 2.0f;
+  uint32_t base = fbits(inf.x);
+  float result = 1.0f;
+  uint32_t node = 0;
+  int idx = dtree_child_idx(px, py);
+  for (;;) {
+    float4 s = g.samp_sums[base + node];
+    float total = s.x + s.y + s.z + s.w;
+    float si = f4c(s, idx);
+    if (si <= 0.0f) {
+      result = 0.0f;
+      break;
+    }
+    result *= 4.0f * si / total;
+    uint32_t ch = dnode_child(g.samp_child[base + node], idx);
+    if (ch == 0) break;
+    node = ch;
+    idx = dtree_child_idx(px, py);
+  }
+  return result * kInv2Pi / 2.0f;
+}
+
+// DTreeNodeT::Sample (:228-277) with the clamp of DTreeT::Sample (:477-482):
+// canonical direction drawn from the sampling tree of DTree `cell`.
+__device__ __forceinline__ void dtree_sample(const SdTreeView& g, uint32_t cell, Pcg32& rng,
+                                             float& ox, float& oy) {
+#pragma clang fp contract(off)
+  float4 inf = g.info[cell];
+  if (dtree_mean(inf.y, inf.z) <= 0.0f) {
+    ox = rnd(rng);
+    oy = rnd(rng);
+    return;
+  }
+  uint32_t base = fbits(inf.x);
+  uint32_t node = 0;
+  float rx = 0.0f, ry = 0.0f, size = 1.0f;
+  for (;;) {
+    float4 s = g.samp_sums[base + node];
+    int idx = 0;
+    float partial = s.x + s.z;
+    float total = s.x + s.y + s.z + s.w;
+    float boundary = partial / total;
+    float ox0 = 0.0f, oy0 = 0.0f;
+    float sample = rnd(rng);
+    if (sample < boundary) {
+      sample = sample / boundary;
+      boundary = s.x / partial;
+    } else {
+      partial = total - partial;
+      ox0 = 0.5f;
+      sample = (sample - boundary) / (1.0f - boundary);
+      boundary = s.y / partial;
+      idx |= 1;
+    }
+    if (sample >= boundary) {
+      oy0 = 0.5f;
+      idx |= 2;
+    }
+    float t0 = size * ox0;
+    float t1 = size * oy0;
+    rx = rx + t0;
+    ry = ry + t1;
+    size *= 0.5f;
+    uint32_t ch = dnode_child(g.samp_child[base + node], idx);
+    if (ch == 0 || f4c(s, idx) <= 0.0f) {
+      float a = rnd(rng);
+      float b = rnd(rng);
+      float t2 = size * a;
+      float t3 = size * b;
+      rx = rx + t2;
+      ry = ry + t3;
+      break;
+    }
+    node = ch;
+  }
+  ox = fminf(fmaxf(rx, 0.0f), 1.0f);
+  oy = fminf(fmaxf(ry, 0.0f), 1.0f);
+}
+
+// ---------------------------------------------------------------- hash grid
+// DoubleBufferedHashGrid3T (OPX7UtilsGrid.h:157-353): cell coordinates, bucket
+// and checksum.  key_mode 1 replaces the pcg1d chain by a 30-bit Morton key of
+// 10-bit cell coordinates: bucket = pcg1d(key) % (slots/32), checksum = key+1.
+RTPG_HD void grid_cell(const HashGridView& h, V3 p, uint32_t& ix, uint32_t& iy, uint32_t& iz) {
+  float fx = (p.x - h.origin[0]) / h.size[0];
+  float fy = (p.y - h.origin[1]) / h.size[1];
+  float fz = (p.z - h.origin[2]) / h.size[2];
+  float gx = (float)h.bounds[0] * fx;
+  float gy = (float)h.bounds[1] * fy;
+  float gz = (float)h.bounds[2] * fz;
+  // static_cast<unsigned int> of a negative float is undefined in C++; the
+  // reference's CUDA conversion saturates it to 0, which is what is kept here.
+  uint32_t cx = gx > 0.0f ? (gx < 4294967040.0f ? (uint32_t)gx : 0xFFFFFFFFu) : 0u;
+  uint32_t cy = gy > 0.0f ? (gy < 4294967040.0f ? (uint32_t)gy : 0xFFFFFFFFu) : 0u;
+  uint32_t cz = gz > 0.0f ? (gz < 4294967040.0f ? (uint32_t)gz : 0xFFFFFFFFu) : 0u;
+  ix = cx < h.bounds[0] - 1u ? cx : h.bounds[0] - 1u;
+  iy = cy < h.bounds[1] - 1u ? cy : h.bounds[1] - 1u;
+  iz = cz < h.bounds[2] - 1u ? cz : h.bounds[2] - 1u;
+}
+
+RTPG_HD void grid_hash(const HashGridView& h, uint32_t ix, uint32_t iy, uint32_t iz,
+                       uint32_t& bucket, uint32_t& checksum) {
+  uint32_t nb = h.slots / 32u;
+  if (h.key_mode == 1u) {
+    uint32_t key = morton3_30(ix, iy, iz);
+    bucket = pcg1d(key) % nb;
+    checksum = key + 1u;
+  } else {
+    bucket = pcg1d(h.slots + iz + pcg1d(iy + pcg1d(ix))) % nb;
+    checksum = 1u + h.bounds[0] * h.bounds[1] * iz + h.bounds[0] * iy + ix;
+  }
+}
+
+// GetPrvCellIndex (:298-319): read-only probe of the previous pass's table.
+RTPG_HD uint32_t grid_find_prv(const HashGridView& h, uint32_t bucket, uint32_t checksum) {
+  for (uint32_t i = 0; i < 32u; ++i) {
+    uint32_t c = h.prv[32u * bucket + i];
+    if (c == 0u) return kInvalidCell;
+    if (c == checksum) return 32u * bucket + i;
+  }
+  return kInvalidCell;
+}
+
+#if defined(__HIPCC__)
+// GetCurCellIndex (:266-297): insert-or-find with atomicCAS(0 -> checksum).
+__device__ __forceinline__ uint32_t grid_find_cur(const HashGridView& h, uint32_t bucket,
+                                                  uint32_t checksum) {
+  for (uint32_t i = 0; i < 32u; ++i) {
+    uint32_t prev = atomicCAS(&h.cur[32u * bucket + i], 0u, checksum);
+    if (prev == 0u || prev == checksum) return 32u * bucket + i;
+  }
+  return kInvalidCell;
+}
+#endif
+
+// ---------------------------------------------------------------- Morton quadtree
+// Leaf offset of level L in the pyramid: (4^L - 1) / 3.
+RTPG_HD uint32_t mq_level_offset(uint32_t level) { return ((1u << (2u * level)) - 1u) / 3u; }
+
+// MortonQuadTreeT::Pdf (OPX7UtilsMorton.h:401-414) times 1/(4 pi) (wrapper :515-523).
+__device__ __forceinline__ float mq_pdf(const MortonView& m, uint32_t cell, float px, float py) {
+  const float* w = m.samp + (size_t)m.per_cell * cell;
+  float root = w[0];
+  if (root <= 0.0f) return 1.0f * 0.25f * kInvPi;
+  uint32_t nn = 1u << m.level;
+  uint32_t x = (uint32_t)(px * (float)nn);
+  uint32_t y = (uint32_t)(py * (float)nn);
+  x = x < nn ? x : nn - 1u;
+  y = y < nn ? y : nn - 1u;
+  uint32_t code = morton2(x, y);
+  float leaf = w[mq_level_offset(m.level) + code];
+  return leaf / root * (float)(nn * nn) * 0.25f * kInvPi;
+}
+
+// MortonQuadTreeT::SampleAndPdf (:286-344); returns canonical coordinates and
+// the density on the sphere.
+__device__ __forceinline__ void mq_sample(const MortonView& m, uint32_t cell, Pcg32& rng,
+                                          float& ox, float& oy, float& pdf) {
+#pragma clang fp contract(off)
+  const float* w = m.samp + (size_t)m.per_cell * cell;
+  float total = w[0];
+  if (total <= 0.0f) {
+    ox = rnd(rng);
+    oy = rnd(rng);
+    pdf = 1.0f * 0.25f * kInvPi;
+    return;
+  }
+  uint32_t size = 4, offset = 1, code = 0;
+  for (uint32_t i = 0; i < m.level; ++i) {
+    float w0 = w[offset + (code << 2) + 0], w1 = w[offset + (code << 2) + 1];
+    float w2 = w[offset + (code << 2) + 2], w3 = w[offset + (code << 2) + 3];
+    uint32_t local = 0;
+    float partial = w0 + w2;
+    float boundary = partial / total;
+    float sample = rnd(rng);
+    if (sample < boundary) {
+      sample /= boundary;
+      boundary = w0 / partial;
+    } else {
+      partial = w1 + w3;
+      sample = (sample - boundary) / (1.0f - boundary);
+      boundary = w1 / partial;
+      local |= 1u;
+    }
+    if (sample >= boundary) local |= 2u;
+    total = local == 0 ? w0 : (local == 1 ? w1 : (local == 2 ? w2 : w3));
+    code = (code << 2) + local;
+    offset += size;
+    size *= 4u;
+  }
+  uint32_t xi = compact1by1_16(code), yi = compact1by1_16(code >> 1);
+  float nn = (float)(1u << m.level);
+  float a = rnd(rng);
+  float b = rnd(rng);
+  ox = ((float)xi + a) / nn;
+  oy = ((float)yi + b) / nn;
+  pdf = total / w[0] * (nn * nn) * 0.25f * kInvPi;
+}
+
+}  // namespace rtpg

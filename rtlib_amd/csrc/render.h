@@ -1,0 +1,51 @@
+// Wavefront path state and the render launchers (host side of render.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "views.h"
+
+namespace rtpg {
+
+constexpr float kRayTmin = 1e-4f;   // SimpleTrace.cu:76
+constexpr float kRayTmax = 1e20f;
+
+// Path meta word (ray_d.w): depth in bits 0..7, flags above.
+constexpr uint32_t kMetaDelta = 0x100u;   // previous event was delta (or the camera)
+
+// Guide vertex flags (vC.w).
+constexpr uint32_t kVtxValid = 1u;
+constexpr uint32_t kVtxDelta = 2u;
+constexpr uint32_t kVtxBadBsdf = 4u;
+
+// Structure-of-arrays path state for one launch chunk of n paths.
+// Path pid = pixel + sample * npix.
+struct PathSoA {
+  uint32_t n;
+  float4* ray_o;     // origin, w = pdf of the sampled direction (MIS)
+  float4* ray_d;     // direction, w = meta word
+  float4* T;         // throughput
+  float4* L;         // radiance
+  float4* hit;       // t, u, v, leaf-order triangle (bits; 0xFFFFFFFF = miss)
+  uint2* rng;        // PCG32 state
+  uint32_t* nhit;    // hits processed
+  uint32_t* queue[2];
+  uint32_t* count;   // live paths per depth [max_depth + 2]
+  uint32_t* scount;  // shadow rays per depth [max_depth + 2]
+  float4* shadow;    // 3 x float4 per shadow ray
+  uint32_t* deep;    // traversal spill stack, (cap - 16) x n
+  uint32_t* err;     // sticky error bits
+  // guide vertices, [depth][pid]
+  float4* vA;        // throughput after the bounce, woPdf
+  float4* vB;        // contribution at this hit, cos of the sampled direction
+  float4* vC;        // canonical direction (u, v), cell, flags
+};
+
+hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n, float4* hits, uint32_t* occl,
+                             bool any, uint32_t* deep, uint32_t* err, hipStream_t s);
+
+hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
+                               const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, int guide,
+                               float* accum, hipStream_t s);
+
+}  // namespace rtpg

@@ -1,0 +1,762 @@
+// Wavefront path tracer for gfx950: camera -> (extend -> shade -> shadow) x depth
+// -> accumulate, plus the guide-record kernels.
+//
+// Replaces the OptiX raygen/closest-hit/miss programs of the reference
+// (SimpleTrace.cu:45-103, 214-425, 625-630; SimpleGuide.cu:48-156, 287-511;
+// SimpleGuide2.cu:48-153, 285-523).  Paths live in structure-of-arrays HBM
+// buffers; between bounces the surviving paths are compacted into a queue with
+// one atomic per wave (ballot + popcount), so each bounce launches only as
+// many work-items as there are live paths.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "guide_dev.h"
+#include "render.h"
+#include "views.h"
+
+namespace rtpg {
+
+constexpr uint32_t kBlock = 128;
+constexpr uint32_t kShortStack = 16;
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated queue append: one atomic per wave, returns this lane's slot.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+  uint64_t mask = __ballot(pred);
+  uint32_t lane = lane_id();
+  uint32_t base = 0;
+  if (mask) {
+    int leader = __ffsll((long long)mask) - 1;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+  }
+  uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64u - lane)));
+  return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ V3 load3(float4 v) { return V3{v.x, v.y, v.z}; }
+
+// Moller-Trumbore without FMA contraction so hits are bit-identical to the
+// CPU brute-force oracle.  Two-sided.
+__device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tmin, float& tbest,
+                                          float& ou, float& ov) {
+#pragma clang fp contract(off)
+  const float4* q = reinterpret_cast<const float4*>(tp);
+  float4 a = q[0], b = q[1], c = q[2];
+  float e1x = b.x, e1y = b.y, e1z = b.z;
+  float e2x = c.x, e2y = c.y, e2z = c.z;
+  float px = d.y * e2z - d.z * e2y;
+  float py = d.z * e2x - d.x * e2z;
+  float pz = d.x * e2y - d.y * e2x;
+  float det = e1x * px + e1y * py + e1z * pz;
+  if (det == 0.0f) return false;
+  float inv = 1.0f / det;
+  float sx = o.x - a.x, sy = o.y - a.y, sz = o.z - a.z;
+  float u = (sx * px + sy * py + sz * pz) * inv;
+  if (!(u >= 0.0f && u <= 1.0f)) return false;
+  float qx = sy * e1z - sz * e1y;
+  float qy = sz * e1x - sx * e1z;
+  float qz = sx * e1y - sy * e1x;
+  float v = (d.x * qx + d.y * qy + d.z * qz) * inv;
+  if (!(v >= 0.0f && u + v <= 1.0f)) return false;
+  float t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  if (!(t > tmin && t < tbest)) return false;
+  tbest = t;
+  ou = u;
+  ov = v;
+  return true;
+}
+
+struct StackRef {
+  uint32_t* lds;     // this lane's column of the LDS short stack
+  uint32_t* deep;    // this ray's column of the spill stack (stride = deep_stride)
+  uint32_t deep_stride;
+  uint32_t cap;      // total capacity
+};
+
+__device__ __forceinline__ void stack_push(const StackRef& s, uint32_t& sp, uint32_t v) {
+  if (sp < kShortStack) {
+    s.lds[sp * kBlock] = v;
+  } else {
+    s.deep[(sp - kShortStack) * s.deep_stride] = v;
+  }
+  ++sp;
+}
+__device__ __forceinline__ uint32_t stack_pop(const StackRef& s, uint32_t& sp) {
+  --sp;
+  return sp < kShortStack ? s.lds[sp * kBlock] : s.deep[(sp - kShortStack) * s.deep_stride];
+}
+
+#define RTPG_CS(a, b)                         \
+  if (tk[b] < tk[a]) {                        \
+    float tt = tk[a]; tk[a] = tk[b]; tk[b] = tt; \
+    int32_t nn = nk[a]; nk[a] = nk[b]; nk[b] = nn; \
+  }
+
+// BVH4 traversal: closest hit (kAny = false) or any hit (kAny = true).
+// Leaf children are intersected when their parent is visited; inner children
+// are visited nearest-first, the others pushed far-to-near.
+template <bool kAny>
+__device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax, const StackRef& st,
+                         uint32_t* err, float& t_out, float& u_out, float& v_out, int32_t& tri_out) {
+  const float eps = 1e-20f;
+  V3 dd = V3{fabsf(d.x) > eps ? d.x : copysignf(eps, d.x), fabsf(d.y) > eps ? d.y : copysignf(eps, d.y),
+             fabsf(d.z) > eps ? d.z : copysignf(eps, d.z)};
+  V3 inv = V3{1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z};
+  V3 oi = o * inv;
+  float tbest = tmax;
+  int32_t best = -1;
+  float bu = 0.0f, bv = 0.0f;
+  uint32_t sp = 0;
+  int32_t node = 0;
+  for (;;) {
+    const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
+    float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    int4 ch = reinterpret_cast<const int4*>(np)[6];
+    uint4 cn = reinterpret_cast<const uint4*>(np)[7];
+    float tk[4];
+    int32_t nk[4];
+    int n_inner = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float lox = f4c(lx, i), hix = f4c(hx, i), loy = f4c(ly, i), hiy = f4c(hy, i);
+      float loz = f4c(lz, i), hiz = f4c(hz, i);
+      float t0x = lox * inv.x - oi.x, t1x = hix * inv.x - oi.x;
+      float t0y = loy * inv.y - oi.y, t1y = hiy * inv.y - oi.y;
+      float t0z = loz * inv.z - oi.z, t1z = hiz * inv.z - oi.z;
+      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
+      bool hit = tn <= tf * 1.00000024f;
+      int32_t c = i == 0 ? ch.x : (i == 1 ? ch.y : (i == 2 ? ch.z : ch.w));
+      uint32_t cnt = i == 0 ? cn.x : (i == 1 ? cn.y : (i == 2 ? cn.z : cn.w));
+      tk[i] = 3.0e38f;
+      nk[i] = 0;
+      if (hit) {
+        if (c < 0) {
+          uint32_t first = (uint32_t)(~c);
+          for (uint32_t k = 0; k < cnt; ++k) {
+            if (isect_tri(sc.tris + first + k, o, d, tmin, tbest, bu, bv)) {
+              best = (int32_t)(first + k);
+              if (kAny) {
+                t_out = tbest;
+                u_out = bu;
+                v_out = bv;
+                tri_out = best;
+                return true;
+              }
+            }
+          }
+        } else {
+          tk[i] = tn;
+          nk[i] = c;
+          ++n_inner;
+        }
+      }
+    }
+    if (n_inner == 0) {
+      if (sp == 0) break;
+      node = (int32_t)stack_pop(st, sp);
+      continue;
+    }
+    RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
+    node = nk[0];
+    if (sp + (uint32_t)(n_inner - 1) > st.cap) {
+      atomicOr(err, 1u);  // stack bound of the build violated: stop this ray
+      break;
+    }
+    for (int k = n_inner - 1; k >= 1; --k) stack_push(st, sp, (uint32_t)nk[k]);
+  }
+  t_out = tbest;
+  u_out = bu;
+  v_out = bv;
+  tri_out = best;
+  return best >= 0;
+}
+#undef RTPG_CS
+
+__device__ __forceinline__ StackRef make_stack(uint32_t (*lds)[kBlock], uint32_t* deep, uint32_t deep_stride,
+                                               uint32_t slot, uint32_t cap) {
+  StackRef s;
+  s.lds = &lds[0][threadIdx.x];
+  s.deep = deep ? deep + slot : nullptr;
+  s.deep_stride = deep_stride;
+  s.cap = cap;
+  return s;
+}
+
+// ------------------------------------------------------------------ ray queries (tests, shadow)
+__global__ void __launch_bounds__(kBlock) k_trace_rays(SceneView sc, const float4* rays, uint32_t n,
+                                                       float4* hits, uint32_t* occl, uint32_t any,
+                                                       uint32_t* deep, uint32_t* err) {
+  __shared__ uint32_t lds[kShortStack][kBlock];
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float4 a = rays[2 * i], b = rays[2 * i + 1];
+  StackRef st = make_stack(lds, deep, n, i, sc.max_stack);
+  float t, u, v;
+  int32_t tri;
+  if (any) {
+    bool h = traverse<true>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, st, err, t, u, v, tri);
+    occl[i] = h ? 1u : 0u;
+  } else {
+    traverse<false>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, st, err, t, u, v, tri);
+    uint32_t prim = tri >= 0 ? fbits(sc.tris[tri].v0[3]) : 0xFFFFFFFFu;
+    hits[i] = make_float4(tri >= 0 ? t : INFINITY, u, v, bitsf(prim));
+  }
+}
+
+// ------------------------------------------------------------------ wavefront kernels
+__global__ void __launch_bounds__(kBlock) k_camera(RenderParams rp, Camera cam, PathSoA ps) {
+  uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
+  if (pid >= ps.n) return;
+  uint32_t npix = rp.width * rp.height;
+  uint32_t pix = pid % npix;
+  uint32_t s = pid / npix;
+  uint32_t x = pix % rp.width, y = pix / rp.width;
+  Pcg32 rng = path_rng(rp.seed, pix, rp.sample_offset + s);
+  float jx = rnd(rng), jy = rnd(rng);
+  float dx = 2.0f * ((float)x + jx) / (float)rp.width - 1.0f;
+  float dy = 2.0f * ((float)y + jy) / (float)rp.height - 1.0f;
+  V3 U{cam.u[0], cam.u[1], cam.u[2]}, Vv{cam.v[0], cam.v[1], cam.v[2]}, W{cam.w[0], cam.w[1], cam.w[2]};
+  V3 d = normalize(dx * U + dy * Vv + W);
+  ps.ray_o[pid] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], 0.0f);
+  ps.ray_d[pid] = make_float4(d.x, d.y, d.z, bitsf(kMetaDelta));  // depth 0, counts emission
+  ps.T[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  ps.L[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ps.rng[pid] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+  ps.queue[0][pid] = pid;
+}
+
+__global__ void __launch_bounds__(kBlock) k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
+  __shared__ uint32_t lds[kShortStack][kBlock];
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t n_act = ps.count[depth];
+  if (i >= n_act) return;
+  uint32_t pid = ps.queue[depth & 1][i];
+  float4 a = ps.ray_o[pid], b = ps.ray_d[pid];
+  StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
+  float t, u, v;
+  int32_t tri;
+  traverse<false>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, kRayTmax, st, ps.err, t, u, v, tri);
+  ps.hit[pid] = make_float4(t, u, v, bitsf((uint32_t)tri));
+}
+
+__device__ __forceinline__ uint32_t pick_light(const SceneView& sc, float u) {
+  uint32_t lo = 0, hi = sc.n_lights - 1u;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (sc.light_cdf[mid] > u) hi = mid; else lo = mid + 1u;
+  }
+  return lo;
+}
+
+struct LightSample {
+  V3 p, n, Le;
+  float area, pdf_sel;
+};
+
+// Uniform point on a light triangle, barycentrics of Random.h:290-296.
+__device__ __forceinline__ LightSample sample_light(const SceneView& sc, Pcg32& rng) {
+  float ul = rnd(rng);
+  float u1 = rnd(rng), u2 = rnd(rng);
+  uint32_t li = pick_light(sc, ul);
+  const float4* L = sc.lights + 4u * li;
+  float4 a = L[0], b = L[1], c = L[2], e = L[3];
+  float lo = fminf(u1, u2), hi = fmaxf(u1, u2);
+  LightSample s;
+  V3 v0{a.x, a.y, a.z}, e1{b.x, b.y, b.z}, e2{c.x, c.y, c.z};
+  s.p = v0 + e1 * (1.0f - hi) + e2 * (hi - lo);
+  s.n = normalize(cross(e1, e2));
+  s.Le = V3{e.x, e.y, e.z};
+  s.area = a.w;
+  s.pdf_sel = b.w;
+  return s;
+}
+
+template <int kGuide>
+__global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd,
+                                                  HashGridView hg, MortonView mq, uint32_t depth) {
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t n_act = ps.count[depth];
+  bool active = i < n_act;
+  bool cont = false, shadow = false;
+  uint32_t pid = 0;
+  V3 p{0, 0, 0}, nd{0, 0, 0}, sdir{0, 0, 0}, scon{0, 0, 0}, Tn{0, 0, 0};
+  float wo_pdf = 0.0f, sdist = 0.0f;
+  uint32_t meta_next = 0;
+  Pcg32 rng;
+  const bool record = (kGuide != 0) && !(rp.flags & kFlagFinal);
+  if (active) {
+    pid = ps.queue[depth & 1][i];
+    float4 ro = ps.ray_o[pid], rd = ps.ray_d[pid], T4 = ps.T[pid], hit = ps.hit[pid];
+    uint2 rs = ps.rng[pid];
+    uint32_t npix = rp.width * rp.height;
+    rng.state = ((uint64_t)rs.y << 32) | rs.x;
+    rng.inc = pcg_stream_inc(pid % npix);
+    uint32_t meta = fbits(rd.w);
+    bool prev_delta = (meta & kMetaDelta) != 0;
+    float prev_pdf = ro.w;
+    V3 o{ro.x, ro.y, ro.z}, d{rd.x, rd.y, rd.z}, T{T4.x, T4.y, T4.z};
+    V3 c{0.0f, 0.0f, 0.0f};
+    float cos_out = 0.0f;
+    int32_t tri = (int32_t)fbits(hit.w);
+    const bool nee = (rp.flags & kFlagNee) && sc.n_lights > 0;
+    const bool ris = nee && (rp.flags & kFlagRis);
+    const bool mis = nee && !ris && (rp.flags & kFlagMis);
+    uint32_t vcell = kInvalidCell;
+    float vcu = 0.0f, vcv = 0.0f;
+    uint32_t vflags = 0;
+    if (tri < 0) {
+      c = T * V3{rp.bg[0], rp.bg[1], rp.bg[2]};
+    } else {
+      const float4* tq = reinterpret_cast<const float4*>(sc.tris + tri);
+      float4 e1q = tq[1], e2q = tq[2];
+      V3 ng = normalize(cross(load3(e1q), load3(e2q)));
+      float dn = dot(d, ng);
+      bool front = dn < 0.0f;
+      V3 n = front ? ng : -ng;
+      p = o + d * hit.x;
+      V3 wo = -d;
+      uint32_t info = sc.info[tri];
+      if (tri_is_light(info)) {
+        // Emission (SimpleTrace.cu:410-413) with MIS against light sampling.
+        if (front) {
+          uint32_t li = tri_light(info);
+          const float4* L = sc.lights + 4u * li;
+          float4 a = L[0], b = L[1], e = L[3];
+          float w = 1.0f;
+          if (depth > 0 && !prev_delta && nee) {
+            if (mis) {
+              float pl = b.w / a.w * hit.x * hit.x / fabsf(dn);
+              w = power_heuristic(prev_pdf, pl);
+            } else {
+              w = 0.0f;
+            }
+          }
+          c = T * V3{e.x, e.y, e.z} * w;
+        }
+      } else if (depth + 1u < rp.max_depth) {
+        float4 sh = sc.shade[tri];
+        uint32_t type = tri_type(info);
+        V3 albedo{sh.x, sh.y, sh.z};
+        float param = sh.w;
+        V3 spec{0.0f, 0.0f, 0.0f};
+        if (type == kMatPhong && sc.spec) {
+          float4 s4 = sc.spec[tri];
+          spec = V3{s4.x, s4.y, s4.z};
+          param = s4.w;
+        }
+        if (type == kMatGlass) {
+          // Dielectric of SimpleTrace.cu:347-407 (Schlick Fresnel), a delta event.
+          float ior = param;
+          float cos_i = dot(ng, d);
+          V3 rn;
+          float eta;
+          if (cos_i < 0.0f) {
+            rn = ng;
+            eta = 1.0f / ior;
+            cos_i = -cos_i;
+          } else {
+            rn = -ng;
+            eta = ior;
+          }
+          float sin_o2 = (1.0f - cos_i * cos_i) * eta * eta;
+          float f0 = (1.0f - eta) / (1.0f + eta);
+          f0 = f0 * f0;
+          float m = 1.0f - cos_i;
+          float fres = f0 + (1.0f - f0) * (m * m * m * m * m);
+          V3 refl = normalize(d - rn * (2.0f * dot(d, rn)));
+          if (rnd(rng) < fres || sin_o2 > 1.0f) {
+            p = p + rn * 1e-4f;
+            nd = refl;
+            Tn = T * albedo;
+          } else {
+            p = p - rn * 1e-4f;
+            float sin_i2 = fmaxf(1.0f - cos_i * cos_i, 0.0f);
+            float cos_o = sqrtf(1.0f - sin_o2);
+            if (sin_i2 > 0.0f) {
+              V3 k = (d + rn * cos_i) / sqrtf(sin_i2);
+              nd = normalize(k * sqrtf(sin_o2) - rn * cos_o);
+            } else {
+              nd = d;
+            }
+            Tn = T;
+          }
+          wo_pdf = fabsf(dot(nd, rn));
+          cos_out = wo_pdf;
+          meta_next = (depth + 1u) | kMetaDelta;
+          vflags = kVtxDelta;
+          cont = max3(Tn) > 0.0f;
+        } else {
+          if (kGuide == 1) {
+            vcell = stree_lookup(sd, p);
+          } else if (kGuide == 2) {
+            uint32_t ix, iy, iz, bucket, csum;
+            grid_cell(hg, p, ix, iy, iz);
+            grid_hash(hg, ix, iy, iz, bucket, csum);
+            if (rp.flags & kFlagLocate) {
+              grid_find_cur(hg, bucket, csum);
+              vcell = kInvalidCell;
+            } else {
+              vcell = grid_find_prv(hg, bucket, csum);
+            }
+          }
+          const bool guided = (kGuide != 0) && (rp.flags & kFlagGuideSample) && vcell != kInvalidCell;
+          const float alpha = rp.fraction;
+          // ---- next-event estimation (SimpleTrace.cu:302-339)
+          if (nee) {
+            if (ris) {
+              float wsum = 0.0f, fa_y = 0.0f, dist_y = 0.0f;
+              uint32_t m = 0;
+              V3 f_y{0, 0, 0}, dir_y{0, 0, 0};
+              for (uint32_t k = 0; k < rp.num_candidates; ++k) {
+                LightSample ls = sample_light(sc, rng);
+                V3 dl = ls.p - p;
+                float dist = length(dl);
+                V3 dir = dl / dist;
+                float ndl = dot(dir, n);
+                float lndl = -dot(dir, ls.n);
+                BsdfEval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
+                float g = fmaxf(ndl, 0.0f) * fmaxf(lndl, 0.0f) / (dist * dist);
+                V3 f = be.f * ls.Le * g;
+                float fa = avg3(f);
+                float wi = fa * (ls.area / ls.pdf_sel);
+                wsum += wi;
+                ++m;
+                float ur = rnd(rng);
+                if ((wi / wsum) >= ur) {
+                  f_y = f;
+                  fa_y = fa;
+                  dir_y = dir;
+                  dist_y = dist;
+                }
+              }
+              if (wsum > 0.0f && fa_y > 0.0f) {
+                float W = wsum / (fa_y * (float)m);
+                scon = T * f_y * W;
+                sdir = dir_y;
+                sdist = dist_y - 1e-4f;
+                shadow = max3(scon) > 0.0f;
+              }
+            } else {
+              LightSample ls = sample_light(sc, rng);
+              V3 dl = ls.p - p;
+              float dist2 = dot(dl, dl);
+              float dist = sqrtf(dist2);
+              V3 dir = dl / dist;
+              float cos_l = -dot(dir, ls.n);
+              float cos_s = dot(dir, n);
+              if (cos_l > 0.0f && cos_s > 0.0f) {
+                BsdfEval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
+                float pl = ls.pdf_sel / ls.area * dist2 / cos_l;
+                float w = 1.0f;
+                if (mis) {
+                  float bp = be.pdf;
+                  if (guided) {
+                    float cu, cv;
+                    dir_to_canonical(dir, cu, cv);
+                    float gp = kGuide == 1 ? dtree_pdf(sd, vcell, cu, cv) : mq_pdf(mq, vcell, cu, cv);
+                    bp = (1.0f - alpha) * bp + alpha * gp;
+                  }
+                  w = power_heuristic(pl, bp);
+                }
+                scon = T * be.f * ls.Le * (cos_s * w / pl);
+                sdir = dir;
+                sdist = dist - 1e-4f;
+                shadow = max3(scon) > 0.0f;
+              }
+            }
+          }
+          // ---- direction sampling: BSDF or guide mixture (SimpleGuide.cu:359-377)
+          V3 dir;
+          if (guided) {
+            float us = rnd(rng);
+            if (us < alpha) {
+              float gu, gv;
+              if (kGuide == 1) {
+                dtree_sample(sd, vcell, rng, gu, gv);
+              } else {
+                float gp;
+                mq_sample(mq, vcell, rng, gu, gv, gp);
+              }
+              dir = normalize(canonical_to_dir(gu, gv));
+            } else {
+              dir = bsdf_sample(type, albedo, param, spec, n, wo, rng);
+            }
+          } else {
+            dir = bsdf_sample(type, albedo, param, spec, n, wo, rng);
+          }
+          BsdfEval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
+          wo_pdf = be.pdf;
+          if (guided) {
+            dir_to_canonical(dir, vcu, vcv);
+            float gp = kGuide == 1 ? dtree_pdf(sd, vcell, vcu, vcv) : mq_pdf(mq, vcell, vcu, vcv);
+            wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
+          } else if (record && vcell != kInvalidCell) {
+            dir_to_canonical(dir, vcu, vcv);
+          }
+          float cosd = dot(dir, n);
+          cos_out = cosd;
+          Tn = (wo_pdf > 0.0f && cosd > 0.0f) ? T * be.f * (cosd / wo_pdf) : V3{0.0f, 0.0f, 0.0f};
+          if ((rp.flags & kFlagRR) && depth >= rp.rr_depth) {
+            float q = fminf(max3(Tn), 0.95f);
+            if (rnd(rng) >= q) {
+              Tn = V3{0.0f, 0.0f, 0.0f};
+            } else {
+              Tn = Tn / q;
+            }
+          }
+          bool bsdf_ok = finite3(be.f) && be.f.x >= 0.0f && be.f.y >= 0.0f && be.f.z >= 0.0f;
+          vflags = bsdf_ok ? 0u : kVtxBadBsdf;
+          nd = dir;
+          meta_next = depth + 1u;
+          cont = max3(Tn) > 0.0f && isfinite(max3(Tn));
+        }
+      }
+    }
+    // Path radiance and, for guiding, the per-hit contribution.
+    float4 L4 = ps.L[pid];
+    ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
+    ps.nhit[pid] = depth + 1u;
+    if (record) {
+      size_t vi = (size_t)depth * ps.n + pid;
+      ps.vB[vi] = make_float4(c.x, c.y, c.z, cos_out);
+      bool has_vertex = cont && vcell != kInvalidCell;
+      ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
+      ps.vC[vi] = make_float4(vcu, vcv, bitsf(has_vertex ? vcell : kInvalidCell),
+                              bitsf(vflags | (has_vertex ? kVtxValid : 0u)));
+    }
+    if (cont) {
+      V3 ndir = nd;
+      ps.ray_o[pid] = make_float4(p.x, p.y, p.z, wo_pdf);
+      ps.ray_d[pid] = make_float4(ndir.x, ndir.y, ndir.z, bitsf(meta_next));
+      ps.T[pid] = make_float4(Tn.x, Tn.y, Tn.z, 0.0f);
+    }
+    ps.rng[pid] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+  }
+  uint32_t slot = wave_append(&ps.count[depth + 1u], cont);
+  if (cont) ps.queue[(depth + 1u) & 1][slot] = pid;
+  uint32_t sslot = wave_append(&ps.scount[depth], shadow);
+  if (shadow) {
+    float4* e = ps.shadow + 3u * (size_t)sslot;
+    e[0] = make_float4(p.x, p.y, p.z, sdist);
+    e[1] = make_float4(sdir.x, sdir.y, sdir.z, bitsf(pid));
+    e[2] = make_float4(scon.x, scon.y, scon.z, 0.0f);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
+  __shared__ uint32_t lds[kShortStack][kBlock];
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ps.scount[depth]) return;
+  const float4* e = ps.shadow + 3u * (size_t)i;
+  float4 a = e[0], b = e[1], c = e[2];
+  uint32_t pid = fbits(b.w);
+  StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
+  float t, u, v;
+  int32_t tri;
+  bool occ = traverse<true>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, a.w, st, ps.err, t, u, v, tri);
+  if (!occ) {
+    float4 L4 = ps.L[pid];
+    ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
+    if (record) {
+      size_t vi = (size_t)depth * ps.n + pid;
+      float4 vb = ps.vB[vi];
+      ps.vB[vi] = make_float4(vb.x + c.x, vb.y + c.y, vb.z + c.z, vb.w);
+    }
+  }
+}
+
+__global__ void k_set_count(uint32_t* c, uint32_t n) { c[0] = n; }
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// Sums the launch's samples per pixel in sample order and adds to the image.
+__global__ void k_accum(PathSoA ps, uint32_t npix, uint32_t spp, float* accum) {
+  uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= npix) return;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (uint32_t s = 0; s < spp; ++s) {
+    float4 l = ps.L[(size_t)s * npix + pix];
+    sx += l.x;
+    sy += l.y;
+    sz += l.z;
+  }
+  accum[3 * pix + 0] += sx;
+  accum[3 * pix + 1] += sy;
+  accum[3 * pix + 2] += sz;
+}
+
+// ------------------------------------------------------------------ guide records
+// TraceVertexT::Commit (OPX7UtilsPathGuiding.h:830-866): the radiance that
+// reached vertex d is the sum of the contributions of the later hits, in hit
+// order; divided by the vertex throughput (per-channel guard T*woPdf > 1e-4),
+// times |cos|, averaged over RGB, divided by woPdf.
+__device__ __forceinline__ bool commit_value(const PathSoA& ps, uint32_t pid, uint32_t d, uint32_t nh,
+                                             uint32_t& cell, float& cu, float& cv, float& value) {
+  size_t vi = (size_t)d * ps.n + pid;
+  float4 vc = ps.vC[vi];
+  uint32_t fl = fbits(vc.w);
+  cell = fbits(vc.z);
+  if (!(fl & kVtxValid) || (fl & kVtxDelta) || cell == kInvalidCell) return false;
+  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+  for (uint32_t k = d + 1u; k < nh; ++k) {
+    float4 b = ps.vB[(size_t)k * ps.n + pid];
+    rx += b.x;
+    ry += b.y;
+    rz += b.z;
+  }
+  float4 va = ps.vA[vi];
+  float wo_pdf = va.w;
+  bool ok = isfinite(rx) && rx >= 0.0f && isfinite(ry) && ry >= 0.0f && isfinite(rz) && rz >= 0.0f;
+  if (wo_pdf <= 0.0f || !ok || (fl & kVtxBadBsdf)) return false;
+  float lx = va.x * wo_pdf > 1e-4f ? rx / va.x : 0.0f;
+  float ly = va.y * wo_pdf > 1e-4f ? ry / va.y : 0.0f;
+  float lz = va.z * wo_pdf > 1e-4f ? rz / va.z : 0.0f;
+  float cosv = fabsf(ps.vB[vi].w);
+  lx *= cosv;
+  ly *= cosv;
+  lz *= cosv;
+  float avg = (lx + ly + lz) / 3.0f;
+  value = avg / wo_pdf;
+  cu = vc.x;
+  cv = vc.y;
+  return true;
+}
+
+// Wave-aggregated float add: lanes sharing an address are summed and one
+// lane issues the atomic.  Returns after all lanes are served.
+__device__ __forceinline__ void wave_add_by_key(float* base, uint32_t key, float val, bool pred) {
+  uint64_t pending = __ballot(pred);
+  while (pending) {
+    int leader = __ffsll((long long)pending) - 1;
+    uint32_t k = __shfl(key, leader);
+    bool mine = pred && key == k && ((pending >> lane_id()) & 1ull);
+    uint64_t same = __ballot(mine);
+    float v = mine ? val : 0.0f;
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if ((int)lane_id() == leader) unsafeAtomicAdd(base + k, v);
+    pending &= ~same;
+  }
+}
+
+// Records every path's vertices into the building SD-tree: statistical
+// weight per DTree with wave aggregation, flux into the first leaf slot on the
+// quadtree descent (DTreeNodeT::Record :113-125, RecordIrradiance :419-427).
+__global__ void __launch_bounds__(kBlock) k_commit_sdtree(PathSoA ps, SdTreeView sd, uint32_t max_vtx) {
+  uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
+  bool in = pid < ps.n;
+  uint32_t nh = in ? ps.nhit[pid] : 0u;
+  uint32_t nv = nh > 0 ? nh - 1u : 0u;
+  nv = nv < max_vtx ? nv : max_vtx;
+  uint32_t nv_max = nv;
+  for (int off = 32; off >= 1; off >>= 1) nv_max = max(nv_max, (uint32_t)__shfl_xor((int)nv_max, off));
+  for (uint32_t d = 0; d < nv_max; ++d) {
+    uint32_t cell = 0;
+    float cu = 0, cv = 0, value = 0;
+    bool ok = (d < nv) && commit_value(ps, pid, d, nh, cell, cu, cv, value);
+    wave_add_by_key(sd.build_sw, cell, 1.0f, ok);
+    if (ok && isfinite(value) && value > 0.0f) {
+      uint32_t base = fbits(sd.info[cell].w);
+      uint32_t node = 0;
+      int idx = dtree_child_idx(cu, cv);
+      for (;;) {
+        uint32_t ch = dnode_child(sd.build_child[base + node], idx);
+        if (ch == 0) break;
+        node = ch;
+        idx = dtree_child_idx(cu, cv);
+      }
+      unsafeAtomicAdd(reinterpret_cast<float*>(sd.build_sums + base + node) + idx, value);
+    }
+  }
+}
+
+// MortonTraceVertexT::Commit (OPX7UtilsMorton.h:550-585) + leaf-only Record (:262-285).
+__global__ void __launch_bounds__(kBlock) k_commit_morton(PathSoA ps, MortonView mq, uint32_t max_vtx) {
+  uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
+  if (pid >= ps.n) return;
+  uint32_t nh = ps.nhit[pid];
+  uint32_t nv = nh > 0 ? nh - 1u : 0u;
+  nv = nv < max_vtx ? nv : max_vtx;
+  uint32_t nn = 1u << mq.level;
+  for (uint32_t d = 0; d < nv; ++d) {
+    uint32_t cell;
+    float cu, cv, value;
+    if (!commit_value(ps, pid, d, nh, cell, cu, cv, value)) continue;
+    uint32_t x = (uint32_t)(cu * (float)nn), y = (uint32_t)(cv * (float)nn);
+    x = x < nn ? x : nn - 1u;
+    y = y < nn ? y : nn - 1u;
+    uint32_t idx = mq_level_offset(mq.level) + morton2(x, y);
+    unsafeAtomicAdd(mq.build + (size_t)mq.per_cell * cell + idx, value);
+  }
+}
+
+// Ray counters of a chunk: paths, radiance rays (live paths per depth) and
+// shadow rays, accumulated in 64 bits.
+__global__ void k_stats_add(const uint32_t* count, const uint32_t* scount, uint32_t depth, uint32_t n,
+                            unsigned long long* stats) {
+  if (threadIdx.x != 0) return;
+  unsigned long long r = 0, sh = 0;
+  for (uint32_t d = 0; d < depth; ++d) {
+    r += count[d];
+    sh += scount[d];
+  }
+  stats[0] += n;
+  stats[1] += r;
+  stats[2] += sh;
+}
+
+// ------------------------------------------------------------------ host launchers
+hipError_t launch_stats_add(const uint32_t* count, const uint32_t* scount, uint32_t depth, uint32_t n,
+                            unsigned long long* stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_add, dim3(1), dim3(64), 0, s, count, scount, depth, n, stats);
+  return hipGetLastError();
+}
+
+static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
+
+hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n, float4* hits, uint32_t* occl,
+                             bool any, uint32_t* deep, uint32_t* err, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trace_rays, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, sc, rays, n, hits, occl,
+                     any ? 1u : 0u, deep, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
+                               const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, int guide,
+                               float* accum, hipStream_t s) {
+  uint32_t nb = blocks_for(ps.n, kBlock);
+  uint32_t ncnt = 2u * (rp.max_depth + 2u);
+  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(ncnt, 64)), dim3(64), 0, s, ps.count, ncnt);
+  hipLaunchKernelGGL(k_camera, dim3(nb), dim3(kBlock), 0, s, rp, cam, ps);
+  // count[0] = n: written by a one-thread kernel to stay stream ordered.
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, ps.count, ps.n);
+  const bool record = guide != 0 && !(rp.flags & kFlagFinal);
+  for (uint32_t d = 0; d < rp.max_depth; ++d) {
+    hipLaunchKernelGGL(k_extend, dim3(nb), dim3(kBlock), 0, s, sc, ps, d);
+    if (guide == 1) {
+      hipLaunchKernelGGL(k_shade<1>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+    } else if (guide == 2) {
+      hipLaunchKernelGGL(k_shade<2>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+    } else {
+      hipLaunchKernelGGL(k_shade<0>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+    }
+    hipLaunchKernelGGL(k_shadow, dim3(nb), dim3(kBlock), 0, s, sc, ps, d, record ? 1u : 0u);
+  }
+  uint32_t npix = rp.width * rp.height;
+  hipLaunchKernelGGL(k_accum, dim3(blocks_for(npix, 256)), dim3(256), 0, s, ps, npix, rp.spp, accum);
+  if (record && guide == 1) {
+    hipLaunchKernelGGL(k_commit_sdtree, dim3(nb), dim3(kBlock), 0, s, ps, sd, rp.max_depth);
+  } else if (record && guide == 2 && !(rp.flags & kFlagLocate)) {
+    hipLaunchKernelGGL(k_commit_morton, dim3(nb), dim3(kBlock), 0, s, ps, mq, rp.max_depth);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace rtpg

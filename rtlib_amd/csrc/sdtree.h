@@ -1,0 +1,67 @@
+// SD-tree (Muller et al. 2017) as flattened device arrays.
+//
+// Device layout (all in HBM, structure of arrays):
+//   stree[node]            u32: leaf -> 0x80000000 | DTree index, inner -> index of the
+//                          first child (children are allocated in pairs, axis = depth % 3)
+//   per DTree j            b_base/b_cnt, s_base/s_cnt (u32), b_sw (recorded weight),
+//                          b_sum/b_area, s_sw/s_sum/s_area (f32), info (float4 view for
+//                          the tracer: s_base, s_sum, s_sw, b_base)
+//   node pools             sums (float4) and children (uint2 = 4 x u16, relative to the
+//                          DTree's base) for the building and the sampling trees
+// DTree nodes are numbered exactly as the reference numbers them (DFS of
+// RTDTreeT::Reset), so exported arrays compare element-wise with it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "views.h"
+
+namespace rtpg {
+
+struct DTreeArrays {
+  uint32_t n = 0;          // DTrees
+  uint32_t b_nodes = 0;    // building pool size
+  uint32_t s_nodes = 0;    // sampling pool size
+  uint32_t* b_base = nullptr;
+  uint32_t* b_cnt = nullptr;
+  uint32_t* s_base = nullptr;
+  uint32_t* s_cnt = nullptr;
+  uint32_t* s_depth = nullptr;  // max depth reached by the last Reset (statistics)
+  uint32_t* b_depth = nullptr;
+  float* b_sw = nullptr;
+  float* b_sum = nullptr;
+  float* b_area = nullptr;
+  float* s_sw = nullptr;
+  float* s_sum = nullptr;
+  float* s_area = nullptr;
+  float4* info = nullptr;
+  float4* b_sums = nullptr;
+  uint2* b_child = nullptr;
+  float4* s_sums = nullptr;
+  uint2* s_child = nullptr;
+};
+
+// Kernel launchers (sdtree.hip).
+hipError_t launch_dtree_build(DTreeArrays a, float* area_tmp, hipStream_t s);
+hipError_t launch_dtree_publish(DTreeArrays a, hipStream_t s);
+// Reset of the building trees of `dst` from the sampling trees of `src`
+// (per new DTree j: source map[j]); pass 0 counts nodes into dst.b_cnt,
+// pass 1 writes children into dst.b_child at dst.b_base.
+hipError_t launch_dtree_reset(DTreeArrays src, DTreeArrays dst, const uint32_t* map, uint32_t max_depth,
+                              float subdiv_th, int pass, uint32_t* err, hipStream_t s);
+// dst sampling trees := copies of src sampling trees (per map).
+hipError_t launch_dtree_copy_sampling(DTreeArrays src, DTreeArrays dst, const uint32_t* map, hipStream_t s);
+hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, hipStream_t s);
+
+// Test/driver kernels on an SdTreeView.
+hipError_t launch_sdtree_record(SdTreeView v, const float* pos, const float* canon, const float* irr,
+                                const float* sw, uint32_t n, hipStream_t s);
+hipError_t launch_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n, uint64_t seed, float* out_canon,
+                                hipStream_t s);
+hipError_t launch_sdtree_pdf(SdTreeView v, const uint32_t* cells, const float* canon, uint32_t n, float* out_pdf,
+                             hipStream_t s);
+hipError_t launch_sdtree_lookup(SdTreeView v, const float* pos, uint32_t n, uint32_t* out_cells, hipStream_t s);
+
+}  // namespace rtpg

@@ -1,0 +1,329 @@
+// Device kernels of the SD-tree: Build (post-order sums + area), Reset of the
+// building trees from the sampling trees, sampling-tree copies after a
+// spatial split, and record/sample/pdf/lookup entry kernels.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "guide_dev.h"
+#include "sdtree.h"
+
+namespace rtpg {
+
+static inline uint32_t nblk(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
+
+// RTDTreeT::Build (OPX7UtilsPathGuiding.h:948-957) -> DTreeNodeT::Build
+// (:356-369) and GetArea (:342-355).  Children always carry larger indices
+// than their parent (DFS allocation in Reset), so one reverse sweep over a
+// DTree's nodes visits every child before its parent and reproduces the
+// recursive sums with the same addition order.
+__global__ void k_dtree_build(DTreeArrays a, float* area) {
+#pragma clang fp contract(off)
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  uint32_t base = a.b_base[j], cnt = a.b_cnt[j];
+  for (uint32_t k = cnt; k-- > 0;) {
+    uint2 c = a.b_child[base + k];
+    float4 s = a.b_sums[base + k];
+    float sv[4] = {s.x, s.y, s.z, s.w};
+    for (int i = 0; i < 4; ++i) {
+      uint32_t ch = dnode_child(c, i);
+      if (ch != 0) {
+        float4 cs = a.b_sums[base + ch];
+        float sum = 0.0f;
+        sum += cs.x;
+        sum += cs.y;
+        sum += cs.z;
+        sum += cs.w;
+        sv[i] = sum;
+      }
+    }
+    float ar = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+      if (sv[i] > 0.0f) {
+        uint32_t ch = dnode_child(c, i);
+        if (ch == 0) {
+          ar += 1.0f / 4.0f;
+        } else {
+          ar += area[base + ch] / 4.0f;
+        }
+      }
+    }
+    a.b_sums[base + k] = make_float4(sv[0], sv[1], sv[2], sv[3]);
+    area[base + k] = ar;
+  }
+  float4 r = a.b_sums[base];
+  float sum = 0.0f;
+  sum += r.x;
+  sum += r.y;
+  sum += r.z;
+  sum += r.w;
+  a.b_sum[j] = sum;
+  a.b_area[j] = area[base];
+}
+
+__global__ void k_dtree_publish(DTreeArrays a) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  a.info[j] = make_float4(bitsf(a.s_base[j]), a.s_sum[j], a.s_sw[j], bitsf(a.b_base[j]));
+}
+
+// RTDTreeT::Reset (OPX7UtilsPathGuiding.h:888-947), one work-item per DTree.
+// The DFS keeps the reference's LIFO order, so nodes get the reference's
+// numbers.  A stack entry either copies a node of the source sampling tree
+// (src != kSelf) or continues below a source leaf (src == kSelf), whose four
+// sums are all equal to `val` (a quarter of the parent slot).
+constexpr uint32_t kSelf = 0xFFFFFFFFu;
+constexpr int kResetStack = 64;
+
+__global__ void k_dtree_reset(DTreeArrays src, DTreeArrays dst, const uint32_t* map, uint32_t max_depth,
+                              float th, int pass, uint32_t* err) {
+#pragma clang fp contract(off)
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= dst.n) return;
+  uint32_t sj = map[j];
+  uint32_t sbase = src.s_base[sj];
+  float total = src.s_sum[sj];
+  uint32_t obase = pass ? dst.b_base[j] : 0u;
+  uint32_t st_dst[kResetStack], st_src[kResetStack], st_depth[kResetStack];
+  float st_val[kResetStack];
+  int sp = 0;
+  st_dst[0] = 0;
+  st_src[0] = 0;
+  st_depth[0] = 1;
+  st_val[0] = 0.0f;
+  sp = 1;
+  uint32_t count = 1;
+  uint32_t maxd = 0;
+  if (pass) {
+    dst.b_child[obase] = make_uint2(0u, 0u);
+    dst.b_sums[obase] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  bool stop = false;
+  while (sp > 0 && !stop) {
+    --sp;
+    uint32_t e_dst = st_dst[sp], e_src = st_src[sp], e_depth = st_depth[sp];
+    float e_val = st_val[sp];
+    maxd = e_depth > maxd ? e_depth : maxd;
+    float4 ss = make_float4(e_val, e_val, e_val, e_val);
+    uint2 sc = make_uint2(0u, 0u);
+    if (e_src != kSelf) {
+      ss = src.s_sums[sbase + e_src];
+      sc = src.s_child[sbase + e_src];
+    }
+    uint32_t newc[4] = {0u, 0u, 0u, 0u};
+    for (int i = 0; i < 4; ++i) {
+      float si = f4c(ss, i);
+      float fraction;
+      if (total > 0.0f) {
+        fraction = si / total;
+      } else {
+        // std::pow(0.25f, depth) evaluates in double; 0.25^d is exact in float
+        // for every depth that can occur.
+        fraction = ldexpf(1.0f, -2 * (int)e_depth);
+      }
+      if (e_depth < max_depth && fraction > th) {
+        if (sp >= kResetStack) {
+          atomicOr(err, 2u);
+          stop = true;
+          break;
+        }
+        uint32_t sch = (e_src != kSelf) ? dnode_child(sc, i) : 0u;
+        if (e_src != kSelf && sch != 0u) {
+          st_src[sp] = sch;
+          st_val[sp] = 0.0f;
+        } else {
+          st_src[sp] = kSelf;
+          st_val[sp] = si / 4.0f;
+        }
+        st_dst[sp] = count;
+        st_depth[sp] = e_depth + 1u;
+        ++sp;
+        newc[i] = count;
+        if (pass) {
+          dst.b_child[obase + count] = make_uint2(0u, 0u);
+          dst.b_sums[obase + count] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        ++count;
+        if (count > 65535u) {
+          stop = true;
+          break;
+        }
+      }
+    }
+    if (pass) {
+      dst.b_child[obase + e_dst] = make_uint2(newc[0] | (newc[1] << 16), newc[2] | (newc[3] << 16));
+    }
+  }
+  if (!pass) {
+    dst.b_cnt[j] = count;
+  } else {
+    dst.b_depth[j] = maxd;
+    dst.b_sw[j] = 0.0f;
+    dst.b_sum[j] = 0.0f;
+    dst.b_area[j] = 0.0f;
+  }
+}
+
+// Sampling tree of new DTree j := sampling tree of map[j] (SubDivide copies the
+// whole wrapper, RTSTreeT::SubDivide :1231-1247).  One block per DTree.
+__global__ void k_dtree_copy_sampling(DTreeArrays src, DTreeArrays dst, const uint32_t* map) {
+  uint32_t j = blockIdx.x;
+  if (j >= dst.n) return;
+  uint32_t sj = map[j];
+  uint32_t sb = src.s_base[sj], cnt = src.s_cnt[sj], db = dst.s_base[j];
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+    dst.s_sums[db + k] = src.s_sums[sb + k];
+    dst.s_child[db + k] = src.s_child[sb + k];
+  }
+  if (threadIdx.x == 0) {
+    dst.s_sw[j] = src.s_sw[sj];
+    dst.s_sum[j] = src.s_sum[sj];
+    dst.s_area[j] = src.s_area[sj];
+    dst.s_depth[j] = src.s_depth[sj];
+  }
+}
+
+// Three-phase exclusive scan (1024-element tiles).
+constexpr uint32_t kScanTile = 1024;
+
+__global__ void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tile_sums) {
+  __shared__ uint32_t s[kScanTile];
+  uint32_t t = threadIdx.x;
+  uint32_t g = blockIdx.x * kScanTile + t;
+  uint32_t v = g < n ? in[g] : 0u;
+  s[t] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < kScanTile; off <<= 1) {
+    uint32_t x = t >= off ? s[t - off] : 0u;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  if (g < n) out[g] = s[t] - v;
+  if (t == kScanTile - 1) tile_sums[blockIdx.x] = s[t];
+}
+
+__global__ void k_scan_sums(uint32_t* tile_sums, uint32_t ntiles, uint32_t* total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t acc = 0;
+  for (uint32_t i = 0; i < ntiles; ++i) {
+    uint32_t v = tile_sums[i];
+    tile_sums[i] = acc;
+    acc += v;
+  }
+  *total = acc;
+}
+
+__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* tile_sums) {
+  uint32_t g = blockIdx.x * kScanTile + threadIdx.x;
+  if (g < n) out[g] += tile_sums[blockIdx.x];
+}
+
+// DTreeT::Impl<Nearest>::RecordIrradiance (:419-427) at the DTree found by
+// the STree lookup of `pos`.
+__global__ void k_sdtree_record(SdTreeView v, const float* pos, const float* canon, const float* irr,
+                                const float* sw, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t cell = stree_lookup(v, V3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+  float w = sw[i];
+  if (!(isfinite(w) && w > 0.0f)) return;
+  unsafeAtomicAdd(v.build_sw + cell, w);
+  float val = irr[i];
+  if (!(isfinite(val) && val > 0.0f)) return;
+  float cu = canon[2 * i], cv = canon[2 * i + 1];
+  uint32_t base = fbits(v.info[cell].w);
+  uint32_t node = 0;
+  int idx = dtree_child_idx(cu, cv);
+  for (;;) {
+    uint32_t ch = dnode_child(v.build_child[base + node], idx);
+    if (ch == 0) break;
+    node = ch;
+    idx = dtree_child_idx(cu, cv);
+  }
+  unsafeAtomicAdd(reinterpret_cast<float*>(v.build_sums + base + node) + idx, val * w);
+}
+
+__global__ void k_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n, uint64_t seed, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Pcg32 rng = pcg32_seed(seed, i);
+  float u, w;
+  dtree_sample(v, cells[i], rng, u, w);
+  out[2 * i] = u;
+  out[2 * i + 1] = w;
+}
+
+__global__ void k_sdtree_pdf(SdTreeView v, const uint32_t* cells, const float* canon, uint32_t n, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = dtree_pdf(v, cells[i], canon[2 * i], canon[2 * i + 1]);
+}
+
+__global__ void k_sdtree_lookup(SdTreeView v, const float* pos, uint32_t n, uint32_t* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = stree_lookup(v, V3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+}
+
+// ------------------------------------------------------------------ launchers
+hipError_t launch_dtree_build(DTreeArrays a, float* area_tmp, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dtree_build, dim3(nblk(a.n, 64)), dim3(64), 0, s, a, area_tmp);
+  return hipGetLastError();
+}
+hipError_t launch_dtree_publish(DTreeArrays a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dtree_publish, dim3(nblk(a.n, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_dtree_reset(DTreeArrays src, DTreeArrays dst, const uint32_t* map, uint32_t max_depth,
+                              float th, int pass, uint32_t* err, hipStream_t s) {
+  if (dst.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dtree_reset, dim3(nblk(dst.n, 64)), dim3(64), 0, s, src, dst, map, max_depth, th, pass,
+                     err);
+  return hipGetLastError();
+}
+hipError_t launch_dtree_copy_sampling(DTreeArrays src, DTreeArrays dst, const uint32_t* map, hipStream_t s) {
+  if (dst.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dtree_copy_sampling, dim3(dst.n), dim3(256), 0, s, src, dst, map);
+  return hipGetLastError();
+}
+hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, hipStream_t s) {
+  uint32_t ntiles = nblk(n, kScanTile);
+  if (ntiles == 0) ntiles = 1;
+  uint32_t* tile_sums = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&tile_sums), ntiles * sizeof(uint32_t), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kScanTile), 0, s, in, out, n, tile_sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1), 0, s, tile_sums, ntiles, total);
+  hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(kScanTile), 0, s, out, n, tile_sums);
+  e = hipGetLastError();
+  hipFreeAsync(tile_sums, s);
+  return e;
+}
+hipError_t launch_sdtree_record(SdTreeView v, const float* pos, const float* canon, const float* irr,
+                                const float* sw, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sdtree_record, dim3(nblk(n, 256)), dim3(256), 0, s, v, pos, canon, irr, sw, n);
+  return hipGetLastError();
+}
+hipError_t launch_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n, uint64_t seed, float* out,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sdtree_sample, dim3(nblk(n, 256)), dim3(256), 0, s, v, cells, n, seed, out);
+  return hipGetLastError();
+}
+hipError_t launch_sdtree_pdf(SdTreeView v, const uint32_t* cells, const float* canon, uint32_t n, float* out,
+                             hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sdtree_pdf, dim3(nblk(n, 256)), dim3(256), 0, s, v, cells, canon, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_sdtree_lookup(SdTreeView v, const float* pos, uint32_t n, uint32_t* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sdtree_lookup, dim3(nblk(n, 256)), dim3(256), 0, s, v, pos, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace rtpg

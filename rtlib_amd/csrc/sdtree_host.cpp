@@ -1,0 +1,669 @@
+// SD-tree host logic and C-ABI: spatial refine (host, topology only), device
+// DTree reset/build driving, export/import and the pass controller.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "../../include/rtpg.h"
+#include "context.h"
+
+using namespace rtpg;
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return RTPG_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return RTPG_ERR_INVALID;
+  } catch (...) {
+    set_last_error("unknown error");
+    return RTPG_ERR_INVALID;
+  }
+}
+
+void require(bool c, const char* msg) {
+  if (!c) throw Error(RTPG_ERR_INVALID, msg);
+}
+
+std::vector<uint32_t> exclusive_prefix(const std::vector<uint32_t>& c, uint32_t& total) {
+  std::vector<uint32_t> b(c.size());
+  uint64_t acc = 0;
+  for (size_t i = 0; i < c.size(); ++i) {
+    b[i] = (uint32_t)acc;
+    acc += c[i];
+  }
+  if (acc > 0xFFFFFFFFull) throw Error(RTPG_ERR_LIMIT, "DTree node pool exceeds 2^32 nodes");
+  total = (uint32_t)acc;
+  return b;
+}
+
+}  // namespace
+
+DTreeArrays rtpg_sdtree_t::Owned::view() {
+  DTreeArrays a;
+  a.n = n;
+  a.b_nodes = b_nodes;
+  a.s_nodes = s_nodes;
+  a.b_base = b_base.p;
+  a.b_cnt = b_cnt.p;
+  a.s_base = s_base.p;
+  a.s_cnt = s_cnt.p;
+  a.s_depth = s_depth.p;
+  a.b_depth = b_depth.p;
+  a.b_sw = b_sw.p;
+  a.b_sum = b_sum.p;
+  a.b_area = b_area.p;
+  a.s_sw = s_sw.p;
+  a.s_sum = s_sum.p;
+  a.s_area = s_area.p;
+  a.info = info.p;
+  a.b_sums = b_sums.p;
+  a.b_child = b_child.p;
+  a.s_sums = s_sums.p;
+  a.s_child = s_child.p;
+  return a;
+}
+
+static void alloc_per_dtree(rtpg_sdtree_t::Owned& o, uint32_t n) {
+  o.n = n;
+  o.b_base.alloc(n);
+  o.b_cnt.alloc(n);
+  o.s_base.alloc(n);
+  o.s_cnt.alloc(n);
+  o.s_depth.alloc(n);
+  o.b_depth.alloc(n);
+  o.b_sw.alloc(n);
+  o.b_sum.alloc(n);
+  o.b_area.alloc(n);
+  o.s_sw.alloc(n);
+  o.s_sum.alloc(n);
+  o.s_area.alloc(n);
+  o.info.alloc(n);
+}
+
+static std::vector<uint32_t> encode_stree(const std::vector<rtpg_sdtree_t::SNode>& sn) {
+  std::vector<uint32_t> e(sn.size());
+  for (size_t i = 0; i < sn.size(); ++i) e[i] = sn[i].leaf ? (0x80000000u | sn[i].dtree) : sn[i].child;
+  return e;
+}
+
+SdTreeView rtpg_sdtree_t::view() {
+  SdTreeView v{};
+  v.stree = d_stree.p;
+  for (int k = 0; k < 3; ++k) v.aabb_min[k] = aabb_min[k];
+  v.aabb_size = size;
+  v.info = dt.info.p;
+  v.build_sw = dt.b_sw.p;
+  v.build_sums = dt.b_sums.p;
+  v.build_child = dt.b_child.p;
+  v.samp_sums = dt.s_sums.p;
+  v.samp_child = dt.s_child.p;
+  v.fraction = fraction;
+  v.n_dtrees = dt.n;
+  return v;
+}
+
+namespace rtpg {
+SdTreeView sdtree_view_of(rtpg_sdtree_t* t) { return t->view(); }
+}  // namespace rtpg
+
+void rtpg_sdtree_t::publish() { RTPG_HIP(launch_dtree_publish(dt.view(), ctx->stream)); }
+
+// RTSTreeWrapperT::Clear (OPX7UtilsPathGuiding.h:1634-1636): a single leaf
+// whose DTrees (building, sampling) are one empty node each (RTDTreeT ctor :879-886).
+void rtpg_sdtree_t::clear() {
+  hipStream_t s = ctx->stream;
+  snodes.assign(1, SNode{true, 0u, 0u});
+  d_stree.upload(encode_stree(snodes), s);
+  alloc_per_dtree(dt, 1);
+  std::vector<uint32_t> zero_u{0u}, one_u{1u};
+  std::vector<float> zero_f{0.0f};
+  dt.b_base.upload(zero_u, s);
+  dt.s_base.upload(zero_u, s);
+  dt.b_cnt.upload(one_u, s);
+  dt.s_cnt.upload(one_u, s);
+  dt.b_depth.upload(zero_u, s);
+  dt.s_depth.upload(zero_u, s);
+  for (auto* f : {&dt.b_sw, &dt.b_sum, &dt.b_area, &dt.s_sw, &dt.s_sum, &dt.s_area}) f->upload(zero_f, s);
+  dt.b_nodes = dt.s_nodes = 1;
+  dt.b_sums.alloc(1);
+  dt.s_sums.alloc(1);
+  dt.b_child.alloc(1);
+  dt.s_child.alloc(1);
+  dt.b_sums.zero(1, s);
+  dt.s_sums.zero(1, s);
+  dt.b_child.zero(1, s);
+  dt.s_child.zero(1, s);
+  publish();
+  RTPG_HIP(hipStreamSynchronize(s));
+}
+
+// RTSTreeWrapperT::Build (:1657-1667): per leaf building.Build(), then
+// sampling = building.
+void rtpg_sdtree_t::build() {
+  hipStream_t s = ctx->stream;
+  DBuf<float> area;
+  area.alloc(dt.b_nodes);
+  RTPG_HIP(launch_dtree_build(dt.view(), area.p, s));
+  uint32_t n = dt.n;
+  dt.s_sums.alloc(dt.b_nodes);
+  dt.s_child.alloc(dt.b_nodes);
+  dt.s_nodes = dt.b_nodes;
+  RTPG_HIP(hipMemcpyAsync(dt.s_sums.p, dt.b_sums.p, sizeof(float4) * dt.b_nodes, hipMemcpyDeviceToDevice, s));
+  RTPG_HIP(hipMemcpyAsync(dt.s_child.p, dt.b_child.p, sizeof(uint2) * dt.b_nodes, hipMemcpyDeviceToDevice, s));
+  auto cp_u = [&](DBuf<uint32_t>& d, DBuf<uint32_t>& src) {
+    RTPG_HIP(hipMemcpyAsync(d.p, src.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  };
+  auto cp_f = [&](DBuf<float>& d, DBuf<float>& src) {
+    RTPG_HIP(hipMemcpyAsync(d.p, src.p, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+  };
+  cp_u(dt.s_base, dt.b_base);
+  cp_u(dt.s_cnt, dt.b_cnt);
+  cp_u(dt.s_depth, dt.b_depth);
+  cp_f(dt.s_sw, dt.b_sw);
+  cp_f(dt.s_sum, dt.b_sum);
+  cp_f(dt.s_area, dt.b_area);
+  publish();
+  RTPG_HIP(hipStreamSynchronize(s));
+}
+
+// RTSTreeWrapperT::Reset (:1645-1656): refine (RTSTreeT::Refine :1275-1311)
+// then per leaf RTDTreeWrapperT::Reset(maxDepth, 0.01).
+void rtpg_sdtree_t::reset(int iter, int spp) {
+  if (iter <= 0) return;
+  hipStream_t s = ctx->stream;
+  size_t th;
+  if (split_mode == 0) {
+    th = (size_t)(std::sqrt(std::pow(2.0, iter) * spp / 4.0f) * (double)split_c);
+  } else {
+    th = (size_t)((double)split_c * std::sqrt(std::pow(2.0, iter)));
+  }
+  uint32_t n = dt.n;
+  std::vector<float> bsw = dt.b_sw.to_vector(n, s);
+  std::vector<uint32_t> bcnt = dt.b_cnt.to_vector(n, s);
+  std::vector<uint32_t> scnt = dt.s_cnt.to_vector(n, s);
+  // Host topology with per-leaf (building weight, source DTree).
+  struct HN {
+    bool leaf;
+    uint8_t axis;
+    uint32_t child[2];
+    float sw;
+    int64_t src;  // DTree the wrapper was copied from; -1 for inner defaults
+  };
+  std::vector<HN> hn(snodes.size());
+  for (size_t i = 0; i < snodes.size(); ++i) {
+    hn[i].leaf = snodes[i].leaf;
+    hn[i].child[0] = snodes[i].child;
+    hn[i].child[1] = snodes[i].child + 1u;
+    hn[i].sw = snodes[i].leaf ? bsw[snodes[i].dtree] : 0.0f;
+    hn[i].src = snodes[i].leaf ? (int64_t)snodes[i].dtree : -1;
+    hn[i].axis = 0;
+  }
+  // axes by depth (root axis 0, children (axis+1)%3)
+  {
+    std::vector<std::pair<uint32_t, uint8_t>> st{{0u, (uint8_t)0}};
+    while (!st.empty()) {
+      auto [i, ax] = st.back();
+      st.pop_back();
+      hn[i].axis = ax;
+      if (!hn[i].leaf) {
+        st.push_back({hn[i].child[0], (uint8_t)((ax + 1) % 3)});
+        st.push_back({hn[i].child[1], (uint8_t)((ax + 1) % 3)});
+      }
+    }
+  }
+  bool refine = true;
+  if (max_mb >= 0) {
+    size_t fp = 0;
+    for (const HN& h : hn) {
+      if (h.leaf) {
+        fp += 24u * ((size_t)bcnt[(size_t)h.src] + scnt[(size_t)h.src]) + 48u;
+      } else {
+        fp += 96u;  // a default RTDTreeWrapper: two one-node trees
+      }
+    }
+    if (fp / 1000000 >= (size_t)max_mb) refine = false;
+  }
+  if (refine) {
+    std::vector<std::pair<size_t, int>> st{{0, 1}};
+    while (!st.empty()) {
+      auto [idx, depth] = st.back();
+      st.pop_back();
+      if (hn[idx].leaf) {
+        bool split = hn.size() < (size_t)std::numeric_limits<uint32_t>::max() - 1 && hn[idx].sw > (float)th;
+        if (split) {
+          size_t c = hn.size();
+          hn.resize(c + 2);
+          HN& cur = hn[idx];
+          for (int i = 0; i < 2; ++i) {
+            HN& ch = hn[c + i];
+            ch.leaf = true;
+            ch.axis = (uint8_t)((cur.axis + 1) % 3);
+            ch.sw = cur.sw / 2.0f;
+            ch.src = cur.src;
+            ch.child[0] = ch.child[1] = 0;
+            cur.child[i] = (uint32_t)(c + i);
+          }
+          cur.leaf = false;
+          cur.src = -1;
+          cur.sw = 0.0f;
+        }
+      }
+      if (!hn[idx].leaf) {
+        uint32_t c0 = hn[idx].child[0], c1 = hn[idx].child[1];
+        st.push_back({c0, depth + 1});
+        st.push_back({c1, depth + 1});
+      }
+    }
+  }
+  if (hn.size() >= 0x80000000ull) throw Error(RTPG_ERR_LIMIT, "spatial tree exceeds 2^31 nodes");
+  // New DTree order: leaves in node order (reference Upload :1480-1512).
+  std::vector<uint32_t> map;
+  std::vector<SNode> ns(hn.size());
+  for (size_t i = 0; i < hn.size(); ++i) {
+    ns[i].leaf = hn[i].leaf;
+    ns[i].child = hn[i].leaf ? 0u : hn[i].child[0];
+    ns[i].dtree = 0;
+    if (hn[i].leaf) {
+      ns[i].dtree = (uint32_t)map.size();
+      map.push_back((uint32_t)hn[i].src);
+    }
+  }
+  uint32_t n2 = (uint32_t)map.size();
+  // Device: new DTree arrays.
+  Owned nd;
+  alloc_per_dtree(nd, n2);
+  DBuf<uint32_t> d_map, d_total;
+  d_map.upload(map, s);
+  d_total.alloc(1);
+  std::vector<uint32_t> s_cnt2(n2);
+  for (uint32_t j = 0; j < n2; ++j) s_cnt2[j] = scnt[map[j]];
+  uint32_t s_total = 0;
+  std::vector<uint32_t> s_base2 = exclusive_prefix(s_cnt2, s_total);
+  nd.s_cnt.upload(s_cnt2, s);
+  nd.s_base.upload(s_base2, s);
+  nd.s_nodes = s_total;
+  nd.s_sums.alloc(s_total);
+  nd.s_child.alloc(s_total);
+  d_err.reserve(1);
+  d_err.zero(1, s);
+  DTreeArrays src = dt.view();
+  DTreeArrays dstv = nd.view();
+  RTPG_HIP(launch_dtree_copy_sampling(src, dstv, d_map.p, s));
+  RTPG_HIP(launch_dtree_reset(src, dstv, d_map.p, max_dtree_depth, subdiv_th, 0, d_err.p, s));
+  RTPG_HIP(launch_exclusive_scan(nd.b_cnt.p, nd.b_base.p, n2, d_total.p, s));
+  uint32_t b_total = 0;
+  d_total.download(&b_total, 1, s);
+  nd.b_nodes = b_total;
+  nd.b_sums.alloc(b_total);
+  nd.b_child.alloc(b_total);
+  dstv = nd.view();
+  RTPG_HIP(launch_dtree_reset(src, dstv, d_map.p, max_dtree_depth, subdiv_th, 1, d_err.p, s));
+  uint32_t err = 0;
+  d_err.download(&err, 1, s);
+  if (err) throw Error(RTPG_ERR_LIMIT, "DTree reset stack bound exceeded");
+  snodes.swap(ns);
+  d_stree.upload(encode_stree(snodes), s);
+  dt = std::move(nd);
+  publish();
+  RTPG_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int rtpg_sdtree_create(rtpg_context ctx, const float aabb_min[3], const float aabb_max[3], uint32_t max_dtree_depth,
+                       float fraction, rtpg_sdtree* out) {
+  return guarded([&] {
+    require(ctx && aabb_min && aabb_max && out, "NULL argument");
+    require(max_dtree_depth >= 1 && max_dtree_depth <= 20, "max_dtree_depth must be in [1, 20]");
+    require(fraction >= 0.0f && fraction <= 1.0f, "fraction must be in [0, 1]");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    auto* t = new rtpg_sdtree_t();
+    t->ctx = ctx;
+    // RTSTreeT ctor (:1202-1208): cube of the largest extent at aabbMin.
+    float sx = aabb_max[0] - aabb_min[0], sy = aabb_max[1] - aabb_min[1], sz = aabb_max[2] - aabb_min[2];
+    float m = std::max(std::max(sx, sy), sz);
+    for (int k = 0; k < 3; ++k) t->aabb_min[k] = aabb_min[k];
+    // size = (aabbMin + maxSize) - aabbMin, as the device computes aabbMax - aabbMin
+    float mx = aabb_min[0] + m;
+    t->size = mx - aabb_min[0];
+    t->max_dtree_depth = max_dtree_depth;
+    t->fraction = fraction;
+    try {
+      t->clear();
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
+
+int rtpg_sdtree_destroy(rtpg_sdtree tree) {
+  return guarded([&] {
+    if (!tree) return;
+    (void)hipStreamSynchronize(tree->ctx->stream);
+    delete tree;
+  });
+}
+
+int rtpg_sdtree_clear(rtpg_sdtree tree) {
+  return guarded([&] {
+    require(tree != nullptr, "tree is NULL");
+    tree->clear();
+  });
+}
+
+int rtpg_sdtree_set_split(rtpg_sdtree tree, int mode, float c, int max_mb, float subdiv_th) {
+  return guarded([&] {
+    require(tree != nullptr, "tree is NULL");
+    require(mode == 0 || mode == 1, "mode must be 0 or 1");
+    require(c > 0.0f, "c must be positive");
+    // Each subdivided slot holds more than subdiv_th of the mass, so a level
+    // has at most 1/subdiv_th of them: keep the DTree under 65536 nodes.
+    require(subdiv_th > 0.0f && 1.0 + 4.0 * (tree->max_dtree_depth - 1) * std::floor(1.0 / subdiv_th) < 65536.0,
+            "subdiv_th too small for 16-bit DTree children");
+    tree->split_mode = mode;
+    tree->split_c = c;
+    tree->max_mb = max_mb;
+    tree->subdiv_th = subdiv_th;
+  });
+}
+
+int rtpg_sdtree_reset(rtpg_sdtree tree, int iter, int samples_per_pass) {
+  return guarded([&] {
+    require(tree != nullptr, "tree is NULL");
+    tree->reset(iter, samples_per_pass);
+  });
+}
+
+int rtpg_sdtree_build(rtpg_sdtree tree) {
+  return guarded([&] {
+    require(tree != nullptr, "tree is NULL");
+    tree->build();
+  });
+}
+
+int rtpg_sdtree_record(rtpg_sdtree tree, const float* pos, const float* canon, const float* irradiance,
+                       const float* stat_weight, uint32_t n) {
+  return guarded([&] {
+    require(tree && pos && canon && irradiance && stat_weight, "NULL argument");
+    if (n == 0) return;
+    hipStream_t s = tree->ctx->stream;
+    DBuf<float> dp, dc, di, dw;
+    dp.upload(pos, 3 * (size_t)n, s);
+    dc.upload(canon, 2 * (size_t)n, s);
+    di.upload(irradiance, n, s);
+    dw.upload(stat_weight, n, s);
+    RTPG_HIP(launch_sdtree_record(tree->view(), dp.p, dc.p, di.p, dw.p, n, s));
+    RTPG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int rtpg_sdtree_lookup(rtpg_sdtree tree, const float* pos, uint32_t n, uint32_t* dtree) {
+  return guarded([&] {
+    require(tree && pos && dtree, "NULL argument");
+    if (n == 0) return;
+    hipStream_t s = tree->ctx->stream;
+    DBuf<float> dp;
+    DBuf<uint32_t> dout;
+    dp.upload(pos, 3 * (size_t)n, s);
+    dout.alloc(n);
+    RTPG_HIP(launch_sdtree_lookup(tree->view(), dp.p, n, dout.p, s));
+    dout.download(dtree, n, s);
+  });
+}
+
+int rtpg_sdtree_sample(rtpg_sdtree tree, const uint32_t* dtree, uint32_t n, uint64_t seed, float* canon) {
+  return guarded([&] {
+    require(tree && dtree && canon, "NULL argument");
+    if (n == 0) return;
+    for (uint32_t i = 0; i < n; ++i) require(dtree[i] < tree->dt.n, "DTree index out of range");
+    hipStream_t s = tree->ctx->stream;
+    DBuf<uint32_t> dc;
+    DBuf<float> dout;
+    dc.upload(dtree, n, s);
+    dout.alloc(2 * (size_t)n);
+    RTPG_HIP(launch_sdtree_sample(tree->view(), dc.p, n, seed, dout.p, s));
+    dout.download(canon, 2 * (size_t)n, s);
+  });
+}
+
+int rtpg_sdtree_pdf(rtpg_sdtree tree, const uint32_t* dtree, const float* canon, uint32_t n, float* pdf) {
+  return guarded([&] {
+    require(tree && dtree && canon && pdf, "NULL argument");
+    if (n == 0) return;
+    for (uint32_t i = 0; i < n; ++i) require(dtree[i] < tree->dt.n, "DTree index out of range");
+    hipStream_t s = tree->ctx->stream;
+    DBuf<uint32_t> dc;
+    DBuf<float> dq, dout;
+    dc.upload(dtree, n, s);
+    dq.upload(canon, 2 * (size_t)n, s);
+    dout.alloc(n);
+    RTPG_HIP(launch_sdtree_pdf(tree->view(), dc.p, dq.p, n, dout.p, s));
+    dout.download(pdf, n, s);
+  });
+}
+
+int rtpg_sdtree_sizes(rtpg_sdtree tree, uint32_t sizes[4]) {
+  return guarded([&] {
+    require(tree && sizes, "NULL argument");
+    sizes[0] = (uint32_t)tree->snodes.size();
+    sizes[1] = tree->dt.n;
+    sizes[2] = tree->dt.b_nodes;
+    sizes[3] = tree->dt.s_nodes;
+  });
+}
+
+int rtpg_sdtree_export(rtpg_sdtree tree, uint32_t* stree, uint32_t* b_cnt, uint32_t* s_cnt, float* scal,
+                       float* b_sums, uint16_t* b_child, float* s_sums, uint16_t* s_child) {
+  return guarded([&] {
+    require(tree != nullptr, "tree is NULL");
+    hipStream_t s = tree->ctx->stream;
+    auto& dt = tree->dt;
+    uint32_t n = dt.n;
+    if (stree) {
+      std::vector<uint32_t> e = encode_stree(tree->snodes);
+      std::memcpy(stree, e.data(), e.size() * 4);
+    }
+    if (b_cnt) dt.b_cnt.download(b_cnt, n, s);
+    if (s_cnt) dt.s_cnt.download(s_cnt, n, s);
+    if (scal) {
+      DBuf<float>* arrs[6] = {&dt.b_sw, &dt.b_sum, &dt.b_area, &dt.s_sw, &dt.s_sum, &dt.s_area};
+      for (int k = 0; k < 6; ++k) arrs[k]->download(scal + (size_t)k * n, n, s);
+    }
+    if (b_sums) dt.b_sums.download(reinterpret_cast<float4*>(b_sums), dt.b_nodes, s);
+    if (b_child) dt.b_child.download(reinterpret_cast<uint2*>(b_child), dt.b_nodes, s);
+    if (s_sums) dt.s_sums.download(reinterpret_cast<float4*>(s_sums), dt.s_nodes, s);
+    if (s_child) dt.s_child.download(reinterpret_cast<uint2*>(s_child), dt.s_nodes, s);
+  });
+}
+
+int rtpg_sdtree_import(rtpg_sdtree tree, uint32_t n_stree, const uint32_t* stree, uint32_t n_dtrees,
+                       const uint32_t* b_cnt, const uint32_t* s_cnt, const float* scal, const float* b_sums,
+                       const uint16_t* b_child, const float* s_sums, const uint16_t* s_child) {
+  return guarded([&] {
+    require(tree && stree && b_cnt && s_cnt && scal && b_sums && b_child && s_sums && s_child, "NULL argument");
+    require(n_stree >= 1 && n_dtrees >= 1, "empty tree");
+    // Validate the topology: pairs of children, every leaf names a distinct DTree.
+    std::vector<rtpg_sdtree_t::SNode> sn(n_stree);
+    std::vector<uint8_t> seen(n_dtrees, 0);
+    uint32_t leaves = 0;
+    for (uint32_t i = 0; i < n_stree; ++i) {
+      uint32_t w = stree[i];
+      if (w & 0x80000000u) {
+        uint32_t d = w & 0x7FFFFFFFu;
+        require(d < n_dtrees && !seen[d], "bad DTree index in stree");
+        seen[d] = 1;
+        sn[i] = {true, 0u, d};
+        ++leaves;
+      } else {
+        require(w > i && w + 1u < n_stree, "bad child index in stree");
+        sn[i] = {false, w, 0u};
+      }
+    }
+    require(leaves == n_dtrees, "stree leaves must match n_dtrees");
+    std::vector<uint32_t> bc(b_cnt, b_cnt + n_dtrees), scv(s_cnt, s_cnt + n_dtrees);
+    for (uint32_t j = 0; j < n_dtrees; ++j) require(bc[j] >= 1 && bc[j] <= 65536u && scv[j] >= 1 && scv[j] <= 65536u,
+                                                    "DTree node counts must be in [1, 65536]");
+    uint32_t bt = 0, st = 0;
+    std::vector<uint32_t> bb = exclusive_prefix(bc, bt), sb = exclusive_prefix(scv, st);
+    // child indices must stay inside their DTree and point forward
+    for (uint32_t j = 0; j < n_dtrees; ++j) {
+      for (uint32_t k = 0; k < bc[j]; ++k)
+        for (int c = 0; c < 4; ++c) {
+          uint32_t ch = b_child[4 * ((size_t)bb[j] + k) + c];
+          require(ch == 0 || (ch > k && ch < bc[j]), "bad building child index");
+        }
+      for (uint32_t k = 0; k < scv[j]; ++k)
+        for (int c = 0; c < 4; ++c) {
+          uint32_t ch = s_child[4 * ((size_t)sb[j] + k) + c];
+          require(ch == 0 || (ch > k && ch < scv[j]), "bad sampling child index");
+        }
+    }
+    hipStream_t s = tree->ctx->stream;
+    rtpg_sdtree_t::Owned o;
+    alloc_per_dtree(o, n_dtrees);
+    o.b_nodes = bt;
+    o.s_nodes = st;
+    o.b_cnt.upload(bc, s);
+    o.s_cnt.upload(scv, s);
+    o.b_base.upload(bb, s);
+    o.s_base.upload(sb, s);
+    std::vector<uint32_t> z(n_dtrees, 0u);
+    o.b_depth.upload(z, s);
+    o.s_depth.upload(z, s);
+    DBuf<float>* arrs[6] = {&o.b_sw, &o.b_sum, &o.b_area, &o.s_sw, &o.s_sum, &o.s_area};
+    for (int k = 0; k < 6; ++k) arrs[k]->upload(scal + (size_t)k * n_dtrees, n_dtrees, s);
+    o.b_sums.upload(reinterpret_cast<const float4*>(b_sums), bt, s);
+    o.b_child.upload(reinterpret_cast<const uint2*>(b_child), bt, s);
+    o.s_sums.upload(reinterpret_cast<const float4*>(s_sums), st, s);
+    o.s_child.upload(reinterpret_cast<const uint2*>(s_child), st, s);
+    RTPG_HIP(hipStreamSynchronize(s));
+    tree->snodes.swap(sn);
+    tree->d_stree.upload(encode_stree(tree->snodes), s);
+    tree->dt = std::move(o);
+    tree->publish();
+    RTPG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ controller
+// Pass schedule of RTSTreeControllerT (OPX7UtilsPathGuiding.h:1769-1920):
+// passes of 2^k * spl samples; the last pass takes the remainder when what is
+// left after it would be less than twice its size, or when the samples spent
+// reach ratio * budget.
+struct rtpg_sdtree_ctrl_t {
+  rtpg_sdtree_t* tree = nullptr;
+  uint32_t budget = 0, spl = 1, iter_built = 0;
+  float ratio = 0.5f;
+  bool start = false, executing = false;
+  uint32_t per_all = 0, per_tmp = 0, remain = 0, pass = 0, iter = 0;
+  uint32_t state = 0;
+};
+
+extern "C" {
+
+int rtpg_sdtree_ctrl_create(rtpg_sdtree tree, uint32_t sample_for_budget, uint32_t iteration_for_built,
+                            float ratio_for_budget, uint32_t sample_per_launch, rtpg_sdtree_ctrl* out) {
+  return guarded([&] {
+    require(tree && out, "NULL argument");
+    require(sample_per_launch >= 1 && sample_for_budget >= 1, "budget and samples per launch must be >= 1");
+    auto* c = new rtpg_sdtree_ctrl_t();
+    c->tree = tree;
+    c->budget = sample_for_budget;
+    c->iter_built = iteration_for_built;
+    c->ratio = ratio_for_budget;
+    c->spl = sample_per_launch;
+    *out = c;
+  });
+}
+
+int rtpg_sdtree_ctrl_destroy(rtpg_sdtree_ctrl c) {
+  delete c;
+  return RTPG_OK;
+}
+
+int rtpg_sdtree_ctrl_start(rtpg_sdtree_ctrl c) {
+  return guarded([&] {
+    require(c != nullptr, "ctrl is NULL");
+    c->start = true;
+  });
+}
+
+int rtpg_sdtree_ctrl_beg_trace(rtpg_sdtree_ctrl c) {
+  return guarded([&] {
+    require(c != nullptr, "ctrl is NULL");
+    if (c->start) {
+      c->per_all = 0;
+      c->per_tmp = 0;
+      c->iter = 0;
+      c->start = false;
+      c->executing = true;
+      c->remain = ((c->budget - 1 + c->spl) / c->spl) * c->spl;
+      c->pass = 0;
+      c->state = 0;
+      c->tree->clear();
+    }
+    if (!c->executing) return;
+    if (c->per_tmp == 0) {
+      c->remain -= c->pass;
+      uint64_t want = (uint64_t)(1ull << std::min(c->iter, 31u)) * c->spl;
+      c->pass = (uint32_t)std::min<uint64_t>(c->remain, want);
+      if ((c->remain - c->pass < 2 * c->pass) || ((float)c->per_all >= c->ratio * (float)c->budget)) {
+        c->pass = c->remain;
+      }
+      if (c->remain > c->pass) c->tree->reset((int)c->iter, (int)c->pass);
+    }
+    if (c->iter > c->iter_built) {
+      c->state = c->remain > c->pass ? 1u : 2u;
+    } else {
+      c->state = 0u;
+    }
+  });
+}
+
+int rtpg_sdtree_ctrl_end_trace(rtpg_sdtree_ctrl c, uint32_t samples) {
+  return guarded([&] {
+    require(c != nullptr, "ctrl is NULL");
+    if (!c->executing) return;
+    uint32_t k = samples ? samples : c->spl;
+    c->per_all += k;
+    c->per_tmp += k;
+    if (c->per_tmp >= c->pass) {
+      c->tree->build();
+      c->iter++;
+      c->per_tmp = 0;
+    }
+    if (c->per_all > c->budget) {
+      c->executing = false;
+      c->per_all = 0;
+    }
+  });
+}
+
+int rtpg_sdtree_ctrl_query(rtpg_sdtree_ctrl c, uint32_t q[6]) {
+  return guarded([&] {
+    require(c && q, "NULL argument");
+    q[0] = c->state;
+    q[1] = c->iter;
+    q[2] = c->per_tmp;
+    q[3] = c->pass;
+    q[4] = c->executing ? 1u : 0u;
+    q[5] = c->per_all;
+  });
+}
+
+}  // extern "C"

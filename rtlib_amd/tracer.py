@@ -1,0 +1,462 @@
+"""Python mirror of RTLib's tracer/integrator interface over the rtpg C ABI.
+
+Class and method names follow the reference's C++ API so code written
+against it reads the same:
+  RTSTreeWrapper / RTSTreeController        Utils/OPX7UtilsPathGuiding.h:1333-1920
+  RTMortonQuadTreeWrapper / ...Controller   Utils/OPX7UtilsMorton.h:590-993
+  DoubleBufferedHashGrid3Buffer             Test/Include/RTLibExtOPX7Test.h:1472-1570
+  Context.render (TracerData::Launch)       Test/Include/RTLibExtOPX7Test.h:1345-1348
+All work runs through librtpg.so on the GPU; there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, ptr
+from .scenes import camera_uvw
+
+FLAG_NEE = 1
+FLAG_RIS = 2
+FLAG_USE_GRID = 4
+FLAG_USE_TREE = 8
+FLAG_GUIDE_SAMPLE = 16
+FLAG_FINAL = 32
+FLAG_LOCATE = 64
+FLAG_MIS = 128
+FLAG_RR = 256
+FLAG_USE_MLP = 512
+
+u32 = C.c_uint32
+u16 = C.c_uint16
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class Context:
+    """Device, stream, scene, camera and frame (OPX7Context + TracerData)."""
+
+    def __init__(self, device: int = 0):
+        _lib.load()
+        h = C.c_void_p()
+        call("rtpg_context_create", device, C.byref(h))
+        self.h = h
+        self.width = self.height = 0
+
+    def close(self):
+        if self.h:
+            call("rtpg_context_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        call("rtpg_synchronize", self.h)
+
+    # -- scene
+    def upload_scene(self, scene: dict, light_mode: int = 0, light_mesh=None):
+        v = _f32(scene["verts"])
+        i = _u32(scene["idx"])
+        alb = _f32(scene["albedo"])
+        mat = _u32(scene["mat"])
+        em = _f32(scene["emission"])
+        sp = None if scene.get("spec") is None else _f32(scene["spec"])
+        lm = None if light_mesh is None else _u32(light_mesh)
+        call("rtpg_scene_upload", self.h, ptr(v), v.shape[0], ptr(i, u32), i.shape[0], ptr(alb), ptr(mat, u32),
+             ptr(em), ptr(sp), light_mode, ptr(lm, u32))
+        self.n_tris = i.shape[0]
+        cam = scene.get("camera")
+        if cam is not None:
+            self.set_camera_look(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], cam["width"], cam["height"])
+
+    def set_albedo(self, albedo_rgb):
+        a = _f32(albedo_rgb)
+        call("rtpg_scene_set_albedo", self.h, ptr(a))
+
+    def scene_info(self):
+        info = np.zeros(6, np.uint32)
+        box = np.zeros(6, np.float32)
+        call("rtpg_scene_info", self.h, ptr(info, u32), ptr(box))
+        keys = ["nodes", "bvh_depth", "stack_bound", "max_leaf", "lights", "tris"]
+        d = {k: int(x) for k, x in zip(keys, info)}
+        d["aabb_min"] = box[:3].copy()
+        d["aabb_max"] = box[3:].copy()
+        return d
+
+    # -- camera / frame
+    def set_camera(self, eye, u, v, w):
+        e, uu, vv, ww = (_f32(x, (3,)) for x in (eye, u, v, w))
+        call("rtpg_camera_set", self.h, ptr(e), ptr(uu), ptr(vv), ptr(ww))
+
+    def set_camera_look(self, eye, lookat, up, fovy, width, height):
+        e, u, v, w = camera_uvw(eye, lookat, up, fovy, width / height)
+        self.set_camera(e, u, v, w)
+        if (width, height) != (self.width, self.height):
+            self.resize(width, height)
+
+    def resize(self, width, height):
+        call("rtpg_frame_resize", self.h, width, height)
+        self.width, self.height = width, height
+
+    def clear(self):
+        call("rtpg_accum_clear", self.h)
+
+    def accum(self):
+        """(sum image H x W x 3 with row 0 at the bottom, samples accumulated)."""
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        spp = C.c_uint32()
+        call("rtpg_accum_download", self.h, ptr(out), C.byref(spp))
+        return out, spp.value
+
+    def image(self):
+        a, spp = self.accum()
+        return a / max(spp, 1)
+
+    # -- render
+    def render(self, spp=1, flags=FLAG_NEE | FLAG_MIS | FLAG_RR, max_depth=8, rr_depth=3, num_candidates=32,
+               seed=0, sample_offset=0, bg=(0.0, 0.0, 0.0), fraction=0.3, sdtree=None, hashgrid=None,
+               mqtree=None, max_paths=0):
+        p = _lib.RenderParams()
+        p.flags = flags
+        p.spp = spp
+        p.max_depth = max_depth
+        p.rr_depth = rr_depth
+        p.num_candidates = num_candidates
+        p.sample_offset = sample_offset
+        p.seed = seed
+        p.bg[0], p.bg[1], p.bg[2] = bg
+        p.fraction = fraction
+        p.sdtree = sdtree.h if sdtree is not None else None
+        p.hashgrid = hashgrid.h if hashgrid is not None else None
+        p.mqtree = mqtree.h if mqtree is not None else None
+        p.max_paths = max_paths
+        call("rtpg_render", self.h, C.byref(p))
+
+    def render_stats(self):
+        s = np.zeros(3, np.uint64)
+        call("rtpg_render_stats", self.h, ptr(s, C.c_uint64))
+        return {"paths": int(s[0]), "rays": int(s[1]), "shadow_rays": int(s[2])}
+
+    # -- ray queries
+    def trace_closest(self, rays):
+        r = _f32(rays, (-1, 8))
+        hits = np.zeros((r.shape[0], 4), np.float32)
+        call("rtpg_trace_closest", self.h, ptr(r), r.shape[0], ptr(hits))
+        prim = hits[:, 3].view(np.uint32).copy()
+        return hits[:, 0].copy(), hits[:, 1].copy(), hits[:, 2].copy(), prim
+
+    def trace_occluded(self, rays):
+        r = _f32(rays, (-1, 8))
+        occ = np.zeros(r.shape[0], np.uint32)
+        call("rtpg_trace_occluded", self.h, ptr(r), r.shape[0], ptr(occ, u32))
+        return occ.astype(bool)
+
+
+class RTSTreeWrapper:
+    """RTSTreeWrapperT: spatial binary tree of directional quadtrees on the GPU."""
+
+    TraceStateRecord, TraceStateRecordAndSample, TraceStateSample = 0, 1, 2
+
+    def __init__(self, ctx: Context, aabb_min, aabb_max, max_dtree_depth=20, fraction=0.3):
+        h = C.c_void_p()
+        lo, hi = _f32(aabb_min, (3,)), _f32(aabb_max, (3,))
+        call("rtpg_sdtree_create", ctx.h, ptr(lo), ptr(hi), max_dtree_depth, fraction, C.byref(h))
+        self.h = h
+        self.ctx = ctx
+        self.fraction = fraction
+
+    def Destroy(self):
+        if self.h:
+            call("rtpg_sdtree_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.Destroy()
+        except Exception:
+            pass
+
+    def Clear(self):
+        call("rtpg_sdtree_clear", self.h)
+
+    def SetSplit(self, mode=0, c=4000.0, max_mb=2000, subdiv_th=0.01):
+        call("rtpg_sdtree_set_split", self.h, mode, c, max_mb, subdiv_th)
+
+    def Reset(self, iteration, samples_per_pass):
+        call("rtpg_sdtree_reset", self.h, iteration, samples_per_pass)
+
+    def Build(self):
+        call("rtpg_sdtree_build", self.h)
+
+    def Record(self, pos, canon, irradiance, stat_weight):
+        p, c, i, w = _f32(pos, (-1, 3)), _f32(canon, (-1, 2)), _f32(irradiance), _f32(stat_weight)
+        call("rtpg_sdtree_record", self.h, ptr(p), ptr(c), ptr(i), ptr(w), p.shape[0])
+
+    def Lookup(self, pos):
+        p = _f32(pos, (-1, 3))
+        out = np.zeros(p.shape[0], np.uint32)
+        call("rtpg_sdtree_lookup", self.h, ptr(p), p.shape[0], ptr(out, u32))
+        return out
+
+    def Sample(self, dtree_idx, seed):
+        d = _u32(dtree_idx)
+        out = np.zeros((d.shape[0], 2), np.float32)
+        call("rtpg_sdtree_sample", self.h, ptr(d, u32), d.shape[0], seed, ptr(out))
+        return out
+
+    def Pdf(self, dtree_idx, canon):
+        d, c = _u32(dtree_idx), _f32(canon, (-1, 2))
+        out = np.zeros(d.shape[0], np.float32)
+        call("rtpg_sdtree_pdf", self.h, ptr(d, u32), ptr(c), d.shape[0], ptr(out))
+        return out
+
+    def Sizes(self):
+        s = np.zeros(4, np.uint32)
+        call("rtpg_sdtree_sizes", self.h, ptr(s, u32))
+        return {"stree_nodes": int(s[0]), "dtrees": int(s[1]), "building_nodes": int(s[2]),
+                "sampling_nodes": int(s[3])}
+
+    def Download(self):
+        """Flattened state (reference Download/Upload layout, see rtpg_sdtree_export)."""
+        sz = self.Sizes()
+        n, bn, sn = sz["dtrees"], sz["building_nodes"], sz["sampling_nodes"]
+        d = {
+            "stree": np.zeros(sz["stree_nodes"], np.uint32),
+            "b_cnt": np.zeros(n, np.uint32),
+            "s_cnt": np.zeros(n, np.uint32),
+            "scal": np.zeros((6, n), np.float32),
+            "b_sums": np.zeros((bn, 4), np.float32),
+            "b_child": np.zeros((bn, 4), np.uint16),
+            "s_sums": np.zeros((sn, 4), np.float32),
+            "s_child": np.zeros((sn, 4), np.uint16),
+        }
+        call("rtpg_sdtree_export", self.h, ptr(d["stree"], u32), ptr(d["b_cnt"], u32), ptr(d["s_cnt"], u32),
+             ptr(d["scal"]), ptr(d["b_sums"]), ptr(d["b_child"], u16), ptr(d["s_sums"]), ptr(d["s_child"], u16))
+        return d
+
+    def Upload(self, d):
+        st, bc, sc = _u32(d["stree"]), _u32(d["b_cnt"]), _u32(d["s_cnt"])
+        scal = _f32(d["scal"])
+        bs, ss = _f32(d["b_sums"]), _f32(d["s_sums"])
+        bch = np.ascontiguousarray(d["b_child"], np.uint16)
+        sch = np.ascontiguousarray(d["s_child"], np.uint16)
+        call("rtpg_sdtree_import", self.h, st.shape[0], ptr(st, u32), bc.shape[0], ptr(bc, u32), ptr(sc, u32),
+             ptr(scal), ptr(bs), ptr(bch, u16), ptr(ss), ptr(sch, u16))
+
+
+class _Ctrl:
+    prefix = ""
+
+    def Start(self):
+        call(self.prefix + "start", self.h)
+
+    def BegTrace(self):
+        call(self.prefix + "beg_trace", self.h)
+
+    def EndTrace(self, samples=0):
+        call(self.prefix + "end_trace", self.h, samples)
+
+    def _q(self):
+        q = np.zeros(6, np.uint32)
+        call(self.prefix + "query", self.h, ptr(q, u32))
+        return q
+
+    def GetState(self):
+        return int(self._q()[0])
+
+    def GetIteration(self):
+        return int(self._q()[1])
+
+    def GetSamplePerTmp(self):
+        return int(self._q()[2])
+
+    def GetSampleForPass(self):
+        return int(self._q()[3])
+
+    def IsExecuting(self):
+        return bool(self._q()[4])
+
+    def __del__(self):
+        try:
+            if self.h:
+                call(self.prefix + "destroy", self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class RTSTreeController(_Ctrl):
+    """RTSTreeControllerT: SD-tree pass schedule and state machine."""
+
+    prefix = "rtpg_sdtree_ctrl_"
+
+    def __init__(self, tree: RTSTreeWrapper, sample_for_budget, iteration_for_built=0, ratio_for_budget=0.5,
+                 sample_per_launch=1):
+        h = C.c_void_p()
+        call("rtpg_sdtree_ctrl_create", tree.h, sample_for_budget, iteration_for_built, ratio_for_budget,
+             sample_per_launch, C.byref(h))
+        self.h = h
+        self.tree = tree
+
+
+class DoubleBufferedHashGrid3Buffer:
+    """Spatial hash with current/previous checksum tables."""
+
+    def __init__(self, ctx: Context, aabb_min, aabb_max, bounds=(128, 64, 128), slots=65536, key_mode=0):
+        h = C.c_void_p()
+        lo, hi, b = _f32(aabb_min, (3,)), _f32(aabb_max, (3,)), _u32(bounds)
+        call("rtpg_hashgrid_create", ctx.h, ptr(lo), ptr(hi), ptr(b, u32), slots, key_mode, C.byref(h))
+        self.h = h
+        self.slots = slots
+
+    def Clear(self):
+        call("rtpg_hashgrid_clear", self.h)
+
+    def Update(self):
+        call("rtpg_hashgrid_update", self.h)
+
+    def Find(self, pos, insert=False):
+        p = _f32(pos, (-1, 3))
+        out = np.zeros(p.shape[0], np.uint32)
+        call("rtpg_hashgrid_find", self.h, ptr(p), p.shape[0], 1 if insert else 0, ptr(out, u32))
+        return out
+
+    def Download(self, which=0):
+        out = np.zeros(self.slots, np.uint32)
+        call("rtpg_hashgrid_download", self.h, which, ptr(out, u32))
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                call("rtpg_hashgrid_destroy", self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class RTMortonQuadTreeWrapper:
+    """Per-slot Morton-ordered directional quadtree pyramids (building/sampling)."""
+
+    def __init__(self, ctx: Context, slots, level, fraction=0.3):
+        h = C.c_void_p()
+        call("rtpg_mqtree_create", ctx.h, slots, level, fraction, C.byref(h))
+        self.h = h
+        self.slots, self.level, self.fraction = slots, level, fraction
+        self.per_cell = ((1 << (2 * (level + 1))) - 1) // 3
+
+    def Allocate(self):
+        call("rtpg_mqtree_allocate", self.h)
+
+    def Update(self):
+        call("rtpg_mqtree_update", self.h)
+
+    def LaunchBuildKernel(self):
+        call("rtpg_mqtree_build", self.h)
+
+    def Record(self, slot, canon, value):
+        s, c, v = _u32(slot), _f32(canon, (-1, 2)), _f32(value)
+        call("rtpg_mqtree_record", self.h, ptr(s, u32), ptr(c), ptr(v), s.shape[0])
+
+    def SampleAndPdf(self, slot, seed):
+        s = _u32(slot)
+        out = np.zeros((s.shape[0], 3), np.float32)
+        call("rtpg_mqtree_sample", self.h, ptr(s, u32), s.shape[0], seed, ptr(out))
+        return out
+
+    def Pdf(self, slot, canon):
+        s, c = _u32(slot), _f32(canon, (-1, 2))
+        out = np.zeros(s.shape[0], np.float32)
+        call("rtpg_mqtree_pdf", self.h, ptr(s, u32), ptr(c), s.shape[0], ptr(out))
+        return out
+
+    def Download(self, which=0):
+        out = np.zeros((self.slots, self.per_cell), np.float32)
+        call("rtpg_mqtree_download", self.h, which, ptr(out))
+        return out
+
+    def Upload(self, weights, which=0):
+        w = _f32(weights, (self.slots, self.per_cell))
+        call("rtpg_mqtree_upload", self.h, which, ptr(w))
+
+    def __del__(self):
+        try:
+            if self.h:
+                call("rtpg_mqtree_destroy", self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class RTMortonQuadTreeController(_Ctrl):
+    """RTMortonQuadTreeControllerT: hash-grid guide pass schedule (with Locate)."""
+
+    prefix = "rtpg_mqtree_ctrl_"
+    TraceStateRecord, TraceStateRecordAndSample, TraceStateSample, TraceStateLocate = 0, 1, 2, 3
+
+    def __init__(self, tree: RTMortonQuadTreeWrapper, grid: DoubleBufferedHashGrid3Buffer, sample_for_budget,
+                 iteration_for_built=3, iteration_for_locate=0, ratio_for_budget=0.5, sample_per_launch=1):
+        h = C.c_void_p()
+        call("rtpg_mqtree_ctrl_create", tree.h, grid.h if grid is not None else None, sample_for_budget,
+             iteration_for_built, iteration_for_locate, ratio_for_budget, sample_per_launch, C.byref(h))
+        self.h = h
+        self.tree, self.grid = tree, grid
+
+
+def guide_flags(state: int, base: int, kind: str) -> int:
+    """Render flags for a controller state (reference TracePipeline,
+    RTLibExtOPX7TestApplication.cpp:2155-2231: Build/Trace/Final/Locate)."""
+    f = base | (FLAG_USE_TREE if kind == "tree" else FLAG_USE_GRID)
+    if state == 1:
+        f |= FLAG_GUIDE_SAMPLE
+    elif state == 2:
+        f |= FLAG_GUIDE_SAMPLE | FLAG_FINAL
+    elif state == 3:
+        f |= FLAG_LOCATE
+    return f
+
+
+def render_guided(ctx: Context, ctrl, guide, kind="tree", base_flags=FLAG_NEE | FLAG_MIS | FLAG_RR, max_depth=10,
+                  rr_depth=3, seed=0, fraction=0.3, grid=None, max_paths=0, on_pass=None):
+    """Runs the controller's whole budget, one render call per pass remainder
+    (the reference launches samplesPerLaunch at a time; a pass is rendered here
+    in one call).  Returns the number of samples traced."""
+    ctrl.Start()
+    traced = 0
+    while True:
+        ctrl.BegTrace()
+        if not ctrl.IsExecuting():
+            break
+        q = ctrl._q()
+        state, per_tmp, pass_spp = int(q[0]), int(q[2]), int(q[3])
+        if pass_spp == 0:
+            # The reference controller stays executing for one extra launch
+            # after the budget is spent (EndTrace tests per_all > budget);
+            # that launch is skipped here.
+            break
+        spp = pass_spp - per_tmp
+        flags = guide_flags(state, base_flags, kind)
+        if kind == "tree":
+            ctx.render(spp=spp, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed,
+                       sample_offset=traced, fraction=fraction, sdtree=guide, max_paths=max_paths)
+        else:
+            ctx.render(spp=spp, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed,
+                       sample_offset=traced, fraction=fraction, hashgrid=grid, mqtree=guide, max_paths=max_paths)
+        traced += spp
+        ctrl.EndTrace(spp)
+        if on_pass is not None:
+            on_pass(state, spp, traced)
+    return traced

@@ -1,0 +1,48 @@
+"""Pins the oracle's integrator: a closed-form furnace, and agreement of
+independent estimators of the same image (BSDF sampling only, NEE, NEE+MIS,
+RIS), which a wrong pdf or MIS weight would bias apart."""
+import numpy as np
+import pytest
+
+from rtlib_amd import scenes
+
+NEE, RIS, MIS, RR = 1, 2, 128, 256
+
+
+def _render(orc, sc, spp, flags, seed=0, w=None, h=None, **kw):
+    w = w or sc["camera"]["width"]
+    h = h or sc["camera"]["height"]
+    cam = sc["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], w / h)
+    s = orc.Scene(sc)
+    acc, rays = orc.render(s, uvw, w, h, spp, flags, seed=seed, **kw)
+    return acc / spp, rays
+
+
+def test_lambert_furnace_closed_form(orc):
+    # Convex Lambertian sphere (albedo a) under uniform background 1: a path
+    # that hits the sphere scatters once and escapes, so L = a exactly; others see 1.
+    sc = scenes.furnace_sphere(albedo=0.5, width=24, height=24)
+    img, _ = _render(orc, sc, 4, NEE | MIS, bg=(1.0, 1.0, 1.0), max_depth=4)
+    v = img[..., 0].ravel()
+    on = np.abs(v - 0.5) < 1e-5
+    off = np.abs(v - 1.0) < 1e-6
+    partial = ~(on | off)  # silhouette pixels mix both
+    assert on.sum() > 100 and off.sum() > 100
+    assert np.all((v[partial] > 0.5 - 1e-5) & (v[partial] < 1.0 + 1e-6))
+
+
+def test_ggx_furnace_energy_bounds(orc):
+    # GGX with F0 = 1 never reflects more than it receives and loses little at low roughness.
+    sc = scenes.furnace_sphere(albedo=1.0, mat=scenes.MAT_GGX, roughness=0.1, width=16, height=16)
+    img, _ = _render(orc, sc, 64, 0, bg=(1.0, 1.0, 1.0), max_depth=2)
+    center = img[6:10, 6:10, 0]
+    assert np.all(center <= 1.0 + 0.02) and center.mean() > 0.9
+
+
+@pytest.mark.parametrize("flags,rtol", [(RR, 0.05), (NEE, 0.02), (NEE | RIS, 0.02)])
+def test_cornell_estimators_agree(orc, flags, rtol):
+    sc = scenes.cornell(width=24, height=24)
+    ref = np.mean([_render(orc, sc, 128, NEE | MIS | RR, seed=s)[0].mean() for s in range(2)])
+    alt = np.mean([_render(orc, sc, 128, flags, seed=100 + s, num_candidates=8)[0].mean() for s in range(2)])
+    assert abs(alt - ref) / ref < rtol, (alt, ref)

@@ -125,6 +125,14 @@ typedef struct rtpg_render_params {
 int rtpg_render(rtpg_context ctx, const rtpg_render_params* params);
 /* Last render's counters: [0] paths, [1] radiance rays, [2] shadow rays. */
 int rtpg_render_stats(rtpg_context ctx, uint64_t stats[3]);
+/* Profiling of later renders (no reference counterpart; the reference only
+ * times whole frames, RTLibExtOPX7TestApplication.cpp:2527-2532).
+ * mode bit 0: time each kernel class with HIP events on the render stream;
+ * bit 1: count BVH nodes fetched and triangles tested. */
+int rtpg_render_profile(rtpg_context ctx, uint32_t mode);
+/* ms[5]: extend, shade, shadow, accumulate, commit of the last render;
+ * work[4]: extend nodes, extend triangles, shadow nodes, shadow triangles. */
+int rtpg_render_kernel_stats(rtpg_context ctx, double ms[5], uint64_t work[4]);
 
 /* --- ray queries (the reference's TraceRadiance / TraceOccluded,
  * SimpleKernel.h:347-366) ----------------------------------------------------

@@ -53,6 +53,8 @@ SIGNATURES = {
     "rtpg_accum_download": [c_vp, c_fp, c_u32p],
     "rtpg_render": [c_vp, C.POINTER(RenderParams)],
     "rtpg_render_stats": [c_vp, c_u64p],
+    "rtpg_render_profile": [c_vp, C.c_uint32],
+    "rtpg_render_kernel_stats": [c_vp, C.POINTER(C.c_double), c_u64p],
     "rtpg_trace_closest": [c_vp, c_fp, C.c_uint32, c_fp],
     "rtpg_trace_occluded": [c_vp, c_fp, C.c_uint32, c_u32p],
     "rtpg_sdtree_create": [c_vp, c_fp, c_fp, C.c_uint32, C.c_float, C.POINTER(c_vp)],

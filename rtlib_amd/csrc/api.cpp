@@ -342,6 +342,17 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     if (guide == 2) guide_views(prm, ctx, hg, mq);
     hipStream_t s = ctx->stream;
     ctx->p_stats.zero(4, s);
+    const bool timing = (ctx->prof_mode & 1u) != 0;
+    const bool counting = (ctx->prof_mode & 2u) != 0;
+    if (timing && ctx->events.empty()) {
+      ctx->events.resize((size_t)kTimedKernels * kMaxDepthEv * 2);
+      for (hipEvent_t& e : ctx->events) RTPG_HIP(hipEventCreate(&e));
+    }
+    for (double& m : ctx->kernel_ms) m = 0.0;
+    if (counting) {
+      ctx->p_work.reserve(4);
+      ctx->p_work.zero(4, s);
+    }
     uint32_t done = 0;
     while (done < prm->spp) {
       uint32_t cs = std::min(chunk_spp, prm->spp - done);
@@ -366,6 +377,7 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       ps.vA = guide ? ctx->p_vA.p : nullptr;
       ps.vB = guide ? ctx->p_vB.p : nullptr;
       ps.vC = guide ? ctx->p_vC.p : nullptr;
+      ps.work = counting ? ctx->p_work.p : nullptr;
       RenderParams rp{};
       rp.width = ctx->width;
       rp.height = ctx->height;
@@ -380,9 +392,27 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       rp.bg[1] = prm->bg[1];
       rp.bg[2] = prm->bg[2];
       rp.fraction = prm->fraction;
-      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, guide, ctx->accum.p, s));
+      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, guide, ctx->accum.p, s,
+                                   timing ? ctx->events.data() : nullptr));
       RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));
+      if (timing) {
+        RTPG_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < kTimedKernels; ++k) {
+          uint32_t nd = (k <= 2) ? prm->max_depth : 1u;
+          for (uint32_t d = 0; d < nd; ++d) {
+            float ms = 0.0f;
+            RTPG_HIP(hipEventElapsedTime(&ms, ctx->events[((size_t)k * kMaxDepthEv + d) * 2],
+                                         ctx->events[((size_t)k * kMaxDepthEv + d) * 2 + 1]));
+            ctx->kernel_ms[k] += ms;
+          }
+        }
+      }
       done += cs;
+    }
+    if (counting) {
+      unsigned long long w[4];
+      ctx->p_work.download(w, 4, s);
+      for (int k = 0; k < 4; ++k) ctx->work[k] = w[k];
     }
     ctx->accum_spp += prm->spp;
     unsigned long long st[4];
@@ -393,6 +423,22 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     ctx->last_stats[0] = st[0];
     ctx->last_stats[1] = st[1];
     ctx->last_stats[2] = st[2];
+  });
+}
+
+extern "C" int rtpg_render_profile(rtpg_context ctx, uint32_t mode) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    require(mode <= 3u, "mode must be in [0, 3]");
+    ctx->prof_mode = mode;
+  });
+}
+
+extern "C" int rtpg_render_kernel_stats(rtpg_context ctx, double ms[5], uint64_t work[4]) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    if (ms) for (int k = 0; k < kTimedKernels; ++k) ms[k] = ctx->kernel_ms[k];
+    if (work) for (int k = 0; k < 4; ++k) work[k] = ctx->work[k];
   });
 }
 

@@ -39,6 +39,15 @@ struct rtpg_context_t {
   rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err;
   rtpg::DBuf<unsigned long long> p_stats;
   uint64_t last_stats[3] = {0, 0, 0};
+  // profiling: bit 0 times kernel classes with events, bit 1 counts traversal work
+  uint32_t prof_mode = 0;
+  std::vector<hipEvent_t> events;
+  double kernel_ms[rtpg::kTimedKernels] = {0, 0, 0, 0, 0};
+  rtpg::DBuf<unsigned long long> p_work;
+  uint64_t work[4] = {0, 0, 0, 0};
+  ~rtpg_context_t() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
 
   rtpg::SceneView scene_view() const;
 };

@@ -39,13 +39,19 @@ struct PathSoA {
   float4* vA;        // throughput after the bounce, woPdf
   float4* vB;        // contribution at this hit, cos of the sampled direction
   float4* vC;        // canonical direction (u, v), cell, flags
+  // traversal work counters (null unless counting): extend nodes, extend
+  // triangles, shadow nodes, shadow triangles
+  unsigned long long* work;
 };
+
+constexpr int kTimedKernels = 5;   // extend, shade, shadow, accumulate, commit
+constexpr int kMaxDepthEv = 64;
 
 hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n, float4* hits, uint32_t* occl,
                              bool any, uint32_t* deep, uint32_t* err, hipStream_t s);
 
 hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
                                const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, int guide,
-                               float* accum, hipStream_t s);
+                               float* accum, hipStream_t s, hipEvent_t* ev = nullptr);
 
 }  // namespace rtpg

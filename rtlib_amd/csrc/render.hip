@@ -98,9 +98,12 @@ __device__ __forceinline__ uint32_t stack_pop(const StackRef& s, uint32_t& sp) {
 // BVH4 traversal: closest hit (kAny = false) or any hit (kAny = true).
 // Leaf children are intersected when their parent is visited; inner children
 // are visited nearest-first, the others pushed far-to-near.
-template <bool kAny>
+// kCount adds the nodes fetched and triangles tested to cnt[0], cnt[1]
+// (the algorithmic-byte accounting of the roofline).
+template <bool kAny, bool kCount = false>
 __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax, const StackRef& st,
-                         uint32_t* err, float& t_out, float& u_out, float& v_out, int32_t& tri_out) {
+                         uint32_t* err, float& t_out, float& u_out, float& v_out, int32_t& tri_out,
+                         uint32_t* wcnt = nullptr) {
   const float eps = 1e-20f;
   V3 dd = V3{fabsf(d.x) > eps ? d.x : copysignf(eps, d.x), fabsf(d.y) > eps ? d.y : copysignf(eps, d.y),
              fabsf(d.z) > eps ? d.z : copysignf(eps, d.z)};
@@ -112,6 +115,7 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
   uint32_t sp = 0;
   int32_t node = 0;
   for (;;) {
+    if (kCount) ++wcnt[0];
     const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
     float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
     int4 ch = reinterpret_cast<const int4*>(np)[6];
@@ -137,6 +141,7 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
         if (c < 0) {
           uint32_t first = (uint32_t)(~c);
           for (uint32_t k = 0; k < cnt; ++k) {
+            if (kCount) ++wcnt[1];
             if (isect_tri(sc.tris + first + k, o, d, tmin, tbest, bu, bv)) {
               best = (int32_t)(first + k);
               if (kAny) {
@@ -229,18 +234,36 @@ __global__ void __launch_bounds__(kBlock) k_camera(RenderParams rp, Camera cam, 
   ps.queue[0][pid] = pid;
 }
 
+// Adds per-lane work counters into 64-bit totals, one atomic per wave.
+__device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t a, uint32_t b) {
+  unsigned long long x = a, y = b;
+  for (int off = 32; off >= 1; off >>= 1) {
+    x += __shfl_xor(x, off);
+    y += __shfl_xor(y, off);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(dst, x);
+    atomicAdd(dst + 1, y);
+  }
+}
+
+template <bool kCount>
 __global__ void __launch_bounds__(kBlock) k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
   __shared__ uint32_t lds[kShortStack][kBlock];
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t n_act = ps.count[depth];
-  if (i >= n_act) return;
-  uint32_t pid = ps.queue[depth & 1][i];
-  float4 a = ps.ray_o[pid], b = ps.ray_d[pid];
-  StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
-  float t, u, v;
-  int32_t tri;
-  traverse<false>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, kRayTmax, st, ps.err, t, u, v, tri);
-  ps.hit[pid] = make_float4(t, u, v, bitsf((uint32_t)tri));
+  uint32_t cnt[2] = {0u, 0u};
+  if (i < n_act) {
+    uint32_t pid = ps.queue[depth & 1][i];
+    float4 a = ps.ray_o[pid], b = ps.ray_d[pid];
+    StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
+    float t, u, v;
+    int32_t tri;
+    traverse<false, kCount>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, kRayTmax, st, ps.err, t, u, v, tri,
+                            cnt);
+    ps.hit[pid] = make_float4(t, u, v, bitsf((uint32_t)tri));
+  }
+  if (kCount && ((blockIdx.x * kBlock) < n_act)) wave_count(ps.work, cnt[0], cnt[1]);
 }
 
 __device__ __forceinline__ uint32_t pick_light(const SceneView& sc, float u) {
@@ -547,17 +570,27 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc,
   }
 }
 
+template <bool kCount>
 __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
   __shared__ uint32_t lds[kShortStack][kBlock];
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= ps.scount[depth]) return;
-  const float4* e = ps.shadow + 3u * (size_t)i;
-  float4 a = e[0], b = e[1], c = e[2];
-  uint32_t pid = fbits(b.w);
-  StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
-  float t, u, v;
-  int32_t tri;
-  bool occ = traverse<true>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, a.w, st, ps.err, t, u, v, tri);
+  uint32_t n_sh = ps.scount[depth];
+  uint32_t cnt[2] = {0u, 0u};
+  bool occ = true;
+  uint32_t pid = 0;
+  float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i < n_sh) {
+    const float4* e = ps.shadow + 3u * (size_t)i;
+    float4 a = e[0], b = e[1];
+    c = e[2];
+    pid = fbits(b.w);
+    StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
+    float t, u, v;
+    int32_t tri;
+    occ = traverse<true, kCount>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, a.w, st, ps.err, t, u, v, tri,
+                                 cnt);
+  }
+  if (kCount && ((blockIdx.x * kBlock) < n_sh)) wave_count(ps.work + 2, cnt[0], cnt[1]);
   if (!occ) {
     float4 L4 = ps.L[pid];
     ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
@@ -730,7 +763,7 @@ hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n
 
 hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
                                const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, int guide,
-                               float* accum, hipStream_t s) {
+                               float* accum, hipStream_t s, hipEvent_t* ev) {
   uint32_t nb = blocks_for(ps.n, kBlock);
   uint32_t ncnt = 2u * (rp.max_depth + 2u);
   hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(ncnt, 64)), dim3(64), 0, s, ps.count, ncnt);
@@ -738,8 +771,21 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
   // count[0] = n: written by a one-thread kernel to stay stream ordered.
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, ps.count, ps.n);
   const bool record = guide != 0 && !(rp.flags & kFlagFinal);
+  const bool count = ps.work != nullptr;
+  // ev: per kernel class k in [0, kTimedKernels): pairs of events (begin, end)
+  // per depth; class 0 extend, 1 shade, 2 shadow, 3 accumulate, 4 commit.
+  auto mark = [&](int k, uint32_t d, int end) {
+    if (ev) (void)hipEventRecord(ev[(k * kMaxDepthEv + d) * 2 + end], s);
+  };
   for (uint32_t d = 0; d < rp.max_depth; ++d) {
-    hipLaunchKernelGGL(k_extend, dim3(nb), dim3(kBlock), 0, s, sc, ps, d);
+    mark(0, d, 0);
+    if (count) {
+      hipLaunchKernelGGL(k_extend<true>, dim3(nb), dim3(kBlock), 0, s, sc, ps, d);
+    } else {
+      hipLaunchKernelGGL(k_extend<false>, dim3(nb), dim3(kBlock), 0, s, sc, ps, d);
+    }
+    mark(0, d, 1);
+    mark(1, d, 0);
     if (guide == 1) {
       hipLaunchKernelGGL(k_shade<1>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     } else if (guide == 2) {
@@ -747,15 +793,26 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     } else {
       hipLaunchKernelGGL(k_shade<0>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     }
-    hipLaunchKernelGGL(k_shadow, dim3(nb), dim3(kBlock), 0, s, sc, ps, d, record ? 1u : 0u);
+    mark(1, d, 1);
+    mark(2, d, 0);
+    if (count) {
+      hipLaunchKernelGGL(k_shadow<true>, dim3(nb), dim3(kBlock), 0, s, sc, ps, d, record ? 1u : 0u);
+    } else {
+      hipLaunchKernelGGL(k_shadow<false>, dim3(nb), dim3(kBlock), 0, s, sc, ps, d, record ? 1u : 0u);
+    }
+    mark(2, d, 1);
   }
   uint32_t npix = rp.width * rp.height;
+  mark(3, 0, 0);
   hipLaunchKernelGGL(k_accum, dim3(blocks_for(npix, 256)), dim3(256), 0, s, ps, npix, rp.spp, accum);
+  mark(3, 0, 1);
+  mark(4, 0, 0);
   if (record && guide == 1) {
     hipLaunchKernelGGL(k_commit_sdtree, dim3(nb), dim3(kBlock), 0, s, ps, sd, rp.max_depth);
   } else if (record && guide == 2 && !(rp.flags & kFlagLocate)) {
     hipLaunchKernelGGL(k_commit_morton, dim3(nb), dim3(kBlock), 0, s, ps, mq, rp.max_depth);
   }
+  mark(4, 0, 1);
   return hipGetLastError();
 }
 

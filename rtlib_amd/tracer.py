@@ -150,6 +150,18 @@ class Context:
         call("rtpg_render_stats", self.h, ptr(s, C.c_uint64))
         return {"paths": int(s[0]), "rays": int(s[1]), "shadow_rays": int(s[2])}
 
+    def set_profile(self, time_kernels=False, count_work=False):
+        call("rtpg_render_profile", self.h, (1 if time_kernels else 0) | (2 if count_work else 0))
+
+    def kernel_stats(self):
+        ms = np.zeros(5, np.float64)
+        work = np.zeros(4, np.uint64)
+        call("rtpg_render_kernel_stats", self.h, ptr(ms, C.c_double), ptr(work, C.c_uint64))
+        names = ["extend", "shade", "shadow", "accumulate", "commit"]
+        return ({n: float(x) for n, x in zip(names, ms)},
+                {"extend_nodes": int(work[0]), "extend_tris": int(work[1]), "shadow_nodes": int(work[2]),
+                 "shadow_tris": int(work[3])})
+
     # -- ray queries
     def trace_closest(self, rays):
         r = _f32(rays, (-1, 8))

@@ -24,9 +24,10 @@ def _check(ctx, orc, sc, rays):
     t, u, v, prim = ctx.trace_closest(rays)
     bt, bu, bv, bprim = orc.brute_closest(sc["verts"], sc["idx"], rays)
     same = prim == bprim
-    # Rays through a shared edge can report either triangle at the same t.
+    # Rays through a shared edge, or onto coplanar faces (a box standing on the
+    # floor), can report either triangle at the same t.
     assert np.all(t[~same] == bt[~same])
-    assert same.mean() > 0.999
+    assert same.mean() > 0.99
     assert np.array_equal(t, bt)
     assert np.array_equal(u[same], bu[same]) and np.array_equal(v[same], bv[same])
     occ = ctx.trace_occluded(np.concatenate([rays[:, :7], np.minimum(bt, 1e20)[:, None] * 0.5], 1))

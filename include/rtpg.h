@@ -134,6 +134,22 @@ int rtpg_render_profile(rtpg_context ctx, uint32_t mode);
  * work[4]: extend nodes, extend triangles, shadow nodes, shadow triangles. */
 int rtpg_render_kernel_stats(rtpg_context ctx, double ms[5], uint64_t work[4]);
 
+/* --- path-replay adjoint ------------------------------------------------------
+ * No counterpart in the reference (its integrator is forward only).  For the
+ * image estimate I that rtpg_render(params) would produce (same spp, seed,
+ * sample offset), adds dLoss/dtheta = sum_pixels dl_dimage . dI/dtheta to
+ * grad_albedo (3*nt, original triangle order; Lambertian triangles only) and
+ * grad_emission (3 per light, light order of rtpg_scene_lights).  Paths are
+ * traced twice with the same random streams (primal, then replay), so memory
+ * is O(paths in flight), not O(path length).  Requires no guide, no RIS and no
+ * Russian roulette; albedo channels equal to 0 receive no gradient. */
+int rtpg_adjoint(rtpg_context ctx, const rtpg_render_params* params, const float* dl_dimage, float* grad_albedo,
+                 float* grad_emission);
+/* Replace the radiance of every light (3 per light, light order). */
+int rtpg_scene_set_emission(rtpg_context ctx, const float* le_rgb);
+/* Original triangle index of every light (emissive triangles of positive area). */
+int rtpg_scene_lights(rtpg_context ctx, uint32_t* light_tri);
+
 /* --- ray queries (the reference's TraceRadiance / TraceOccluded,
  * SimpleKernel.h:347-366) ----------------------------------------------------
  * rays: 8*n floats (ox, oy, oz, tmin, dx, dy, dz, tmax).

@@ -1165,13 +1165,28 @@ static uint32_t pick_light(const orc_scene* s, float u) {
 typedef struct {
   v3 p, n, le;
   float area, pdf;
+  uint32_t idx;
 } lsample;
+
+/* Per-hit terms of one path for the adjoint: emission and NEE contributions
+ * with their lights, and the Lambertian triangle that scattered at each hit. */
+typedef struct {
+  uint32_t nh;
+  v3 e[64];
+  int32_t el[64];
+  v3 n[64];
+  int32_t nl[64];
+  int32_t tri[64];
+  v3 rho[64];
+} adjrec;
 
 static lsample sample_light(const orc_scene* s, pcg32* r) {
   float ul = rnd(r), u1 = rnd(r), u2 = rnd(r);
-  const float* L = s->light + 15 * pick_light(s, ul);
+  uint32_t li = pick_light(s, ul);
+  const float* L = s->light + 15 * li;
   float lo = fminf(u1, u2), hi = fmaxf(u1, u2);
   lsample ls;
+  ls.idx = li;
   v3 v0 = ld3(L), e1 = ld3(L + 3), e2 = ld3(L + 6);
   ls.p = add(add(v0, scl(e1, 1.0f - hi)), scl(e2, hi - lo));
   ls.n = nrm(cross(e1, e2));
@@ -1190,7 +1205,7 @@ typedef struct {
 enum { VTX_VALID = 1, VTX_DELTA = 2, VTX_BAD_BSDF = 4 };
 
 static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_sdtree* g, uint32_t pix, uint32_t smp,
-                            v3* Lout) {
+                            v3* Lout, adjrec* ar) {
   uint64_t rays = 0;
   pcg32 r = path_rng(p->seed, pix, p->sample_offset + smp);
   uint32_t x = pix % p->width, y = pix / p->width;
@@ -1222,8 +1237,14 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
     v3 Tn = V(0, 0, 0), nd = V(0, 0, 0), pos = V(0, 0, 0);
     float wo_pdf = 0.0f;
     int next_delta = 0;
+    if (ar) {
+      ar->e[nh] = V(0, 0, 0);
+      ar->n[nh] = V(0, 0, 0);
+      ar->el[nh] = ar->nl[nh] = ar->tri[nh] = -1;
+    }
     if (tri < 0) {
       c = mul(T, ld3(p->bg));
+      if (ar) ar->e[nh] = c;  /* background: no light parameter, but depends on earlier albedos */
     } else {
       const float* q = s->tri + 9 * tri;
       v3 ng = nrm(cross(ld3(q + 3), ld3(q + 6)));
@@ -1245,11 +1266,19 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
             }
           }
           c = scl(mul(T, ld3(Lr + 11)), w);
+          if (ar) {
+            ar->e[nh] = c;
+            ar->el[nh] = s->light_of[tri];
+          }
         }
       } else if (depth + 1u < p->max_depth) {
         uint32_t type = s->mat[tri];
         v3 albedo = ld3(s->albedo + 4 * tri);
         float param = s->albedo[4 * tri + 3];
+        if (ar && type == M_LAMBERT) {
+          ar->tri[nh] = (int32_t)tri;
+          ar->rho[nh] = albedo;
+        }
         v3 spec = V(0, 0, 0);
         if (type == M_PHONG && s->spec) {
           spec = ld3(s->spec + 4 * tri);
@@ -1353,7 +1382,13 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
                 }
                 v3 con = scl(mul(mul(T, be.f), ls.le), cs * w / pl);
                 float tt;
-                if (max3(con) > 0.0f && trace(s, pos, dir, 1e-4f, dist - 1e-4f, 1, &tt, &rays) < 0) c = add(c, con);
+                if (max3(con) > 0.0f && trace(s, pos, dir, 1e-4f, dist - 1e-4f, 1, &tt, &rays) < 0) {
+                  c = add(c, con);
+                  if (ar) {
+                    ar->n[nh] = con;
+                    ar->nl[nh] = (int32_t)ls.idx;
+                  }
+                }
               }
             }
           }
@@ -1437,7 +1472,49 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
     }
   }
   *Lout = L;
+  if (ar) ar->nh = nh;
   return rays;
+}
+
+/* Reference gradient of the path-replay adjoint, accumulated in double:
+ * d/d rho_t = sum over Lambertian visits v of t of (NEE at v + all
+ * contributions after v) / rho; d/d Le_l = contributions of light l / Le_l.
+ * Weighted by dl_dimage / spp per pixel.  RIS and Russian roulette are not
+ * differentiated (as in the GPU adjoint). */
+void orc_adjoint(const orc_scene* s, const orc_params* p, const float* dldi, double* grad_alb, double* grad_le) {
+  adjrec* ar = (adjrec*)malloc(sizeof(adjrec));
+  for (uint32_t pix = p->px_begin; pix < p->px_end; ++pix) {
+    double w[3] = {dldi[3 * pix] / (double)p->spp, dldi[3 * pix + 1] / (double)p->spp,
+                   dldi[3 * pix + 2] / (double)p->spp};
+    for (uint32_t smp = 0; smp < p->spp; ++smp) {
+      v3 L;
+      render_path(s, p, NULL, pix, smp, &L, ar);
+      for (uint32_t k = 0; k < ar->nh; ++k) {
+        const float* cs[2] = {&ar->e[k].x, &ar->n[k].x};
+        int32_t ls[2] = {ar->el[k], ar->nl[k]};
+        for (int q = 0; q < 2; ++q) {
+          if (ls[q] < 0) continue;
+          const float* le = s->light + 15 * ls[q] + 11;
+          for (int ch = 0; ch < 3; ++ch)
+            if (le[ch] > 0.0f) grad_le[3 * ls[q] + ch] += w[ch] * (double)cs[q][ch] / (double)le[ch];
+        }
+        if (ar->tri[k] < 0) continue;
+        double aff[3];
+        aff[0] = ar->n[k].x;
+        aff[1] = ar->n[k].y;
+        aff[2] = ar->n[k].z;
+        for (uint32_t j = k + 1; j < ar->nh; ++j) {
+          aff[0] += (double)ar->e[j].x + (double)ar->n[j].x;
+          aff[1] += (double)ar->e[j].y + (double)ar->n[j].y;
+          aff[2] += (double)ar->e[j].z + (double)ar->n[j].z;
+        }
+        const float* rho = &ar->rho[k].x;
+        for (int ch = 0; ch < 3; ++ch)
+          if (rho[ch] > 0.0f) grad_alb[3 * ar->tri[k] + ch] += w[ch] * aff[ch] / (double)rho[ch];
+      }
+    }
+  }
+  free(ar);
 }
 
 typedef struct {
@@ -1463,7 +1540,7 @@ static void* worker(void* arg) {
       float sx = 0.0f, sy = 0.0f, sz = 0.0f;
       for (uint32_t smp = 0; smp < j->p->spp; ++smp) {
         v3 L;
-        rays += render_path(j->s, j->p, j->g, pix, smp, &L);
+        rays += render_path(j->s, j->p, j->g, pix, smp, &L, NULL);
         sx += L.x;
         sy += L.y;
         sz += L.z;

@@ -123,6 +123,9 @@ typedef struct orc_sdtree {
 } orc_sdtree;
 /* accum: 3*W*H floats, added to; returns the number of radiance rays traced. */
 uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum);
+/* FP64-accumulated gradient of sum_pixels dldi . image w.r.t. per-triangle
+ * albedo (3*nt, Lambertian triangles) and per-light radiance (3*nl). */
+void orc_adjoint(const orc_scene* s, const orc_params* p, const float* dldi, double* grad_alb, double* grad_le);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,8 @@ def lib():
         L.orc_scene_destroy.argtypes = [C.c_void_p]
         L.orc_render.restype = C.c_uint64
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(SdTree), f32p]
+        L.orc_adjoint.argtypes = [C.c_void_p, C.POINTER(Params), f32p, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -353,6 +355,27 @@ def render(scene: Scene, camera_uvw, width, height, spp, flags, max_depth=8, rr_
         gp = C.byref(guide.struct)
     rays = lib().orc_render(scene.h, C.byref(p), gp, _p(accum))
     return accum, rays
+
+
+def adjoint(scene: Scene, camera_uvw, width, height, spp, flags, dldi, n_lights, max_depth=6, seed=0,
+            sample_offset=0, bg=(0, 0, 0)):
+    """FP64 reference gradient (see orc_adjoint): (grad_albedo nt x 3, grad_le nl x 3)."""
+    p = Params()
+    p.width, p.height, p.flags, p.max_depth, p.rr_depth = width, height, flags, max_depth, 1000
+    p.num_candidates, p.spp, p.sample_offset, p.seed = 1, spp, sample_offset, seed
+    for k in range(3):
+        p.bg[k] = bg[k]
+        p.eye[k], p.u[k], p.v[k], p.w[k] = (float(x[k]) for x in camera_uvw)
+    p.fraction = 0.0
+    p.px_begin, p.px_end = 0, width * height
+    p.threads = 1
+    nt = scene._keep[1].shape[0]
+    ga = np.zeros((nt, 3), np.float64)
+    gl = np.zeros((max(n_lights, 1), 3), np.float64)
+    d = _f(dldi, (-1,))
+    lib().orc_adjoint(scene.h, C.byref(p), _p(d), ga.ctypes.data_as(C.POINTER(C.c_double)),
+                      gl.ctypes.data_as(C.POINTER(C.c_double)))
+    return ga, gl[:n_lights]
 
 
 class Guide:

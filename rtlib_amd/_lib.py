@@ -54,6 +54,9 @@ SIGNATURES = {
     "rtpg_render": [c_vp, C.POINTER(RenderParams)],
     "rtpg_render_stats": [c_vp, c_u64p],
     "rtpg_render_profile": [c_vp, C.c_uint32],
+    "rtpg_adjoint": [c_vp, C.POINTER(RenderParams), c_fp, c_fp, c_fp],
+    "rtpg_scene_set_emission": [c_vp, c_fp],
+    "rtpg_scene_lights": [c_vp, c_u32p],
     "rtpg_render_kernel_stats": [c_vp, C.POINTER(C.c_double), c_u64p],
     "rtpg_trace_closest": [c_vp, c_fp, C.c_uint32, c_fp],
     "rtpg_trace_occluded": [c_vp, c_fp, C.c_uint32, c_u32p],

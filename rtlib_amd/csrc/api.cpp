@@ -197,6 +197,8 @@ int rtpg_scene_upload(rtpg_context ctx, const float* verts, uint32_t nv, const u
     ctx->light_cdf.upload(cdf, s);
     RTPG_HIP(hipStreamSynchronize(s));
     ctx->has_spec = spec != nullptr;
+    ctx->lights_host = lights;
+    ctx->light_tri = light_tris;
     ctx->order = b.order;
     ctx->slot_of = slot_of;
     ctx->n_tris = nt;
@@ -313,7 +315,142 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
     c->p_vA.reserve((size_t)max_depth * n);
     c->p_vB.reserve((size_t)max_depth * n);
     c->p_vC.reserve((size_t)max_depth * n);
+    c->p_recs.reserve((size_t)(max_depth > 1 ? max_depth - 1 : 1) * n);
+    c->p_rec_count.reserve(1);
   }
+}
+
+static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, bool guide, bool counting) {
+  ensure_arena(ctx, n, max_depth, guide);
+  PathSoA ps{};
+  ps.n = n;
+  ps.ray_o = ctx->p_ray_o.p;
+  ps.ray_d = ctx->p_ray_d.p;
+  ps.T = ctx->p_T.p;
+  ps.L = ctx->p_L.p;
+  ps.hit = ctx->p_hit.p;
+  ps.rng = ctx->p_rng.p;
+  ps.nhit = ctx->p_nhit.p;
+  ps.queue[0] = ctx->p_q0.p;
+  ps.queue[1] = ctx->p_q1.p;
+  ps.count = ctx->p_count.p;
+  ps.scount = ctx->p_count.p + (max_depth + 2u);
+  ps.shadow = ctx->p_shadow.p;
+  ps.deep = ctx->max_stack > 16u ? ctx->p_deep.p : nullptr;
+  ps.err = ctx->p_err.p;
+  ps.vA = guide ? ctx->p_vA.p : nullptr;
+  ps.vB = guide ? ctx->p_vB.p : nullptr;
+  ps.vC = guide ? ctx->p_vC.p : nullptr;
+  ps.work = counting ? ctx->p_work.p : nullptr;
+  ps.recs = guide ? ctx->p_recs.p : nullptr;
+  ps.rec_count = guide ? ctx->p_rec_count.p : nullptr;
+  ps.npix = ctx->width * ctx->height;
+  return ps;
+}
+
+static RenderParams make_rp(rtpg_context_t* ctx, const rtpg_render_params* prm, uint32_t spp, uint32_t done) {
+  RenderParams rp{};
+  rp.width = ctx->width;
+  rp.height = ctx->height;
+  rp.flags = prm->flags;
+  rp.max_depth = prm->max_depth;
+  rp.rr_depth = prm->rr_depth;
+  rp.num_candidates = prm->num_candidates;
+  rp.spp = spp;
+  rp.sample_offset = prm->sample_offset + done;
+  rp.seed = prm->seed;
+  rp.bg[0] = prm->bg[0];
+  rp.bg[1] = prm->bg[1];
+  rp.bg[2] = prm->bg[2];
+  rp.fraction = prm->fraction;
+  return rp;
+}
+
+extern "C" int rtpg_adjoint(rtpg_context ctx, const rtpg_render_params* prm, const float* dl_dimage,
+                            float* grad_albedo, float* grad_emission) {
+  return guarded([&] {
+    require(ctx && prm && dl_dimage, "NULL argument");
+    require(ctx->n_tris > 0 && ctx->accum.p != nullptr, "no scene or frame");
+    require(prm->max_depth >= 1 && prm->max_depth <= 64, "max_depth must be in [1, 64]");
+    require(prm->spp >= 1, "spp must be >= 1");
+    require(!(prm->flags & (RTPG_FLAG_USE_TREE | RTPG_FLAG_USE_GRID | RTPG_FLAG_USE_MLP)),
+            "the adjoint renders without guides");
+    require(!(prm->flags & RTPG_FLAG_RIS), "the adjoint does not support RIS (its resampling depends on albedo)");
+    require(!(prm->flags & RTPG_FLAG_RR), "the adjoint needs Russian roulette off (its survival depends on albedo)");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    uint32_t npix = ctx->width * ctx->height;
+    uint32_t nl = std::max(ctx->n_lights, 1u);
+    DBuf<float> d_dldi, d_galb, d_gle, scratch;
+    DBuf<float4> d_ltot;
+    d_dldi.upload(dl_dimage, 3 * (size_t)npix, s);
+    d_galb.alloc(3 * (size_t)ctx->n_tris);
+    d_galb.zero(3 * (size_t)ctx->n_tris, s);
+    d_gle.alloc(3 * (size_t)nl);
+    d_gle.zero(3 * (size_t)nl, s);
+    scratch.alloc(3 * (size_t)npix);
+    uint32_t max_paths = prm->max_paths ? prm->max_paths : (1u << 22);
+    uint32_t chunk_spp = std::min(std::max(1u, max_paths / npix), prm->spp);
+    SceneView sc = ctx->scene_view();
+    SdTreeView sd{};
+    HashGridView hg{};
+    MortonView mq{};
+    uint32_t done = 0;
+    while (done < prm->spp) {
+      uint32_t cs = std::min(chunk_spp, prm->spp - done);
+      uint32_t n = cs * npix;
+      PathSoA ps = make_soa(ctx, n, prm->max_depth, false, false);
+      d_ltot.reserve(n);
+      ps.dldi = d_dldi.p;
+      ps.adjL = d_ltot.p;
+      ps.grad_alb = d_galb.p;
+      ps.grad_le = d_gle.p;
+      ps.inv_spp = 1.0f / (float)prm->spp;
+      RenderParams rp = make_rp(ctx, prm, cs, done);
+      // primal pass: the total radiance of every path
+      ps.adj_mode = 0u;
+      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, 0, scratch.p, s));
+      RTPG_HIP(hipMemcpyAsync(d_ltot.p, ps.L, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
+      // replay with the same random streams; scatter the derivatives
+      ps.adj_mode = 1u;
+      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, 0, scratch.p, s));
+      done += cs;
+    }
+    std::vector<float> galb = d_galb.to_vector(3 * (size_t)ctx->n_tris, s);
+    std::vector<float> gle = d_gle.to_vector(3 * (size_t)nl, s);
+    uint32_t err = 0;
+    ctx->p_err.download(&err, 1, s);
+    if (err) throw Error(RTPG_ERR_LIMIT, "traversal stack bound exceeded");
+    if (grad_albedo) {
+      for (uint32_t slot = 0; slot < ctx->n_tris; ++slot) {
+        uint32_t t = ctx->order[slot];
+        for (int k = 0; k < 3; ++k) grad_albedo[3 * (size_t)t + k] += galb[3 * (size_t)slot + k];
+      }
+    }
+    if (grad_emission) {
+      for (uint32_t l = 0; l < ctx->n_lights; ++l)
+        for (int k = 0; k < 3; ++k) grad_emission[3 * (size_t)l + k] += gle[3 * (size_t)l + k];
+    }
+  });
+}
+
+extern "C" int rtpg_scene_set_emission(rtpg_context ctx, const float* le_rgb) {
+  return guarded([&] {
+    require(ctx && le_rgb, "NULL argument");
+    require(ctx->n_lights > 0, "scene has no lights");
+    for (uint32_t l = 0; l < ctx->n_lights; ++l) {
+      ctx->lights_host[4 * (size_t)l + 3] = make_float4(le_rgb[3 * l], le_rgb[3 * l + 1], le_rgb[3 * l + 2], 0.0f);
+    }
+    ctx->lights.upload(ctx->lights_host, ctx->stream);
+    RTPG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int rtpg_scene_lights(rtpg_context ctx, uint32_t* light_tri) {
+  return guarded([&] {
+    require(ctx && light_tri, "NULL argument");
+    for (uint32_t l = 0; l < ctx->n_lights; ++l) light_tri[l] = ctx->light_tri[l];
+  });
 }
 
 extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
@@ -330,6 +467,8 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     require(!use_tree || prm->sdtree != nullptr, "USE_TREE needs an sdtree");
     require(!use_grid || (prm->hashgrid != nullptr && prm->mqtree != nullptr), "USE_GRID needs hashgrid + mqtree");
     int guide = use_tree ? 1 : (use_grid ? 2 : 0);
+    require(guide == 0 || (prm->flags & RTPG_FLAG_FINAL) || prm->max_depth <= 16,
+            "recording guide vertices supports max_depth <= 16");
     uint32_t npix = ctx->width * ctx->height;
     uint32_t max_paths = prm->max_paths ? prm->max_paths : (1u << 22);
     uint32_t chunk_spp = std::max(1u, max_paths / npix);
@@ -357,41 +496,8 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     while (done < prm->spp) {
       uint32_t cs = std::min(chunk_spp, prm->spp - done);
       uint32_t n = cs * npix;
-      ensure_arena(ctx, n, prm->max_depth, guide != 0);
-      PathSoA ps{};
-      ps.n = n;
-      ps.ray_o = ctx->p_ray_o.p;
-      ps.ray_d = ctx->p_ray_d.p;
-      ps.T = ctx->p_T.p;
-      ps.L = ctx->p_L.p;
-      ps.hit = ctx->p_hit.p;
-      ps.rng = ctx->p_rng.p;
-      ps.nhit = ctx->p_nhit.p;
-      ps.queue[0] = ctx->p_q0.p;
-      ps.queue[1] = ctx->p_q1.p;
-      ps.count = ctx->p_count.p;
-      ps.scount = ctx->p_count.p + (prm->max_depth + 2u);
-      ps.shadow = ctx->p_shadow.p;
-      ps.deep = ctx->max_stack > 16u ? ctx->p_deep.p : nullptr;
-      ps.err = ctx->p_err.p;
-      ps.vA = guide ? ctx->p_vA.p : nullptr;
-      ps.vB = guide ? ctx->p_vB.p : nullptr;
-      ps.vC = guide ? ctx->p_vC.p : nullptr;
-      ps.work = counting ? ctx->p_work.p : nullptr;
-      RenderParams rp{};
-      rp.width = ctx->width;
-      rp.height = ctx->height;
-      rp.flags = prm->flags;
-      rp.max_depth = prm->max_depth;
-      rp.rr_depth = prm->rr_depth;
-      rp.num_candidates = prm->num_candidates;
-      rp.spp = cs;
-      rp.sample_offset = prm->sample_offset + done;
-      rp.seed = prm->seed;
-      rp.bg[0] = prm->bg[0];
-      rp.bg[1] = prm->bg[1];
-      rp.bg[2] = prm->bg[2];
-      rp.fraction = prm->fraction;
+      PathSoA ps = make_soa(ctx, n, prm->max_depth, guide != 0, counting);
+      RenderParams rp = make_rp(ctx, prm, cs, done);
       RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, guide, ctx->accum.p, s,
                                    timing ? ctx->events.data() : nullptr));
       RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));

@@ -24,6 +24,8 @@ struct rtpg_context_t {
   std::vector<uint32_t> order;       // leaf slot -> original triangle
   std::vector<uint32_t> slot_of;     // original triangle -> leaf slot
   std::vector<float> shade_host;     // leaf order, 4 per triangle
+  std::vector<float4> lights_host;   // 4 per light, as on the device
+  std::vector<uint32_t> light_tri;   // light -> original triangle
   uint32_t n_tris = 0, n_lights = 0, n_nodes = 0;
   uint32_t bvh_depth = 0, max_stack = 0, max_leaf = 0;
   float world[6] = {0, 0, 0, 0, 0, 0};
@@ -34,9 +36,9 @@ struct rtpg_context_t {
   uint32_t accum_spp = 0;
   rtpg::Camera cam{};
   // path arena
-  rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC;
+  rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC, p_recs;
   rtpg::DBuf<uint2> p_rng;
-  rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err;
+  rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err, p_rec_count;
   rtpg::DBuf<unsigned long long> p_stats;
   uint64_t last_stats[3] = {0, 0, 0};
   // profiling: bit 0 times kernel classes with events, bit 1 counts traversal work

@@ -42,6 +42,17 @@ struct PathSoA {
   // traversal work counters (null unless counting): extend nodes, extend
   // triangles, shadow nodes, shadow triangles
   unsigned long long* work;
+  // compacted guide records (cell, u, v, value) and their count
+  float4* recs;
+  uint32_t* rec_count;
+  // path-replay adjoint (grad_alb non-null selects the adjoint kernels)
+  uint32_t adj_mode;     // 0 primal pass, 1 replay and scatter
+  uint32_t npix;
+  float inv_spp;
+  const float* dldi;     // dLoss/dimage, 3 per pixel
+  const float4* adjL;    // primal radiance per path
+  float* grad_alb;       // 3 per leaf-order triangle
+  float* grad_le;        // 3 per light
 };
 
 constexpr int kTimedKernels = 5;   // extend, shade, shadow, accumulate, commit

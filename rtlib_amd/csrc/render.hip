@@ -278,6 +278,7 @@ __device__ __forceinline__ uint32_t pick_light(const SceneView& sc, float u) {
 struct LightSample {
   V3 p, n, Le;
   float area, pdf_sel;
+  uint32_t idx;
 };
 
 // Uniform point on a light triangle, barycentrics of Random.h:290-296.
@@ -295,19 +296,72 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& sc, Pcg32& 
   s.Le = V3{e.x, e.y, e.z};
   s.area = a.w;
   s.pdf_sel = b.w;
+  s.idx = li;
   return s;
 }
 
-template <int kGuide>
-__global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd,
-                                                  HashGridView hg, MortonView mq, uint32_t depth) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// Path-replay adjoint (no reference counterpart): the radiance still to come
+// after a vertex, L_total - L_so_far, scales with that vertex's Lambertian
+// albedo once per visit, and each emitted or light-sampled contribution scales
+// with its light's radiance.  ps.adj_mode 0 is the primal pass (stores
+// nothing extra); 1 replays the identical paths and scatters dLoss/dparam.
+__device__ __forceinline__ void adj_add_le(const PathSoA& ps, uint32_t li, V3 w, V3 c, V3 le) {
+  if (le.x > 0.0f) unsafeAtomicAdd(ps.grad_le + 3u * li + 0u, w.x * c.x / le.x);
+  if (le.y > 0.0f) unsafeAtomicAdd(ps.grad_le + 3u * li + 1u, w.y * c.y / le.y);
+  if (le.z > 0.0f) unsafeAtomicAdd(ps.grad_le + 3u * li + 2u, w.z * c.z / le.z);
+}
+
+__device__ __forceinline__ V3 adj_weight(const PathSoA& ps, uint32_t pid) {
+  uint32_t pix = pid % ps.npix;
+  return V3{ps.dldi[3u * pix], ps.dldi[3u * pix + 1u], ps.dldi[3u * pix + 2u]} * ps.inv_spp;
+}
+
+// Two queue appends with one global atomic per counter per work-group: the
+// waves' counts are combined in LDS, so the hot counters see 8x fewer
+// same-address atomics than with per-wave appends.
+constexpr uint32_t kShadeBlock = 512;
+constexpr uint32_t kShadeWaves = kShadeBlock / 64;
+
+__device__ __forceinline__ void block_append2(uint32_t* ca, bool pa, uint32_t* cb, bool pb, uint32_t& sa,
+                                              uint32_t& sb) {
+  __shared__ uint32_t s_a[kShadeWaves + 1], s_b[kShadeWaves + 1];
+  uint64_t ma = __ballot(pa), mb = __ballot(pb);
+  uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
+  uint32_t ba = (uint32_t)__popcll(ma & lt), bb = (uint32_t)__popcll(mb & lt);
+  if (lane == 0) {
+    s_a[wave] = (uint32_t)__popcll(ma);
+    s_b[wave] = (uint32_t)__popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ta = 0, tb = 0;
+    for (uint32_t w = 0; w < kShadeWaves; ++w) {
+      uint32_t x = s_a[w], y = s_b[w];
+      s_a[w] = ta;
+      s_b[w] = tb;
+      ta += x;
+      tb += y;
+    }
+    s_a[kShadeWaves] = ta ? atomicAdd(ca, ta) : 0u;
+    s_b[kShadeWaves] = tb ? atomicAdd(cb, tb) : 0u;
+  }
+  __syncthreads();
+  sa = s_a[kShadeWaves] + s_a[wave] + ba;
+  sb = s_b[kShadeWaves] + s_b[wave] + bb;
+}
+
+template <int kGuide, bool kAdj = false>
+__global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd,
+                                                       HashGridView hg, MortonView mq, uint32_t depth) {
+  uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
   uint32_t n_act = ps.count[depth];
   bool active = i < n_act;
   bool cont = false, shadow = false;
   uint32_t pid = 0;
   V3 p{0, 0, 0}, nd{0, 0, 0}, sdir{0, 0, 0}, scon{0, 0, 0}, Tn{0, 0, 0};
   float wo_pdf = 0.0f, sdist = 0.0f;
+  uint32_t slight = 0;
   uint32_t meta_next = 0;
   Pcg32 rng;
   const bool record = (kGuide != 0) && !(rp.flags & kFlagFinal);
@@ -331,6 +385,9 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc,
     uint32_t vcell = kInvalidCell;
     float vcu = 0.0f, vcv = 0.0f;
     uint32_t vflags = 0;
+    const bool adj = kAdj && ps.adj_mode != 0u;
+    V3 adj_w{0.0f, 0.0f, 0.0f};
+    if (adj) adj_w = adj_weight(ps, pid);
     if (tri < 0) {
       c = T * V3{rp.bg[0], rp.bg[1], rp.bg[2]};
     } else {
@@ -359,12 +416,21 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc,
             }
           }
           c = T * V3{e.x, e.y, e.z} * w;
+          if (adj) adj_add_le(ps, li, adj_w, c, V3{e.x, e.y, e.z});
         }
       } else if (depth + 1u < rp.max_depth) {
         float4 sh = sc.shade[tri];
         uint32_t type = tri_type(info);
         V3 albedo{sh.x, sh.y, sh.z};
         float param = sh.w;
+        if (adj && type == kMatLambert) {
+          // This vertex's NEE and everything after it carry one factor of albedo.
+          float4 lt = ps.adjL[pid], lc = ps.L[pid];
+          V3 rest{lt.x - lc.x, lt.y - lc.y, lt.z - lc.z};
+          if (albedo.x > 0.0f) unsafeAtomicAdd(ps.grad_alb + 3u * tri + 0u, adj_w.x * rest.x / albedo.x);
+          if (albedo.y > 0.0f) unsafeAtomicAdd(ps.grad_alb + 3u * tri + 1u, adj_w.y * rest.y / albedo.y);
+          if (albedo.z > 0.0f) unsafeAtomicAdd(ps.grad_alb + 3u * tri + 2u, adj_w.z * rest.z / albedo.z);
+        }
         V3 spec{0.0f, 0.0f, 0.0f};
         if (type == kMatPhong && sc.spec) {
           float4 s4 = sc.spec[tri];
@@ -454,6 +520,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc,
                   fa_y = fa;
                   dir_y = dir;
                   dist_y = dist;
+                  slight = ls.idx;
                 }
               }
               if (wsum > 0.0f && fa_y > 0.0f) {
@@ -488,6 +555,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc,
                 scon = T * be.f * ls.Le * (cos_s * w / pl);
                 sdir = dir;
                 sdist = dist - 1e-4f;
+                slight = ls.idx;
                 shadow = max3(scon) > 0.0f;
               }
             }
@@ -559,18 +627,18 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderParams rp, SceneView sc,
     }
     ps.rng[pid] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
   }
-  uint32_t slot = wave_append(&ps.count[depth + 1u], cont);
+  uint32_t slot, sslot;
+  block_append2(&ps.count[depth + 1u], cont, &ps.scount[depth], shadow, slot, sslot);
   if (cont) ps.queue[(depth + 1u) & 1][slot] = pid;
-  uint32_t sslot = wave_append(&ps.scount[depth], shadow);
   if (shadow) {
     float4* e = ps.shadow + 3u * (size_t)sslot;
     e[0] = make_float4(p.x, p.y, p.z, sdist);
     e[1] = make_float4(sdir.x, sdir.y, sdir.z, bitsf(pid));
-    e[2] = make_float4(scon.x, scon.y, scon.z, 0.0f);
+    e[2] = make_float4(scon.x, scon.y, scon.z, bitsf(slight));
   }
 }
 
-template <bool kCount>
+template <bool kCount, bool kAdj = false>
 __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
   __shared__ uint32_t lds[kShortStack][kBlock];
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -578,11 +646,9 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uin
   uint32_t cnt[2] = {0u, 0u};
   bool occ = true;
   uint32_t pid = 0;
-  float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4* e = ps.shadow + 3u * (size_t)i;
   if (i < n_sh) {
-    const float4* e = ps.shadow + 3u * (size_t)i;
     float4 a = e[0], b = e[1];
-    c = e[2];
     pid = fbits(b.w);
     StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
     float t, u, v;
@@ -592,8 +658,14 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uin
   }
   if (kCount && ((blockIdx.x * kBlock) < n_sh)) wave_count(ps.work + 2, cnt[0], cnt[1]);
   if (!occ) {
+    float4 c = e[2];  // contribution, loaded after the traversal to keep it out of its registers
     float4 L4 = ps.L[pid];
     ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
+    if (kAdj && ps.adj_mode != 0u) {
+      uint32_t li = fbits(c.w);
+      float4 le = sc.lights[4u * li + 3u];
+      adj_add_le(ps, li, adj_weight(ps, pid), V3{c.x, c.y, c.z}, V3{le.x, le.y, le.z});
+    }
     if (record) {
       size_t vi = (size_t)depth * ps.n + pid;
       float4 vb = ps.vB[vi];
@@ -662,39 +734,132 @@ __device__ __forceinline__ bool commit_value(const PathSoA& ps, uint32_t pid, ui
   return true;
 }
 
-// Wave-aggregated float add: lanes sharing an address are summed and one
-// lane issues the atomic.  Returns after all lanes are served.
-__device__ __forceinline__ void wave_add_by_key(float* base, uint32_t key, float val, bool pred) {
-  uint64_t pending = __ballot(pred);
-  while (pending) {
-    int leader = __ffsll((long long)pending) - 1;
-    uint32_t k = __shfl(key, leader);
-    bool mine = pred && key == k && ((pending >> lane_id()) & 1ull);
-    uint64_t same = __ballot(mine);
-    float v = mine ? val : 0.0f;
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    if ((int)lane_id() == leader) unsafeAtomicAdd(base + k, v);
-    pending &= ~same;
-  }
-}
+// Recording runs in two stages.  Stage 1 (one work-item per path) turns the
+// path's vertices into compact records (DTree or hash slot, canonical u, v,
+// value), appended with one atomic per wave.  Stage 2 applies a contiguous run
+// of records per work-group: each record's target (statistical weight of its
+// DTree, flux slot of its leaf) is accumulated in an LDS hash table, and the
+// table is flushed with one global float atomic per distinct address, so hot
+// leaves near the quadtree roots do not serialise on HBM atomics.
+constexpr int kMaxCommitVtx = 16;
+constexpr uint32_t kApplyBlock = 256;
+constexpr uint32_t kApplyPerThread = 16;
+constexpr uint32_t kLeafTab = 4096;
+constexpr uint32_t kCellTab = 1024;
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
 
-// Records every path's vertices into the building SD-tree: statistical
-// weight per DTree with wave aggregation, flux into the first leaf slot on the
-// quadtree descent (DTreeNodeT::Record :113-125, RecordIrradiance :419-427).
-__global__ void __launch_bounds__(kBlock) k_commit_sdtree(PathSoA ps, SdTreeView sd, uint32_t max_vtx) {
+__global__ void __launch_bounds__(kBlock) k_commit_emit(PathSoA ps, uint32_t max_vtx, float4* recs,
+                                                        uint32_t* rec_count) {
   uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
   bool in = pid < ps.n;
   uint32_t nh = in ? ps.nhit[pid] : 0u;
+  nh = nh < (uint32_t)kMaxCommitVtx ? nh : (uint32_t)kMaxCommitVtx;
   uint32_t nv = nh > 0 ? nh - 1u : 0u;
   nv = nv < max_vtx ? nv : max_vtx;
-  uint32_t nv_max = nv;
-  for (int off = 32; off >= 1; off >>= 1) nv_max = max(nv_max, (uint32_t)__shfl_xor((int)nv_max, off));
-  for (uint32_t d = 0; d < nv_max; ++d) {
+  uint32_t nh_max = nh;
+  for (int off = 32; off >= 1; off >>= 1) nh_max = max(nh_max, (uint32_t)__shfl_xor((int)nh_max, off));
+  // TraceVertexT::Commit (OPX7UtilsPathGuiding.h:830-866): the radiance that
+  // reached vertex d is the sum of the contributions of hits d+1 .. nh-1.  It
+  // is accumulated here from the last hit backwards (one running sum instead
+  // of the reference's per-vertex forward sums: same terms, other rounding).
+  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+  for (int d = (int)nh_max - 2; d >= 0; --d) {
+    bool ok = false;
     uint32_t cell = 0;
-    float cu = 0, cv = 0, value = 0;
-    bool ok = (d < nv) && commit_value(ps, pid, d, nh, cell, cu, cv, value);
-    wave_add_by_key(sd.build_sw, cell, 1.0f, ok);
-    if (ok && isfinite(value) && value > 0.0f) {
+    float cu = 0.0f, cv = 0.0f, value = 0.0f;
+    if ((uint32_t)d + 1u < nh) {
+      float4 b = ps.vB[(size_t)(d + 1) * ps.n + pid];
+      rx += b.x;
+      ry += b.y;
+      rz += b.z;
+    }
+    if ((uint32_t)d < nv) {
+      size_t vi = (size_t)d * ps.n + pid;
+      float4 vc = ps.vC[vi];
+      uint32_t fl = fbits(vc.w);
+      cell = fbits(vc.z);
+      if ((fl & kVtxValid) && !(fl & kVtxDelta) && cell != kInvalidCell) {
+        float4 va = ps.vA[vi];
+        float wo_pdf = va.w;
+        bool fin = isfinite(rx) && rx >= 0.0f && isfinite(ry) && ry >= 0.0f && isfinite(rz) && rz >= 0.0f;
+        if (wo_pdf > 0.0f && fin && !(fl & kVtxBadBsdf)) {
+          float lx = va.x * wo_pdf > 1e-4f ? rx / va.x : 0.0f;
+          float ly = va.y * wo_pdf > 1e-4f ? ry / va.y : 0.0f;
+          float lz = va.z * wo_pdf > 1e-4f ? rz / va.z : 0.0f;
+          float cosv = fabsf(ps.vB[vi].w);
+          lx *= cosv;
+          ly *= cosv;
+          lz *= cosv;
+          float avg = (lx + ly + lz) / 3.0f;
+          value = avg / wo_pdf;
+          cu = vc.x;
+          cv = vc.y;
+          ok = true;
+        }
+      }
+    }
+    // fixed slot per (vertex, path): no counter to contend on; stage 2 reads
+    // slot (d, pid) only for d < nhit[pid] - 1, all of which are written here
+    if (in && (uint32_t)d < max_vtx) {
+      recs[(size_t)d * ps.n + pid] = make_float4(bitsf(ok ? cell : kInvalidCell), cu, cv, value);
+    }
+  }
+}
+
+// Insert-or-add into an LDS open-addressing table; false when 16 probes miss
+// (the caller then adds to global memory directly).
+template <class T>
+__device__ __forceinline__ bool lds_table_add(uint32_t* keys, T* vals, uint32_t mask, uint32_t key, T v) {
+  uint32_t h = (key * 0x9E3779B1u) & mask;
+  for (int probe = 0; probe < 16; ++probe) {
+    uint32_t prev = atomicCAS(&keys[h], kEmptyKey, key);
+    if (prev == kEmptyKey || prev == key) {
+      atomicAdd(&vals[h], v);
+      return true;
+    }
+    h = (h + 1u) & mask;
+  }
+  return false;
+}
+
+// kind 1: SD-tree (DTreeT::RecordIrradiance :419-427 -> DTreeNodeT::Record
+// :113-125: weight += 1, flux into the first leaf slot on the descent);
+// kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285).
+template <int kKind>
+__global__ void __launch_bounds__(kApplyBlock) k_commit_apply(const float4* recs, const uint32_t* nhit,
+                                                              uint32_t npaths, uint32_t n, SdTreeView sd,
+                                                              MortonView mq) {
+  __shared__ uint32_t lkey[kLeafTab];
+  __shared__ float lval[kLeafTab];
+  __shared__ uint32_t ckey[kCellTab];
+  __shared__ uint32_t cval[kCellTab];
+  uint32_t begin = blockIdx.x * kApplyBlock * kApplyPerThread;
+  if (begin >= n) return;
+  for (uint32_t i = threadIdx.x; i < kLeafTab; i += kApplyBlock) {
+    lkey[i] = kEmptyKey;
+    lval[i] = 0.0f;
+  }
+  if (kKind == 1) {
+    for (uint32_t i = threadIdx.x; i < kCellTab; i += kApplyBlock) {
+      ckey[i] = kEmptyKey;
+      cval[i] = 0u;
+    }
+  }
+  __syncthreads();
+  uint32_t end = min(n, begin + kApplyBlock * kApplyPerThread);
+  float* flat = kKind == 1 ? reinterpret_cast<float*>(sd.build_sums) : mq.build;
+  for (uint32_t r = begin + threadIdx.x; r < end; r += kApplyBlock) {
+    uint32_t d = r / npaths, pid = r - d * npaths;
+    uint32_t nh = nhit[pid];
+    if (d + 1u >= nh) continue;  // no vertex d on this path
+    float4 rec = recs[r];
+    uint32_t cell = fbits(rec.x);
+    if (cell == kInvalidCell) continue;
+    float value = rec.w;
+    float cu = rec.y, cv = rec.z;
+    if (kKind == 1) {
+      if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) unsafeAtomicAdd(sd.build_sw + cell, 1.0f);
+      if (!(isfinite(value) && value > 0.0f)) continue;
       uint32_t base = fbits(sd.info[cell].w);
       uint32_t node = 0;
       int idx = dtree_child_idx(cu, cv);
@@ -704,28 +869,25 @@ __global__ void __launch_bounds__(kBlock) k_commit_sdtree(PathSoA ps, SdTreeView
         node = ch;
         idx = dtree_child_idx(cu, cv);
       }
-      unsafeAtomicAdd(reinterpret_cast<float*>(sd.build_sums + base + node) + idx, value);
+      uint32_t key = (base + node) * 4u + (uint32_t)idx;
+      if (!lds_table_add<float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
+    } else {
+      uint32_t nn = 1u << mq.level;
+      uint32_t x = (uint32_t)(cu * (float)nn), y = (uint32_t)(cv * (float)nn);
+      x = x < nn ? x : nn - 1u;
+      y = y < nn ? y : nn - 1u;
+      uint32_t key = mq.per_cell * cell + mq_level_offset(mq.level) + morton2(x, y);
+      if (!lds_table_add<float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
     }
   }
-}
-
-// MortonTraceVertexT::Commit (OPX7UtilsMorton.h:550-585) + leaf-only Record (:262-285).
-__global__ void __launch_bounds__(kBlock) k_commit_morton(PathSoA ps, MortonView mq, uint32_t max_vtx) {
-  uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
-  if (pid >= ps.n) return;
-  uint32_t nh = ps.nhit[pid];
-  uint32_t nv = nh > 0 ? nh - 1u : 0u;
-  nv = nv < max_vtx ? nv : max_vtx;
-  uint32_t nn = 1u << mq.level;
-  for (uint32_t d = 0; d < nv; ++d) {
-    uint32_t cell;
-    float cu, cv, value;
-    if (!commit_value(ps, pid, d, nh, cell, cu, cv, value)) continue;
-    uint32_t x = (uint32_t)(cu * (float)nn), y = (uint32_t)(cv * (float)nn);
-    x = x < nn ? x : nn - 1u;
-    y = y < nn ? y : nn - 1u;
-    uint32_t idx = mq_level_offset(mq.level) + morton2(x, y);
-    unsafeAtomicAdd(mq.build + (size_t)mq.per_cell * cell + idx, value);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kLeafTab; i += kApplyBlock) {
+    if (lkey[i] != kEmptyKey) unsafeAtomicAdd(flat + lkey[i], lval[i]);
+  }
+  if (kKind == 1) {
+    for (uint32_t i = threadIdx.x; i < kCellTab; i += kApplyBlock) {
+      if (ckey[i] != kEmptyKey) unsafeAtomicAdd(sd.build_sw + ckey[i], (float)cval[i]);
+    }
   }
 }
 
@@ -786,16 +948,24 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     }
     mark(0, d, 1);
     mark(1, d, 0);
-    if (guide == 1) {
-      hipLaunchKernelGGL(k_shade<1>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+    const bool adjoint = ps.grad_alb != nullptr;
+    const uint32_t nbs = blocks_for(ps.n, kShadeBlock);
+    if (adjoint) {
+      // both passes of the adjoint run this one instantiation, so the replay
+      // reproduces the primal paths operation for operation
+      hipLaunchKernelGGL((k_shade<0, true>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+    } else if (guide == 1) {
+      hipLaunchKernelGGL(k_shade<1>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     } else if (guide == 2) {
-      hipLaunchKernelGGL(k_shade<2>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      hipLaunchKernelGGL(k_shade<2>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     } else {
-      hipLaunchKernelGGL(k_shade<0>, dim3(nb), dim3(kBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      hipLaunchKernelGGL(k_shade<0>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     }
     mark(1, d, 1);
     mark(2, d, 0);
-    if (count) {
+    if (adjoint) {
+      hipLaunchKernelGGL((k_shadow<false, true>), dim3(nb), dim3(kBlock), 0, s, sc, ps, d, 0u);
+    } else if (count) {
       hipLaunchKernelGGL(k_shadow<true>, dim3(nb), dim3(kBlock), 0, s, sc, ps, d, record ? 1u : 0u);
     } else {
       hipLaunchKernelGGL(k_shadow<false>, dim3(nb), dim3(kBlock), 0, s, sc, ps, d, record ? 1u : 0u);
@@ -807,10 +977,18 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
   hipLaunchKernelGGL(k_accum, dim3(blocks_for(npix, 256)), dim3(256), 0, s, ps, npix, rp.spp, accum);
   mark(3, 0, 1);
   mark(4, 0, 0);
-  if (record && guide == 1) {
-    hipLaunchKernelGGL(k_commit_sdtree, dim3(nb), dim3(kBlock), 0, s, ps, sd, rp.max_depth);
-  } else if (record && guide == 2 && !(rp.flags & kFlagLocate)) {
-    hipLaunchKernelGGL(k_commit_morton, dim3(nb), dim3(kBlock), 0, s, ps, mq, rp.max_depth);
+  if (record && (guide == 1 || (guide == 2 && !(rp.flags & kFlagLocate)))) {
+    uint32_t max_vtx = rp.max_depth - 1u;
+    uint32_t cap = max_vtx * ps.n;
+    hipLaunchKernelGGL(k_commit_emit, dim3(nb), dim3(kBlock), 0, s, ps, max_vtx, ps.recs, ps.rec_count);
+    uint32_t na = blocks_for(cap, kApplyBlock * kApplyPerThread);
+    if (guide == 1) {
+      hipLaunchKernelGGL(k_commit_apply<1>, dim3(na), dim3(kApplyBlock), 0, s, ps.recs, ps.nhit, ps.n, cap, sd,
+                         mq);
+    } else {
+      hipLaunchKernelGGL(k_commit_apply<2>, dim3(na), dim3(kApplyBlock), 0, s, ps.recs, ps.nhit, ps.n, cap, sd,
+                         mq);
+    }
   }
   mark(4, 0, 1);
   return hipGetLastError();

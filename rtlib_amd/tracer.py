@@ -150,6 +150,37 @@ class Context:
         call("rtpg_render_stats", self.h, ptr(s, C.c_uint64))
         return {"paths": int(s[0]), "rays": int(s[1]), "shadow_rays": int(s[2])}
 
+    def set_emission(self, le_rgb):
+        le = _f32(le_rgb)
+        call("rtpg_scene_set_emission", self.h, ptr(le))
+
+    def lights(self):
+        n = self.scene_info()["lights"]
+        out = np.zeros(max(n, 1), np.uint32)
+        call("rtpg_scene_lights", self.h, ptr(out, u32))
+        return out[:n]
+
+    def adjoint(self, dl_dimage, spp=1, flags=FLAG_NEE | FLAG_MIS, max_depth=6, seed=0, sample_offset=0,
+                bg=(0.0, 0.0, 0.0), max_paths=0):
+        """Gradient of sum(dl_dimage * image) for the image render(spp, flags,
+        seed, ...) would produce: (albedo grad nt x 3, light radiance grad nl x 3)."""
+        p = _lib.RenderParams()
+        p.flags = flags
+        p.spp = spp
+        p.max_depth = max_depth
+        p.rr_depth = 1000
+        p.num_candidates = 1
+        p.sample_offset = sample_offset
+        p.seed = seed
+        p.bg[0], p.bg[1], p.bg[2] = bg
+        p.max_paths = max_paths
+        g = _f32(dl_dimage, (self.height, self.width, 3))
+        nl = self.scene_info()["lights"]
+        ga = np.zeros((self.n_tris, 3), np.float32)
+        gl = np.zeros((max(nl, 1), 3), np.float32)
+        call("rtpg_adjoint", self.h, C.byref(p), ptr(g), ptr(ga), ptr(gl))
+        return ga, gl[:nl]
+
     def set_profile(self, time_kernels=False, count_work=False):
         call("rtpg_render_profile", self.h, (1 if time_kernels else 0) | (2 if count_work else 0))
 

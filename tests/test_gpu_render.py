@@ -57,6 +57,17 @@ def test_cornell_matches_cpu_integrator(ctx, orc, flags):
     assert st["paths"] == 48 * 48 * 16 and st["rays"] >= st["paths"]
 
 
+def test_many_light_ris_matches_cpu_integrator(ctx, orc):
+    # a reduced many-light scene (icospheres, emissive triangles log-uniform in
+    # [0.1, 100]) rendered with RIS M = 32 and with plain NEE: GPU vs CPU
+    sc = scenes.many_lights(width=48, height=32, n_spheres=60, subdiv=2, n_emissive=300, seed=4)
+    for flags in (FLAG_NEE | FLAG_RIS | FLAG_RR, FLAG_NEE | FLAG_MIS | FLAG_RR):
+        kw = dict(max_depth=6, rr_depth=3, num_candidates=32, seed=8)
+        g = _gpu(ctx, sc, 8, flags, **kw)
+        c = _cpu(orc, sc, 8, flags, **kw)
+        assert _rel_mse(g, c) < 2e-3, (flags, _rel_mse(g, c))
+
+
 def test_chunked_launch_equals_single_launch(ctx):
     sc = scenes.cornell(32, 32)
     a = _gpu(ctx, sc, 8, FLAG_NEE | FLAG_MIS | FLAG_RR, seed=1)

@@ -46,3 +46,38 @@ def test_cornell_estimators_agree(orc, flags, rtol):
     ref = np.mean([_render(orc, sc, 128, NEE | MIS | RR, seed=s)[0].mean() for s in range(2)])
     alt = np.mean([_render(orc, sc, 128, flags, seed=100 + s, num_candidates=8)[0].mean() for s in range(2)])
     assert abs(alt - ref) / ref < rtol, (alt, ref)
+
+
+def _fd_scene(sc, t, ch, delta):
+    s2 = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in sc.items()}
+    s2["albedo"][t, ch] += delta
+    return s2
+
+
+def test_oracle_adjoint_matches_finite_differences(orc):
+    # The FP64 reference gradient must agree with central differences of the
+    # oracle's own primal render under the same random numbers (h = 1e-3).
+    sc = scenes.cornell(16, 16)
+    cam = sc["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 1.0)
+    rng = np.random.default_rng(5)
+    dldi = rng.uniform(-1, 1, (16, 16, 3)).astype(np.float32)
+    flags = NEE | MIS
+    ga, gl = orc.adjoint(orc.Scene(sc), uvw, 16, 16, 4, flags, dldi, 2, max_depth=6, seed=3)
+
+    def loss(s):
+        img, _ = orc.render(orc.Scene(s), uvw, 16, 16, 4, flags, max_depth=6, seed=3, threads=1)
+        return float((img.astype(np.float64) / 4 * dldi).sum())
+
+    h = 1e-3
+    # walls (0..9) and box faces carry large gradients; check a spread of them
+    for t, ch in [(0, 0), (2, 1), (4, 2), (6, 0), (7, 1), (9, 2), (12, 0), (20, 1), (30, 2)]:
+        fd = (loss(_fd_scene(sc, t, ch, h)) - loss(_fd_scene(sc, t, ch, -h))) / (2 * h)
+        assert abs(fd - ga[t, ch]) <= 1e-2 * abs(ga[t, ch]) + 1e-3 * np.abs(ga).max(), (t, ch, fd, ga[t, ch])
+    # emission: loss is linear in Le, so the gradient is exact under rescaling
+    le = sc["emission"][sc["emission"].sum(1) > 0][0]
+    s2 = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in sc.items()}
+    s2["emission"][s2["emission"].sum(1) > 0] *= 2.0
+    want = loss(s2) - loss(sc)
+    got = float((gl * le[None, :]).sum())
+    assert abs(want - got) <= 1e-3 * abs(want)

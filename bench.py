@@ -56,7 +56,12 @@ def main(argv=None):
     ap.add_argument("--ref-spp", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--small", action="store_true", help="reduced size for a quick functional check")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (once untimed, then 5 times) and exit; used to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args(argv)
+    if a.copy_calib:
+        return copy_calibration(a.copy_calib)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in the design notes "
                           "but not launched", "n_gpus": a.gpus}))
@@ -85,11 +90,10 @@ def main(argv=None):
     tree = RTSTreeWrapper(ctx, lo, hi, 20, 0.3)
     tree.SetSplit(mode=1, c=12000.0, max_mb=2000, subdiv_th=0.01)
 
-    def guided_step(seed, sample_offset=0):
+    def guided_step(seed, on_pass=None):
         ctrl = RTSTreeController(tree, a.budget, 0, 0.5, 1)
         ctx.clear()
-        n = render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=seed)
-        return n
+        return render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=seed, on_pass=on_pass)
 
     def timed(fn):
         ctx.synchronize()
@@ -98,97 +102,104 @@ def main(argv=None):
         ctx.synchronize()
         return time.perf_counter() - t, r
 
-    # ---- guided: warmup, then K timed steps with kernel timing on the render stream
+    # ---- guided: warmup, then K timed steps.  Kernel classes are timed with
+    # HIP events on the render stream during the timed steps themselves; the
+    # window of the timed steps is logged (CLOCK_BOOTTIME and CLOCK_MONOTONIC
+    # ns) so that a kernel trace can be cut to exactly these dispatches.
     for w in range(a.warmup):
         dt, _ = timed(lambda: guided_step(100 + w))
         log(f"warmup step {w}: {dt * 1e3:.1f} ms")
+    names = ["extend", "shade", "shadow", "accumulate", "commit"]
     ctx.set_profile(time_kernels=True, count_work=False)
-    times, kms = [], np.zeros(5)
+    times, step_ms, step_launches = [], [], []
     rays = shadow = 0
-    for k in range(a.steps):
-        dt, n = timed(lambda: guided_step(k + 1))
-        times.append(dt)
-        log(f"step {k}: {dt * 1e3:.1f} ms ({n} spp)")
-    # kernel times accumulate only over the last render call; re-time one step
-    # pass by pass to collect per-kernel totals over a whole step
-    ctx.set_profile(time_kernels=True, count_work=False)
-    per_kernel = np.zeros(5)
-    launches = 0
-    ctrl = RTSTreeController(tree, a.budget, 0, 0.5, 1)
-    ctx.clear()
 
     def acc_pass(state, spp, traced):
-        nonlocal launches, rays, shadow
-        ms, _ = ctx.kernel_stats()
-        per_kernel[:] += np.array([ms[k] for k in ("extend", "shade", "shadow", "accumulate", "commit")])
+        nonlocal rays, shadow
+        ms, _, nl = ctx.kernel_stats()
+        step_ms[-1] += np.array([ms[k] for k in names])
+        step_launches[-1] += np.array([nl[k] for k in names])
         st = ctx.render_stats()
         rays += st["rays"]
         shadow += st["shadow_rays"]
-        launches += 1
 
-    t_prof0 = time.perf_counter()
-    render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=1, on_pass=acc_pass)
-    ctx.synchronize()
-    t_prof = time.perf_counter() - t_prof0
+    win = [time.clock_gettime_ns(time.CLOCK_BOOTTIME), time.monotonic_ns()]
+    for k in range(a.steps):
+        step_ms.append(np.zeros(5))
+        step_launches.append(np.zeros(5, np.int64))
+        dt, n = timed(lambda: guided_step(k + 1, on_pass=acc_pass))
+        times.append(dt)
+        log(f"step {k}: {dt * 1e3:.1f} ms ({n} spp)")
+    win += [time.clock_gettime_ns(time.CLOCK_BOOTTIME), time.monotonic_ns()]
+    log(f"timed window ns: boottime {win[0]} {win[2]} monotonic {win[1]} {win[3]}")
     guided_img = ctx.image()
-    # ---- work counts of the same step (separate, untimed run)
+    per_kernel = np.mean(step_ms, axis=0)
+    launches = step_launches[0]
+    # ---- work counts of step 0 (seed 1) in a separate, untimed run: same
+    # random streams, so the same paths and the same traversal work
     ctx.set_profile(time_kernels=False, count_work=True)
     work = np.zeros(4)
     rays_c = 0
 
     def acc_work(state, spp, traced):
         nonlocal rays_c
-        _, wk = ctx.kernel_stats()
+        _, wk, _ = ctx.kernel_stats()
         work[:] += np.array([wk["extend_nodes"], wk["extend_tris"], wk["shadow_nodes"], wk["shadow_tris"]])
         rays_c += ctx.render_stats()["rays"]
 
-    ctrl = RTSTreeController(tree, a.budget, 0, 0.5, 1)
-    ctx.clear()
-    render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=1, on_pass=acc_work)
+    guided_step(1, on_pass=acc_work)
     ctx.set_profile(False, False)
 
     t_step = float(np.median(times))
     samples = a.budget * npix
     value = samples / t_step / 1e6
+    t_rays = float(np.sum(times))
 
     # ---- plain path tracing on the same scene
     ctx.clear()
     dt_plain, _ = timed(lambda: ctx.render(spp=a.budget, flags=base, max_depth=depth, seed=77))
     plain_rate = a.budget * npix / dt_plain / 1e6
     st_plain = ctx.render_stats()
-    # equal time: as many plain samples as the guided step's wall clock allows
-    spp_eq = max(1, int(round(t_step * plain_rate * 1e6 / npix)))
-    ctx.clear()
-    ctx.render(spp=spp_eq, flags=base, max_depth=depth, seed=78)
-    plain_img = ctx.image()
-    # ---- reference image (plain, many samples, independent seed)
-    t0 = time.time()
-    ctx.clear()
-    done = 0
-    while done < a.ref_spp:
-        k = min(256, a.ref_spp - done)
-        ctx.render(spp=k, flags=base, max_depth=depth, seed=4242, sample_offset=done)
-        done += k
-    ref = ctx.image()
-    log(f"reference {a.ref_spp} spp in {time.time() - t0:.1f} s")
-    rm_guided = rel_mse(guided_img, ref)
-    rm_plain = rel_mse(plain_img, ref)
+    quality = None
+    if a.ref_spp > 0:
+        # equal time: as many plain samples as the guided step's wall clock allows
+        spp_eq = max(1, int(round(t_step * plain_rate * 1e6 / npix)))
+        ctx.clear()
+        ctx.render(spp=spp_eq, flags=base, max_depth=depth, seed=78)
+        plain_img = ctx.image()
+        # reference image (plain, many samples, independent seed)
+        t0 = time.time()
+        ctx.clear()
+        done = 0
+        while done < a.ref_spp:
+            k = min(256, a.ref_spp - done)
+            ctx.render(spp=k, flags=base, max_depth=depth, seed=4242, sample_offset=done)
+            done += k
+        ref = ctx.image()
+        log(f"reference {a.ref_spp} spp in {time.time() - t0:.1f} s")
+        quality = {"guided": round(rel_mse(guided_img, ref), 5), "plain": round(rel_mse(plain_img, ref), 5),
+                   "plain_spp": spp_eq, "guided_spp": a.budget, "reference_spp": a.ref_spp}
 
-    # ---- roofline of the dominant kernel
-    names = ["extend", "shade", "shadow", "accumulate", "commit"]
+    # ---- roofline of the dominant kernel (k_extend): algorithmic bytes of
+    # step 0 over its event-timed k_extend time
     dom = int(np.argmax(per_kernel))
     ext_bytes = rays_c * (RAY_IN_BYTES + HIT_OUT_BYTES + QUEUE_BYTES) + work[0] * NODE_BYTES + work[1] * TRI_BYTES
-    n_ext_launch = launches * depth
-    achieved = ext_bytes / (per_kernel[0] * 1e-3) / 1e9 if per_kernel[0] > 0 else 0.0
+    n_ext = int(launches[0])
+    ext_ms0 = float(step_ms[0][0])
+    achieved = ext_bytes / (ext_ms0 * 1e-3) / 1e9 if ext_ms0 > 0 else 0.0
+    traffic, traffic_src = _traffic(W, H)
     roofline = {"kernel": "k_extend (closest-hit BVH4 traversal)", "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_step": float(ext_bytes), "launches_per_step": n_ext_launch,
-                "avg_launch_ms": round(per_kernel[0] / max(n_ext_launch, 1), 4),
+                "traffic": traffic, "traffic_source": traffic_src,
+                "algorithmic_bytes_per_launch": round(ext_bytes / max(n_ext, 1)),
+                "launches_per_step": n_ext,
+                "avg_launch_ms": round(float(per_kernel[0]) / max(n_ext, 1), 4),
                 "dominant_by_time": names[dom],
-                "kernel_ms_per_step": {n: round(float(x), 2) for n, x in zip(names, per_kernel)}}
+                "kernel_ms_per_step": {n: round(float(x), 2) for n, x in zip(names, per_kernel)},
+                "kernel_launches_per_step": {n: int(x) for n, x in zip(names, launches)}}
 
     # ---- CPU baseline: the oracle integrator on the host cores, guided final pass
-    cpu = cpu_baseline(sc, tree, lo, hi, W, H, depth, a.cpu_seconds)
+    cpu = cpu_baseline(sc, tree, lo, hi, W, H, depth, a.cpu_seconds) if a.cpu_seconds > 0 else None
 
     out = {
         "metric": METRIC,
@@ -208,15 +219,45 @@ def main(argv=None):
                    "width": W, "height": H, "triangles": int(sc["idx"].shape[0]), "budget_spp": a.budget,
                    "max_depth": depth, "parallelism": "single GPU"},
         "plain_msamples_per_s": round(plain_rate, 2),
-        "mrays_per_s": {"guided": round((rays + shadow) / t_prof / 1e6, 1),
+        "mrays_per_s": {"guided": round((rays + shadow) / t_rays / 1e6, 1),
                         "plain": round((st_plain["rays"] + st_plain["shadow_rays"]) / dt_plain / 1e6, 1)},
-        "relmse_equal_time": {"guided": round(rm_guided, 5), "plain": round(rm_plain, 5),
-                              "plain_spp": spp_eq, "guided_spp": a.budget, "reference_spp": a.ref_spp},
+        "relmse_equal_time": quality,
         "sdtree": tree.Sizes(),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "k_extend_guided_room_traffic.json"
+
+
+def _traffic(W, H):
+    """HBM bytes per k_extend launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by the counter's reading for a copy kernel of known size)."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None, None
+    if t.get("width") != W or t.get("height") != H:
+        return None, None
+    b = t["fetch_kib_per_dispatch"] * 1024 * t["fetch_scale"] + t["write_kib_per_dispatch"] * 1024 * t["write_scale"]
+    return round(b), TRAFFIC_FILE.name
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.arange(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "dispatches": 5}), flush=True)
     return 0
 
 

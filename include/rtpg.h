@@ -131,8 +131,10 @@ int rtpg_render_stats(rtpg_context ctx, uint64_t stats[3]);
  * bit 1: count BVH nodes fetched and triangles tested. */
 int rtpg_render_profile(rtpg_context ctx, uint32_t mode);
 /* ms[5]: extend, shade, shadow, accumulate, commit of the last render;
- * work[4]: extend nodes, extend triangles, shadow nodes, shadow triangles. */
-int rtpg_render_kernel_stats(rtpg_context ctx, double ms[5], uint64_t work[4]);
+ * work[4]: extend nodes, extend triangles, shadow nodes, shadow triangles;
+ * launches[5]: kernel launches per class in the last render.  Any pointer may
+ * be NULL. */
+int rtpg_render_kernel_stats(rtpg_context ctx, double ms[5], uint64_t work[4], uint64_t launches[5]);
 
 /* --- path-replay adjoint ------------------------------------------------------
  * No counterpart in the reference (its integrator is forward only).  For the

@@ -57,7 +57,7 @@ SIGNATURES = {
     "rtpg_adjoint": [c_vp, C.POINTER(RenderParams), c_fp, c_fp, c_fp],
     "rtpg_scene_set_emission": [c_vp, c_fp],
     "rtpg_scene_lights": [c_vp, c_u32p],
-    "rtpg_render_kernel_stats": [c_vp, C.POINTER(C.c_double), c_u64p],
+    "rtpg_render_kernel_stats": [c_vp, C.POINTER(C.c_double), c_u64p, c_u64p],
     "rtpg_trace_closest": [c_vp, c_fp, C.c_uint32, c_fp],
     "rtpg_trace_occluded": [c_vp, c_fp, C.c_uint32, c_u32p],
     "rtpg_sdtree_create": [c_vp, c_fp, c_fp, C.c_uint32, C.c_float, C.POINTER(c_vp)],

@@ -488,6 +488,7 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       for (hipEvent_t& e : ctx->events) RTPG_HIP(hipEventCreate(&e));
     }
     for (double& m : ctx->kernel_ms) m = 0.0;
+    for (uint64_t& l : ctx->kernel_launches) l = 0;
     if (counting) {
       ctx->p_work.reserve(4);
       ctx->p_work.zero(4, s);
@@ -501,6 +502,10 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, guide, ctx->accum.p, s,
                                    timing ? ctx->events.data() : nullptr));
       RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));
+      for (int k = 0; k < 3; ++k) ctx->kernel_launches[k] += prm->max_depth;
+      ctx->kernel_launches[3] += 1;
+      if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL) && !(guide == 2 && (prm->flags & RTPG_FLAG_LOCATE)))
+        ctx->kernel_launches[4] += 2;
       if (timing) {
         RTPG_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < kTimedKernels; ++k) {
@@ -540,11 +545,12 @@ extern "C" int rtpg_render_profile(rtpg_context ctx, uint32_t mode) {
   });
 }
 
-extern "C" int rtpg_render_kernel_stats(rtpg_context ctx, double ms[5], uint64_t work[4]) {
+extern "C" int rtpg_render_kernel_stats(rtpg_context ctx, double ms[5], uint64_t work[4], uint64_t launches[5]) {
   return guarded([&] {
     require(ctx != nullptr, "ctx is NULL");
     if (ms) for (int k = 0; k < kTimedKernels; ++k) ms[k] = ctx->kernel_ms[k];
     if (work) for (int k = 0; k < 4; ++k) work[k] = ctx->work[k];
+    if (launches) for (int k = 0; k < kTimedKernels; ++k) launches[k] = ctx->kernel_launches[k];
   });
 }
 

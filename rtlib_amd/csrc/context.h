@@ -45,6 +45,7 @@ struct rtpg_context_t {
   uint32_t prof_mode = 0;
   std::vector<hipEvent_t> events;
   double kernel_ms[rtpg::kTimedKernels] = {0, 0, 0, 0, 0};
+  uint64_t kernel_launches[rtpg::kTimedKernels] = {0, 0, 0, 0, 0};
   rtpg::DBuf<unsigned long long> p_work;
   uint64_t work[4] = {0, 0, 0, 0};
   ~rtpg_context_t() {

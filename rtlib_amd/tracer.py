@@ -187,11 +187,14 @@ class Context:
     def kernel_stats(self):
         ms = np.zeros(5, np.float64)
         work = np.zeros(4, np.uint64)
-        call("rtpg_render_kernel_stats", self.h, ptr(ms, C.c_double), ptr(work, C.c_uint64))
+        launches = np.zeros(5, np.uint64)
+        call("rtpg_render_kernel_stats", self.h, ptr(ms, C.c_double), ptr(work, C.c_uint64),
+             ptr(launches, C.c_uint64))
         names = ["extend", "shade", "shadow", "accumulate", "commit"]
         return ({n: float(x) for n, x in zip(names, ms)},
                 {"extend_nodes": int(work[0]), "extend_tris": int(work[1]), "shadow_nodes": int(work[2]),
-                 "shadow_tris": int(work[3])})
+                 "shadow_tris": int(work[3])},
+                {n: int(x) for n, x in zip(names, launches)})
 
     # -- ray queries
     def trace_closest(self, rays):

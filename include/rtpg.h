@@ -59,6 +59,7 @@ typedef struct rtpg_sdtree_ctrl_t* rtpg_sdtree_ctrl;
 typedef struct rtpg_hashgrid_t* rtpg_hashgrid;
 typedef struct rtpg_mqtree_t* rtpg_mqtree;
 typedef struct rtpg_mqtree_ctrl_t* rtpg_mqtree_ctrl;
+typedef struct rtpg_mlp_t* rtpg_mlp;
 
 const char* rtpg_last_error(void);
 const char* rtpg_version(void);
@@ -121,6 +122,7 @@ typedef struct rtpg_render_params {
   rtpg_hashgrid hashgrid;  /* spatial hash for RTPG_FLAG_USE_GRID */
   rtpg_mqtree mqtree;      /* direction quadtrees for RTPG_FLAG_USE_GRID */
   uint32_t max_paths;      /* paths in flight per chunk (0 = default 4M) */
+  rtpg_mlp mlp;            /* learned guide for RTPG_FLAG_USE_MLP, else NULL */
 } rtpg_render_params;
 int rtpg_render(rtpg_context ctx, const rtpg_render_params* params);
 /* Last render's counters: [0] paths, [1] radiance rays, [2] shadow rays. */
@@ -261,6 +263,71 @@ int rtpg_mqtree_ctrl_start(rtpg_mqtree_ctrl c);
 int rtpg_mqtree_ctrl_beg_trace(rtpg_mqtree_ctrl c);
 int rtpg_mqtree_ctrl_end_trace(rtpg_mqtree_ctrl c, uint32_t samples);
 int rtpg_mqtree_ctrl_query(rtpg_mqtree_ctrl c, uint32_t q[6]);
+
+/* --- learned directional guide ------------------------------------------------
+ * No counterpart in the reference (its guides are the SD-tree and the Morton
+ * hash grid).  A multiresolution hash-grid encoding of the position (16 levels
+ * x 2 features, 2^log2_table entries per level, resolutions base_res ..
+ * max_res, geometric) concatenated with the outgoing direction, the normal and
+ * the roughness feeds a 3 x 64 ReLU MLP (bf16 on the matrix cores, fp32
+ * accumulation).  Its 41 outputs give an 8-lobe von Mises-Fisher mixture q and
+ * the probability c in [0.05, 0.95] of sampling q instead of the BSDF; the
+ * renderer samples the one-sample MIS mixture c q + (1 - c) p_bsdf.  Training
+ * minimises mean_i -w_i log(c q(w_i) + (1 - c) p_bsdf(w_i)) with Adam, where
+ * w_i = avg_rgb(f Li |cos|) / p_sample of a radiance sample recorded by the
+ * tracer (the KL divergence from the radiance-weighted target, up to a
+ * constant).  With RTPG_FLAG_USE_MLP a render records training samples from
+ * the first record_paths paths of every launch chunk (unless RTPG_FLAG_FINAL)
+ * and, with RTPG_FLAG_GUIDE_SAMPLE, samples directions from the guide. */
+typedef struct rtpg_mlp_config {
+  uint32_t log2_table;   /* hash entries per level (19) */
+  float base_res;        /* coarsest grid resolution (16) */
+  float max_res;         /* finest grid resolution (2048) */
+  float lr;              /* Adam step size (1e-2) */
+  float beta1, beta2;    /* Adam moments (0.9, 0.99) */
+  float eps;             /* Adam epsilon (1e-15) */
+  uint32_t batch;        /* samples per training step (2^18) */
+  uint32_t capacity;     /* recorded samples kept between trainings (2^20) */
+  uint32_t record_paths; /* paths per render chunk that record samples (2^16) */
+  uint64_t seed;         /* initialisation */
+} rtpg_mlp_config;
+/* cfg = NULL takes the defaults above.  The position box is made cubic. */
+int rtpg_mlp_create(rtpg_context ctx, const float aabb_min[3], const float aabb_max[3], const rtpg_mlp_config* cfg,
+                    rtpg_mlp* out);
+int rtpg_mlp_destroy(rtpg_mlp m);
+/* Re-initialise weights and table from seed; clear the Adam state and the recorded samples. */
+int rtpg_mlp_reset(rtpg_mlp m, uint64_t seed);
+/* Network queries at positions pos[3n] with outgoing directions wo[3n], normals
+ * nrm[3n] and roughness rough[n].  raw: 41 outputs per query; mixture: per query
+ * 8 x (pi, mu xyz, kappa), then c.  Either may be NULL. */
+int rtpg_mlp_forward(rtpg_mlp m, const float* pos, const float* wo, const float* nrm, const float* rough, uint32_t n,
+                     float* raw, float* mixture);
+/* One direction per query from the vMF mixture of rtpg_mlp_forward's mixture
+ * output; out: 4 per query (w xyz, mixture density q(w)); PCG32 stream i of seed. */
+int rtpg_mlp_sample(rtpg_mlp m, const float* mixture, uint32_t n, uint64_t seed, float* out);
+/* Training samples: pos, wo, nrm (3n each), rough[n], omega[3n] (sampled
+ * direction), weight[n] (w_i), pdf_bsdf[n] (BSDF density of omega). */
+/* Loss and gradient of the mean loss, without an update.  grad_w: the 14336
+ * weights (W0 64x48, W1 64x64, W2 64x64, W3 48x64, row-major out x in);
+ * grad_table: 16 * 2^log2_table * 2 (may be NULL). */
+int rtpg_mlp_gradient(rtpg_mlp m, const float* pos, const float* wo, const float* nrm, const float* rough,
+                      const float* omega, const float* weight, const float* pdf_bsdf, uint32_t n, float* loss,
+                      float* grad_w, float* grad_table);
+/* `steps` Adam steps on the whole sample set; loss[steps] (may be NULL) before each update. */
+int rtpg_mlp_train(rtpg_mlp m, const float* pos, const float* wo, const float* nrm, const float* rough,
+                   const float* omega, const float* weight, const float* pdf_bsdf, uint32_t n, uint32_t steps,
+                   float* loss);
+/* `steps` Adam steps on the samples recorded by renders since the last call
+ * (batches of cfg.batch, in recording order), then forget them.  used: the
+ * number of recorded samples (may be NULL). */
+int rtpg_mlp_train_recorded(rtpg_mlp m, uint32_t steps, float* loss, uint32_t* used);
+/* fp32 master copies: weights[14336], table[16 * 2^log2_table * 2]; either may be NULL. */
+int rtpg_mlp_export(rtpg_mlp m, float* weights, float* table);
+int rtpg_mlp_import(rtpg_mlp m, const float* weights, const float* table);
+/* info: [0] weights, [1] table floats, [2] recorded samples, [3] Adam steps
+ * taken, [4] levels, [5] entries per level, [6] dense-level mask; res[16]:
+ * grid resolution per level (either may be NULL). */
+int rtpg_mlp_info(rtpg_mlp m, uint64_t info[7], float res[16]);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,23 @@ class RenderParams(C.Structure):
         ("hashgrid", c_vp),
         ("mqtree", c_vp),
         ("max_paths", C.c_uint32),
+        ("mlp", c_vp),
+    ]
+
+
+class MlpConfig(C.Structure):
+    _fields_ = [
+        ("log2_table", C.c_uint32),
+        ("base_res", C.c_float),
+        ("max_res", C.c_float),
+        ("lr", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("batch", C.c_uint32),
+        ("capacity", C.c_uint32),
+        ("record_paths", C.c_uint32),
+        ("seed", C.c_uint64),
     ]
 
 
@@ -58,6 +75,17 @@ SIGNATURES = {
     "rtpg_scene_set_emission": [c_vp, c_fp],
     "rtpg_scene_lights": [c_vp, c_u32p],
     "rtpg_render_kernel_stats": [c_vp, C.POINTER(C.c_double), c_u64p, c_u64p],
+    "rtpg_mlp_create": [c_vp, c_fp, c_fp, C.POINTER(MlpConfig), C.POINTER(c_vp)],
+    "rtpg_mlp_destroy": [c_vp],
+    "rtpg_mlp_reset": [c_vp, C.c_uint64],
+    "rtpg_mlp_forward": [c_vp, c_fp, c_fp, c_fp, c_fp, C.c_uint32, c_fp, c_fp],
+    "rtpg_mlp_sample": [c_vp, c_fp, C.c_uint32, C.c_uint64, c_fp],
+    "rtpg_mlp_gradient": [c_vp, c_fp, c_fp, c_fp, c_fp, c_fp, c_fp, c_fp, C.c_uint32, c_fp, c_fp, c_fp],
+    "rtpg_mlp_train": [c_vp, c_fp, c_fp, c_fp, c_fp, c_fp, c_fp, c_fp, C.c_uint32, C.c_uint32, c_fp],
+    "rtpg_mlp_train_recorded": [c_vp, C.c_uint32, c_fp, c_u32p],
+    "rtpg_mlp_export": [c_vp, c_fp, c_fp],
+    "rtpg_mlp_import": [c_vp, c_fp, c_fp],
+    "rtpg_mlp_info": [c_vp, c_u64p, c_fp],
     "rtpg_trace_closest": [c_vp, c_fp, C.c_uint32, c_fp],
     "rtpg_trace_occluded": [c_vp, c_fp, C.c_uint32, c_u32p],
     "rtpg_sdtree_create": [c_vp, c_fp, c_fp, C.c_uint32, C.c_float, C.POINTER(c_vp)],

@@ -298,7 +298,7 @@ SdTreeView sdtree_view_of(rtpg_sdtree_t* t);
 void guide_views(const rtpg_render_params* rp, rtpg_context_t* ctx, HashGridView& hg, MortonView& mq);
 }  // namespace rtpg
 
-static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool guide) {
+static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool guide, uint32_t mlp_rec) {
   c->p_ray_o.reserve(n);
   c->p_ray_d.reserve(n);
   c->p_T.reserve(n);
@@ -318,10 +318,17 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
     c->p_recs.reserve((size_t)(max_depth > 1 ? max_depth - 1 : 1) * n);
     c->p_rec_count.reserve(1);
   }
+  if (mlp_rec) {
+    c->p_mix.reserve((size_t)kMlpMixF4 * n);
+    c->p_mixc.reserve(n);
+    c->p_mv.reserve((size_t)5 * max_depth * mlp_rec);
+  }
 }
 
-static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, bool guide, bool counting) {
-  ensure_arena(ctx, n, max_depth, guide);
+static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, bool guide, bool counting,
+                        rtpg_mlp_t* mlp = nullptr) {
+  uint32_t mlp_rec = mlp ? std::min(n, mlp->cfg.record_paths) : 0u;
+  ensure_arena(ctx, n, max_depth, guide, mlp_rec);
   PathSoA ps{};
   ps.n = n;
   ps.ray_o = ctx->p_ray_o.p;
@@ -345,6 +352,16 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
   ps.recs = guide ? ctx->p_recs.p : nullptr;
   ps.rec_count = guide ? ctx->p_rec_count.p : nullptr;
   ps.npix = ctx->width * ctx->height;
+  ps.rec_limit = mlp ? mlp_rec : n;
+  if (mlp) {
+    ps.mix = ctx->p_mix.p;
+    ps.mixc = ctx->p_mixc.p;
+    ps.mv = ctx->p_mv.p;
+    ps.mlp_rec = mlp_rec;
+    ps.mlp_out = mlp->samples.p;
+    ps.mlp_count = mlp->count.p;
+    ps.mlp_cap = mlp->cfg.capacity;
+  }
   return ps;
 }
 
@@ -409,11 +426,11 @@ extern "C" int rtpg_adjoint(rtpg_context ctx, const rtpg_render_params* prm, con
       RenderParams rp = make_rp(ctx, prm, cs, done);
       // primal pass: the total radiance of every path
       ps.adj_mode = 0u;
-      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, 0, scratch.p, s));
+      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, MlpView{}, 0, scratch.p, s));
       RTPG_HIP(hipMemcpyAsync(d_ltot.p, ps.L, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
       // replay with the same random streams; scatter the derivatives
       ps.adj_mode = 1u;
-      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, 0, scratch.p, s));
+      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, MlpView{}, 0, scratch.p, s));
       done += cs;
     }
     std::vector<float> galb = d_galb.to_vector(3 * (size_t)ctx->n_tris, s);
@@ -463,10 +480,13 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     RTPG_HIP(hipSetDevice(ctx->device));
     const bool use_tree = (prm->flags & RTPG_FLAG_USE_TREE) != 0;
     const bool use_grid = (prm->flags & RTPG_FLAG_USE_GRID) != 0;
-    require(!(use_tree && use_grid), "USE_TREE and USE_GRID are exclusive");
+    const bool use_mlp = (prm->flags & RTPG_FLAG_USE_MLP) != 0;
+    require((int)use_tree + (int)use_grid + (int)use_mlp <= 1, "USE_TREE, USE_GRID and USE_MLP are exclusive");
     require(!use_tree || prm->sdtree != nullptr, "USE_TREE needs an sdtree");
     require(!use_grid || (prm->hashgrid != nullptr && prm->mqtree != nullptr), "USE_GRID needs hashgrid + mqtree");
-    int guide = use_tree ? 1 : (use_grid ? 2 : 0);
+    require(!use_mlp || prm->mlp != nullptr, "USE_MLP needs an mlp");
+    require(!use_mlp || prm->mlp->ctx == ctx, "the mlp belongs to another context");
+    int guide = use_tree ? 1 : (use_grid ? 2 : (use_mlp ? 3 : 0));
     require(guide == 0 || (prm->flags & RTPG_FLAG_FINAL) || prm->max_depth <= 16,
             "recording guide vertices supports max_depth <= 16");
     uint32_t npix = ctx->width * ctx->height;
@@ -477,8 +497,10 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     SdTreeView sd{};
     HashGridView hg{};
     MortonView mq{};
+    MlpView ml{};
     if (guide == 1) sd = sdtree_view_of(prm->sdtree);
     if (guide == 2) guide_views(prm, ctx, hg, mq);
+    if (guide == 3) ml = mlp_view_of(prm->mlp, false);
     hipStream_t s = ctx->stream;
     ctx->p_stats.zero(4, s);
     const bool timing = (ctx->prof_mode & 1u) != 0;
@@ -497,15 +519,16 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     while (done < prm->spp) {
       uint32_t cs = std::min(chunk_spp, prm->spp - done);
       uint32_t n = cs * npix;
-      PathSoA ps = make_soa(ctx, n, prm->max_depth, guide != 0, counting);
+      PathSoA ps = make_soa(ctx, n, prm->max_depth, guide != 0, counting, guide == 3 ? prm->mlp : nullptr);
       RenderParams rp = make_rp(ctx, prm, cs, done);
-      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, guide, ctx->accum.p, s,
+      RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, ml, guide, ctx->accum.p, s,
                                    timing ? ctx->events.data() : nullptr));
       RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));
       for (int k = 0; k < 3; ++k) ctx->kernel_launches[k] += prm->max_depth;
       ctx->kernel_launches[3] += 1;
       if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL) && !(guide == 2 && (prm->flags & RTPG_FLAG_LOCATE)))
-        ctx->kernel_launches[4] += 2;
+        ctx->kernel_launches[4] += guide == 3 ? 1 : 2;
+      if (guide == 3 && (prm->flags & RTPG_FLAG_GUIDE_SAMPLE)) ctx->kernel_launches[1] += prm->max_depth;
       if (timing) {
         RTPG_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < kTimedKernels; ++k) {

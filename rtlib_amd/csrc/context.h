@@ -5,8 +5,10 @@
 
 #include <vector>
 
+#include "../../include/rtpg.h"
 #include "bvh.h"
 #include "host.h"
+#include "mlp_dev.h"
 #include "render.h"
 #include "sdtree.h"
 #include "views.h"
@@ -37,6 +39,8 @@ struct rtpg_context_t {
   rtpg::Camera cam{};
   // path arena
   rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC, p_recs;
+  rtpg::DBuf<float4> p_mix, p_mv;  // learned guide: mixture per queue slot; training vertices (5 arrays)
+  rtpg::DBuf<float> p_mixc;
   rtpg::DBuf<uint2> p_rng;
   rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err, p_rec_count;
   rtpg::DBuf<unsigned long long> p_stats;
@@ -90,3 +94,27 @@ struct rtpg_sdtree_t {
   void reset(int iter, int spp);
   void publish();
 };
+
+// Learned directional guide (no counterpart in the reference).
+struct rtpg_mlp_t {
+  rtpg_context_t* ctx = nullptr;
+  rtpg_mlp_config cfg{};
+  float lo[3] = {0, 0, 0};
+  float size = 1.0f;
+  float res[rtpg::kMlpLevels] = {};
+  uint32_t dense = 0;
+  size_t entries = 0;  // per level
+  rtpg::DBuf<float> w, w_m1, w_m2, w_grad;
+  rtpg::DBuf<uint16_t> w_bf16;
+  rtpg::DBuf<float> t, t_m1, t_m2, t_grad;
+  rtpg::DBuf<uint32_t> t_bf16;
+  rtpg::DBuf<float> partial, loss_partial, loss;
+  rtpg::DBuf<float4> samples;  // recorded by renders: capacity x 4
+  rtpg::DBuf<uint32_t> count;
+  rtpg::DBuf<float4> staging;
+  rtpg::DBuf<float> out;
+  uint64_t steps = 0;
+  size_t table_floats() const { return (size_t)rtpg::kMlpLevels * entries * rtpg::kMlpFeat; }
+};
+
+rtpg::MlpView mlp_view_of(const rtpg_mlp_t* m, bool training);

@@ -1,0 +1,170 @@
+// Learned directional guide: a multiresolution hash-grid encoding of the
+// shading position, concatenated with the outgoing direction, the normal and
+// the roughness, feeds a 3 x 64 ReLU MLP whose 41 outputs parameterise an
+// 8-lobe von Mises-Fisher mixture and the probability c of sampling that
+// mixture instead of the BSDF.  No counterpart in the reference.
+//
+// Layout of one forward pass on the matrix cores: a wave evaluates 16 samples.
+// Weights are the A operand (rows = output features) and activations the B
+// operand (columns = samples) of v_mfma_f32_16x16x16_bf16, so the D tile of
+// one layer (lane l: sample l%16, features 4*(l/16)+j) is already the B
+// operand of the next layer after ReLU and the cast to bf16.  Weights are
+// bf16 in LDS; accumulation is fp32.
+#pragma once
+#include "common.h"
+
+namespace rtpg {
+
+constexpr int kMlpLevels = 16;
+constexpr int kMlpFeat = 2;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpLobes = 8;
+constexpr int kMlpIn = 48;       // 32 encoded + wo(3) + n(3) + roughness + 1 + 8 zeros
+constexpr int kMlpOut = 48;      // 8 x (logit, m xyz, log kappa) + selection logit + 7 zeros
+constexpr int kMlpOutUsed = 41;
+constexpr int kMlpW0 = 0;                                   // 64 x 48 (out x in, row-major)
+constexpr int kMlpW1 = kMlpW0 + kMlpHidden * kMlpIn;        // 64 x 64
+constexpr int kMlpW2 = kMlpW1 + kMlpHidden * kMlpHidden;    // 64 x 64
+constexpr int kMlpW3 = kMlpW2 + kMlpHidden * kMlpHidden;    // 48 x 64
+constexpr int kMlpWeights = kMlpW3 + kMlpOut * kMlpHidden;  // 14336
+constexpr float kMlpLogKappaMin = -3.0f;
+constexpr float kMlpLogKappaMax = 8.0f;
+constexpr float kMlpSelMin = 0.05f;   // c = kMlpSelMin + kMlpSelSpan * sigmoid(t)
+constexpr float kMlpSelSpan = 0.9f;
+constexpr int kMlpMixF4 = 12;         // per-vertex mixture: 8 x (mu, kappa), pi[8], A[8]
+
+struct MlpView {
+  const uint32_t* table;  // [levels][T]: two bf16 features per entry
+  float* grad;            // [levels][T][2] fp32 gradient (training)
+  const uint16_t* w;      // kMlpWeights bf16
+  float lo[3];
+  float inv_size;         // position -> [0,1]^3 of the cubic box
+  uint32_t tmask;         // T - 1
+  uint32_t dense;         // bit l: level l indexed densely
+  float res[kMlpLevels];  // grid resolution of level l (integral)
+  uint32_t ready;         // trained: the renderer may sample from it
+};
+
+// ------------------------------------------------------------- bf16 helpers
+RTPG_HD uint16_t f2bf(float f) {
+  uint32_t u = fbits(f);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+RTPG_HD float bf2f(uint16_t h) { return bitsf((uint32_t)h << 16); }
+
+// ------------------------------------------------------------- hash encoding
+// Corner index of level l (instant-NGP spatial hash, or dense for coarse levels).
+RTPG_HD uint32_t mlp_index(const MlpView& m, int l, uint32_t x, uint32_t y, uint32_t z) {
+  if ((m.dense >> l) & 1u) {
+    uint32_t r1 = (uint32_t)m.res[l] + 1u;
+    return x + r1 * (y + r1 * z);
+  }
+  return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & m.tmask;
+}
+
+struct MlpCell {
+  uint32_t x, y, z;
+  float fx, fy, fz;
+};
+
+RTPG_HD MlpCell mlp_cell(const MlpView& m, int l, float px, float py, float pz) {
+  float r = m.res[l];
+  float gx = px * r, gy = py * r, gz = pz * r;
+  float ix = fminf(fmaxf(floorf(gx), 0.0f), r - 1.0f);
+  float iy = fminf(fmaxf(floorf(gy), 0.0f), r - 1.0f);
+  float iz = fminf(fmaxf(floorf(gz), 0.0f), r - 1.0f);
+  MlpCell c;
+  c.x = (uint32_t)ix;
+  c.y = (uint32_t)iy;
+  c.z = (uint32_t)iz;
+  c.fx = fminf(fmaxf(gx - ix, 0.0f), 1.0f);
+  c.fy = fminf(fmaxf(gy - iy, 0.0f), 1.0f);
+  c.fz = fminf(fmaxf(gz - iz, 0.0f), 1.0f);
+  return c;
+}
+
+// Trilinear interpolation of the two features of level l at p in [0,1]^3.
+__device__ __forceinline__ void mlp_encode_level(const MlpView& m, int l, float px, float py, float pz, float& f0,
+                                                 float& f1) {
+  MlpCell c = mlp_cell(m, l, px, py, pz);
+  const uint32_t* tab = m.table + (size_t)l * (m.tmask + 1u);
+  f0 = 0.0f;
+  f1 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    float w = (dx ? c.fx : 1.0f - c.fx) * (dy ? c.fy : 1.0f - c.fy) * (dz ? c.fz : 1.0f - c.fz);
+    uint32_t e = tab[mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz)];
+    f0 += w * bf2f((uint16_t)(e & 0xFFFFu));
+    f1 += w * bf2f((uint16_t)(e >> 16));
+  }
+}
+
+// Scatter d loss / d (features of level l) to the fp32 gradient table.
+__device__ __forceinline__ void mlp_encode_level_grad(const MlpView& m, int l, float px, float py, float pz, float g0,
+                                                      float g1) {
+  MlpCell c = mlp_cell(m, l, px, py, pz);
+  float* gt = m.grad + (size_t)l * (m.tmask + 1u) * 2u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    float w = (dx ? c.fx : 1.0f - c.fx) * (dy ? c.fy : 1.0f - c.fy) * (dz ? c.fz : 1.0f - c.fz);
+    uint32_t i = mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz);
+    if (g0 != 0.0f) unsafeAtomicAdd(gt + 2u * i, w * g0);
+    if (g1 != 0.0f) unsafeAtomicAdd(gt + 2u * i + 1u, w * g1);
+  }
+}
+
+// ------------------------------------------------------------- mixture
+// Unit-sphere vMF: v(w) = kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)).
+RTPG_HD float vmf_norm(float kappa) { return kappa / (2.0f * kPi * -expm1f(-2.0f * kappa)); }
+
+// Mixture density at unit direction d; mx: kMlpMixF4 float4 of one vertex.
+__device__ __forceinline__ float mix_pdf(const float4* mx, V3 d) {
+  float4 a0 = mx[10], a1 = mx[11];
+  float A[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  float q = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kMlpLobes; ++j) {
+    float4 l = mx[j];
+    q += A[j] * expf(l.w * (l.x * d.x + l.y * d.y + l.z * d.z - 1.0f));
+  }
+  return q;
+}
+
+// vMF sample around mu (W. Jakob, "Numerically stable sampling of the von
+// Mises Fisher distribution on S^2", 2012).
+RTPG_HD V3 vmf_sample(V3 mu, float kappa, float u1, float u2) {
+  float a = fmaxf(u1 + (1.0f - u1) * expf(-2.0f * kappa), 1e-38f);
+  float w = fminf(fmaxf(1.0f + logf(a) / kappa, -1.0f), 1.0f);
+  float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  float phi = 2.0f * kPi * u2;
+  Onb o = make_onb(mu);
+  return normalize(o.t * (s * cosf(phi)) + o.b * (s * sinf(phi)) + o.n * w);
+}
+
+__device__ __forceinline__ V3 mix_sample(const float4* mx, Pcg32& rng) {
+  float4 p0 = mx[8], p1 = mx[9];
+  float P[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+  float u = rnd(rng);
+  int j = kMlpLobes - 1;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kMlpLobes - 1; ++k) {
+    acc += P[k];
+    if (u < acc) {
+      j = k;
+      break;
+    }
+  }
+  float4 l = mx[0];
+#pragma unroll
+  for (int k = 1; k < kMlpLobes; ++k)
+    if (k == j) l = mx[k];
+  float u1 = rnd(rng), u2 = rnd(rng);
+  return vmf_sample(V3{l.x, l.y, l.z}, l.w, u1, u2);
+}
+
+}  // namespace rtpg

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mlp_dev.h"
 #include "views.h"
 
 namespace rtpg {
@@ -53,6 +54,18 @@ struct PathSoA {
   const float4* adjL;    // primal radiance per path
   float* grad_alb;       // 3 per leaf-order triangle
   float* grad_le;        // 3 per light
+  // guide vertices (vA, vB, shadow contributions) are recorded for pid < rec_limit
+  uint32_t rec_limit;
+  // learned guide: mixture per queue slot (kMlpMixF4 float4 + c), training
+  // vertices of the first mlp_rec paths ([5][depth][pid] float4) and the
+  // sample buffer they are turned into
+  float4* mix;
+  float* mixc;
+  float4* mv;
+  uint32_t mlp_rec;
+  float4* mlp_out;
+  uint32_t* mlp_count;
+  uint32_t mlp_cap;
 };
 
 constexpr int kTimedKernels = 5;   // extend, shade, shadow, accumulate, commit
@@ -62,7 +75,7 @@ hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n
                              bool any, uint32_t* deep, uint32_t* err, hipStream_t s);
 
 hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
-                               const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, int guide,
-                               float* accum, hipStream_t s, hipEvent_t* ev = nullptr);
+                               const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, const MlpView& ml,
+                               int guide, float* accum, hipStream_t s, hipEvent_t* ev = nullptr);
 
 }  // namespace rtpg

@@ -9,8 +9,11 @@
 // many work-items as there are live paths.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.h"
 #include "guide_dev.h"
+#include "mlp_tile.h"
 #include "render.h"
 #include "views.h"
 
@@ -492,8 +495,16 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
               vcell = grid_find_prv(hg, bucket, csum);
             }
           }
-          const bool guided = (kGuide != 0) && (rp.flags & kFlagGuideSample) && vcell != kInvalidCell;
-          const float alpha = rp.fraction;
+          // learned guide: the mixture the inference pass wrote for this queue slot
+          const float4* mx = nullptr;
+          float csel = 0.0f;
+          if (kGuide == 3 && (rp.flags & kFlagGuideSample)) {
+            mx = ps.mix + (size_t)i * kMlpMixF4;
+            csel = ps.mixc[i];
+          }
+          const bool guided = kGuide == 3 ? mx != nullptr
+                                          : (kGuide != 0) && (rp.flags & kFlagGuideSample) && vcell != kInvalidCell;
+          const float alpha = kGuide == 3 ? csel : rp.fraction;
           // ---- next-event estimation (SimpleTrace.cu:302-339)
           if (nee) {
             if (ris) {
@@ -545,9 +556,14 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
                 if (mis) {
                   float bp = be.pdf;
                   if (guided) {
-                    float cu, cv;
-                    dir_to_canonical(dir, cu, cv);
-                    float gp = kGuide == 1 ? dtree_pdf(sd, vcell, cu, cv) : mq_pdf(mq, vcell, cu, cv);
+                    float gp;
+                    if (kGuide == 3) {
+                      gp = mix_pdf(mx, dir);
+                    } else {
+                      float cu, cv;
+                      dir_to_canonical(dir, cu, cv);
+                      gp = kGuide == 1 ? dtree_pdf(sd, vcell, cu, cv) : mq_pdf(mq, vcell, cu, cv);
+                    }
                     bp = (1.0f - alpha) * bp + alpha * gp;
                   }
                   w = power_heuristic(pl, bp);
@@ -565,14 +581,18 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
           if (guided) {
             float us = rnd(rng);
             if (us < alpha) {
-              float gu, gv;
-              if (kGuide == 1) {
-                dtree_sample(sd, vcell, rng, gu, gv);
+              if (kGuide == 3) {
+                dir = mix_sample(mx, rng);
               } else {
-                float gp;
-                mq_sample(mq, vcell, rng, gu, gv, gp);
+                float gu, gv;
+                if (kGuide == 1) {
+                  dtree_sample(sd, vcell, rng, gu, gv);
+                } else {
+                  float gp;
+                  mq_sample(mq, vcell, rng, gu, gv, gp);
+                }
+                dir = normalize(canonical_to_dir(gu, gv));
               }
-              dir = normalize(canonical_to_dir(gu, gv));
             } else {
               dir = bsdf_sample(type, albedo, param, spec, n, wo, rng);
             }
@@ -582,8 +602,13 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
           BsdfEval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
           wo_pdf = be.pdf;
           if (guided) {
-            dir_to_canonical(dir, vcu, vcv);
-            float gp = kGuide == 1 ? dtree_pdf(sd, vcell, vcu, vcv) : mq_pdf(mq, vcell, vcu, vcv);
+            float gp;
+            if (kGuide == 3) {
+              gp = mix_pdf(mx, dir);
+            } else {
+              dir_to_canonical(dir, vcu, vcv);
+              gp = kGuide == 1 ? dtree_pdf(sd, vcell, vcu, vcv) : mq_pdf(mq, vcell, vcu, vcv);
+            }
             wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
           } else if (record && vcell != kInvalidCell) {
             dir_to_canonical(dir, vcu, vcv);
@@ -601,6 +626,19 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
           }
           bool bsdf_ok = finite3(be.f) && be.f.x >= 0.0f && be.f.y >= 0.0f && be.f.z >= 0.0f;
           vflags = bsdf_ok ? 0u : kVtxBadBsdf;
+          if (kGuide == 3 && record && pid < ps.mlp_rec) {
+            // training vertex: network inputs, sampled direction, BSDF density
+            // and the BSDF weight f |cos| / p_sample before Russian roulette
+            size_t mi = (size_t)depth * ps.mlp_rec + pid, st = (size_t)rp.max_depth * ps.mlp_rec;
+            float rough = type == kMatGGX ? param : 0.0f;
+            V3 bw = (wo_pdf > 0.0f) ? be.f * (fabsf(cosd) / wo_pdf) : V3{0.0f, 0.0f, 0.0f};
+            uint32_t ok = (wo_pdf > 0.0f && bsdf_ok) ? kVtxValid : 0u;
+            ps.mv[mi] = make_float4(p.x, p.y, p.z, rough);
+            ps.mv[st + mi] = make_float4(n.x, n.y, n.z, be.pdf);
+            ps.mv[2 * st + mi] = make_float4(wo.x, wo.y, wo.z, bitsf(ok));
+            ps.mv[3 * st + mi] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+            ps.mv[4 * st + mi] = make_float4(bw.x, bw.y, bw.z, 0.0f);
+          }
           nd = dir;
           meta_next = depth + 1u;
           cont = max3(Tn) > 0.0f && isfinite(max3(Tn));
@@ -611,13 +649,19 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
     float4 L4 = ps.L[pid];
     ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
     ps.nhit[pid] = depth + 1u;
-    if (record) {
+    if (record && pid < ps.rec_limit) {
       size_t vi = (size_t)depth * ps.n + pid;
       ps.vB[vi] = make_float4(c.x, c.y, c.z, cos_out);
-      bool has_vertex = cont && vcell != kInvalidCell;
       ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
-      ps.vC[vi] = make_float4(vcu, vcv, bitsf(has_vertex ? vcell : kInvalidCell),
-                              bitsf(vflags | (has_vertex ? kVtxValid : 0u)));
+      if (kGuide != 3) {
+        bool has_vertex = cont && vcell != kInvalidCell;
+        ps.vC[vi] = make_float4(vcu, vcv, bitsf(has_vertex ? vcell : kInvalidCell),
+                                bitsf(vflags | (has_vertex ? kVtxValid : 0u)));
+      } else if (pid < ps.mlp_rec && (!cont || (vflags & kVtxDelta))) {
+        // no training vertex here (light, miss, delta or absorbed path)
+        size_t st = (size_t)rp.max_depth * ps.mlp_rec;
+        ps.mv[2 * st + (size_t)depth * ps.mlp_rec + pid] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
+      }
     }
     if (cont) {
       V3 ndir = nd;
@@ -666,7 +710,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uin
       float4 le = sc.lights[4u * li + 3u];
       adj_add_le(ps, li, adj_weight(ps, pid), V3{c.x, c.y, c.z}, V3{le.x, le.y, le.z});
     }
-    if (record) {
+    if (record && pid < ps.rec_limit) {
       size_t vi = (size_t)depth * ps.n + pid;
       float4 vb = ps.vB[vi];
       ps.vB[vi] = make_float4(vb.x + c.x, vb.y + c.y, vb.z + c.z, vb.w);
@@ -891,6 +935,117 @@ __global__ void __launch_bounds__(kApplyBlock) k_commit_apply(const float4* recs
   }
 }
 
+// ------------------------------------------------------------------ learned guide
+// Inference pass of one bounce: for every live path of the queue, the network
+// maps (hit point, direction towards the previous vertex, front-facing normal,
+// roughness) to the vMF mixture and the selection probability, written per
+// queue slot for k_shade.  Persistent work-groups of 4 waves x 16 paths keep
+// the bf16 weights in LDS.
+constexpr uint32_t kInferBlock = 256;
+constexpr uint32_t kInferGrid = 2048;
+
+__global__ void __launch_bounds__(kInferBlock) k_mlp_infer(SceneView sc, PathSoA ps, MlpView m, uint32_t depth) {
+  __shared__ uint16_t lw[kLdsWeights];
+  __shared__ float hs[4][16 * kHeadPitch];
+  const uint32_t n_act = ps.count[depth];
+  if (blockIdx.x * 64u >= n_act) return;  // uniform over the work-group
+  mlp_load_weights(lw, m.w);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15;
+  float* row = hs[wave] + r * kHeadPitch;
+  for (uint32_t tile = blockIdx.x; tile * 64u < n_act; tile += gridDim.x) {
+    uint32_t i = tile * 64u + 16u * (uint32_t)wave + (uint32_t)r;
+    bool valid = i < n_act;
+    MlpInput in{V3{0, 0, 0}, V3{0, 0, 1}, V3{0, 0, 1}, 0.0f};
+    if (valid) {
+      uint32_t pid = ps.queue[depth & 1][i];
+      float4 hit = ps.hit[pid];
+      int32_t tri = (int32_t)fbits(hit.w);
+      if (tri >= 0) {
+        float4 ro = ps.ray_o[pid], rd = ps.ray_d[pid];
+        V3 o{ro.x, ro.y, ro.z}, d{rd.x, rd.y, rd.z};
+        const float4* tq = reinterpret_cast<const float4*>(sc.tris + tri);
+        V3 ng = normalize(cross(load3(tq[1]), load3(tq[2])));
+        in.p = o + d * hit.x;
+        in.n = dot(d, ng) < 0.0f ? ng : -ng;
+        in.wo = -d;
+        in.rough = tri_type(sc.info[tri]) == kMatGGX ? sc.shade[tri].w : 0.0f;
+      }
+    }
+    MlpActs act;
+    mlp_inputs(m, in, lane, act.x0);
+    f32x4_t o[3];
+    mlp_forward(lw, act, o, lane);
+    mlp_spill(hs[wave], o, lane);
+    __syncthreads();
+    MlpHead h = mlp_head(row, lane);
+    if (valid) mlp_write_mix(h, reinterpret_cast<float*>(ps.mix + (size_t)i * kMlpMixF4), ps.mixc + i, lane);
+    __syncthreads();
+  }
+}
+
+// Training samples of the first mlp_rec paths: the radiance that reached
+// vertex d from its sampled direction is the sum of the contributions of the
+// later hits divided by the throughput after the bounce (per channel, guarded
+// as in the SD-tree record); the sample weight is avg_rgb(f |cos| / p_sample *
+// Li).  Appended with one atomic per wave; samples beyond the capacity are
+// dropped.
+__global__ void __launch_bounds__(kBlock) k_mlp_emit(PathSoA ps, uint32_t max_depth, uint32_t max_vtx) {
+  uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
+  bool in = pid < ps.n && pid < ps.mlp_rec;
+  uint32_t nh = in ? ps.nhit[pid] : 0u;
+  uint32_t nv = nh > 0 ? nh - 1u : 0u;
+  nv = nv < max_vtx ? nv : max_vtx;
+  uint32_t nh_max = nh;
+  for (int off = 32; off >= 1; off >>= 1) nh_max = max(nh_max, (uint32_t)__shfl_xor((int)nh_max, off));
+  const size_t st = (size_t)max_depth * ps.mlp_rec;
+  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+  for (int d = (int)nh_max - 2; d >= 0; --d) {
+    if ((uint32_t)d + 1u < nh) {
+      float4 b = ps.vB[(size_t)(d + 1) * ps.n + pid];
+      rx += b.x;
+      ry += b.y;
+      rz += b.z;
+    }
+    bool ok = false;
+    float w = 0.0f;
+    size_t mi = (size_t)d * ps.mlp_rec + pid;
+    float4 mc{0, 0, 0, 0};
+    if ((uint32_t)d < nv) {
+      mc = ps.mv[2 * st + mi];
+      if (fbits(mc.w) & kVtxValid) {
+        float4 va = ps.vA[(size_t)d * ps.n + pid];
+        float4 bw = ps.mv[4 * st + mi];
+        float pdf = va.w;
+        bool fin = isfinite(rx) && rx >= 0.0f && isfinite(ry) && ry >= 0.0f && isfinite(rz) && rz >= 0.0f;
+        if (fin && pdf > 0.0f) {
+          float lx = va.x * pdf > 1e-4f ? rx / va.x : 0.0f;
+          float ly = va.y * pdf > 1e-4f ? ry / va.y : 0.0f;
+          float lz = va.z * pdf > 1e-4f ? rz / va.z : 0.0f;
+          w = (bw.x * lx + bw.y * ly + bw.z * lz) / 3.0f;
+          ok = isfinite(w) && w > 0.0f;
+        }
+      }
+    }
+    uint64_t mask = __ballot(ok);
+    if (!mask) continue;
+    uint32_t lane = lane_id();
+    uint32_t base = 0;
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    if (lane == leader) base = atomicAdd(ps.mlp_count, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    if (!ok) continue;
+    uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (slot >= ps.mlp_cap) continue;
+    float4 ma = ps.mv[mi], mb = ps.mv[st + mi], md = ps.mv[3 * st + mi];
+    float4* o = ps.mlp_out + 4u * (size_t)slot;
+    o[0] = ma;
+    o[1] = mb;
+    o[2] = make_float4(mc.x, mc.y, mc.z, w);
+    o[3] = md;
+  }
+}
+
 // Ray counters of a chunk: paths, radiance rays (live paths per depth) and
 // shadow rays, accumulated in 64 bits.
 __global__ void k_stats_add(const uint32_t* count, const uint32_t* scount, uint32_t depth, uint32_t n,
@@ -924,8 +1079,8 @@ hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n
 }
 
 hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
-                               const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, int guide,
-                               float* accum, hipStream_t s, hipEvent_t* ev) {
+                               const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, const MlpView& ml,
+                               int guide, float* accum, hipStream_t s, hipEvent_t* ev) {
   uint32_t nb = blocks_for(ps.n, kBlock);
   uint32_t ncnt = 2u * (rp.max_depth + 2u);
   hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(ncnt, 64)), dim3(64), 0, s, ps.count, ncnt);
@@ -958,6 +1113,13 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
       hipLaunchKernelGGL(k_shade<1>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     } else if (guide == 2) {
       hipLaunchKernelGGL(k_shade<2>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+    } else if (guide == 3) {
+      // the learned guide's inference pass is timed with the shade class
+      if (rp.flags & kFlagGuideSample) {
+        uint32_t ni = std::min(blocks_for(ps.n, 64u), kInferGrid);
+        hipLaunchKernelGGL(k_mlp_infer, dim3(ni), dim3(kInferBlock), 0, s, sc, ps, ml, d);
+      }
+      hipLaunchKernelGGL(k_shade<3>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     } else {
       hipLaunchKernelGGL(k_shade<0>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     }
@@ -989,6 +1151,10 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
       hipLaunchKernelGGL(k_commit_apply<2>, dim3(na), dim3(kApplyBlock), 0, s, ps.recs, ps.nhit, ps.n, cap, sd,
                          mq);
     }
+  }
+  if (record && guide == 3) {
+    uint32_t ne = blocks_for(std::min(ps.n, ps.mlp_rec), kBlock);
+    hipLaunchKernelGGL(k_mlp_emit, dim3(ne), dim3(kBlock), 0, s, ps, rp.max_depth, rp.max_depth - 1u);
   }
   mark(4, 0, 1);
   return hipGetLastError();

@@ -128,7 +128,7 @@ class Context:
     # -- render
     def render(self, spp=1, flags=FLAG_NEE | FLAG_MIS | FLAG_RR, max_depth=8, rr_depth=3, num_candidates=32,
                seed=0, sample_offset=0, bg=(0.0, 0.0, 0.0), fraction=0.3, sdtree=None, hashgrid=None,
-               mqtree=None, max_paths=0):
+               mqtree=None, max_paths=0, mlp=None):
         p = _lib.RenderParams()
         p.flags = flags
         p.spp = spp
@@ -143,6 +143,7 @@ class Context:
         p.hashgrid = hashgrid.h if hashgrid is not None else None
         p.mqtree = mqtree.h if mqtree is not None else None
         p.max_paths = max_paths
+        p.mlp = mlp.h if mlp is not None else None
         call("rtpg_render", self.h, C.byref(p))
 
     def render_stats(self):
@@ -505,4 +506,149 @@ def render_guided(ctx: Context, ctrl, guide, kind="tree", base_flags=FLAG_NEE | 
         ctrl.EndTrace(spp)
         if on_pass is not None:
             on_pass(state, spp, traced)
+    return traced
+
+
+# ---------------------------------------------------------------- learned guide
+MLP_WEIGHTS = 14336
+MLP_LEVELS = 16
+
+
+class MLPGuide:
+    """Learned directional guide (rtpg_mlp_*): hash-grid encoding + 3 x 64 ReLU
+    MLP on the matrix cores -> 8-lobe vMF mixture and the probability c of
+    sampling it against the BSDF.  No counterpart in the reference; it sits
+    beside RTSTreeWrapper and is passed to Context.render(mlp=...) with
+    FLAG_USE_MLP.
+
+    Training samples are dicts of arrays: pos, wo, nrm (n x 3), rough (n),
+    omega (n x 3, sampled direction), weight (n, f Li |cos| / p_sample),
+    pdf_bsdf (n)."""
+
+    def __init__(self, ctx: Context, aabb_min, aabb_max, log2_table=19, base_res=16.0, max_res=2048.0, lr=1e-2,
+                 beta1=0.9, beta2=0.99, eps=1e-15, batch=1 << 18, capacity=1 << 20, record_paths=1 << 16, seed=0):
+        cfg = _lib.MlpConfig(log2_table, base_res, max_res, lr, beta1, beta2, eps, batch, capacity, record_paths,
+                             seed)
+        h = C.c_void_p()
+        call("rtpg_mlp_create", ctx.h, ptr(_f32(aabb_min)), ptr(_f32(aabb_max)), C.byref(cfg), C.byref(h))
+        self.h = h.value
+        self.ctx = ctx
+        self.log2_table = log2_table
+
+    def close(self):
+        if getattr(self, "h", None):
+            call("rtpg_mlp_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def reset(self, seed=0):
+        call("rtpg_mlp_reset", self.h, seed)
+
+    def info(self):
+        i = np.zeros(7, np.uint64)
+        res = np.zeros(16, np.float32)
+        call("rtpg_mlp_info", self.h, ptr(i, C.c_uint64), ptr(res))
+        return {"weights": int(i[0]), "table_floats": int(i[1]), "recorded": int(i[2]), "steps": int(i[3]),
+                "levels": int(i[4]), "entries": int(i[5]), "dense_mask": int(i[6]), "res": res}
+
+    @staticmethod
+    def _q(pos, wo, nrm, rough):
+        pos, wo, nrm = _f32(pos, (-1, 3)), _f32(wo, (-1, 3)), _f32(nrm, (-1, 3))
+        rough = _f32(rough, (-1,))
+        n = pos.shape[0]
+        assert wo.shape[0] == n and nrm.shape[0] == n and rough.shape[0] == n
+        return pos, wo, nrm, rough, n
+
+    def forward(self, pos, wo, nrm, rough):
+        """(raw outputs n x 41, mixture n x 41: 8 x (pi, mu xyz, kappa), c)."""
+        pos, wo, nrm, rough, n = self._q(pos, wo, nrm, rough)
+        raw = np.zeros((n, 41), np.float32)
+        mix = np.zeros((n, 41), np.float32)
+        call("rtpg_mlp_forward", self.h, ptr(pos), ptr(wo), ptr(nrm), ptr(rough), n, ptr(raw), ptr(mix))
+        return raw, mix
+
+    def sample(self, mixture, seed=0):
+        """One direction per row of a forward() mixture: (n x 4) = w xyz, q(w)."""
+        mix = _f32(mixture, (-1, 41))
+        out = np.zeros((mix.shape[0], 4), np.float32)
+        call("rtpg_mlp_sample", self.h, ptr(mix), mix.shape[0], seed, ptr(out))
+        return out
+
+    @staticmethod
+    def _s(s):
+        pos, wo, nrm, rough, n = MLPGuide._q(s["pos"], s["wo"], s["nrm"], s["rough"])
+        om = _f32(s["omega"], (-1, 3))
+        w = _f32(s["weight"], (-1,))
+        pb = _f32(s["pdf_bsdf"], (-1,))
+        assert om.shape[0] == n and w.shape[0] == n and pb.shape[0] == n
+        return pos, wo, nrm, rough, om, w, pb, n
+
+    def gradient(self, samples, table=True):
+        """(loss, d loss / d weights [14336], d loss / d table or None) of the mean loss."""
+        pos, wo, nrm, rough, om, w, pb, n = self._s(samples)
+        loss = np.zeros(1, np.float32)
+        gw = np.zeros(MLP_WEIGHTS, np.float32)
+        gt = np.zeros(MLP_LEVELS * (1 << self.log2_table) * 2, np.float32) if table else None
+        call("rtpg_mlp_gradient", self.h, ptr(pos), ptr(wo), ptr(nrm), ptr(rough), ptr(om), ptr(w), ptr(pb), n,
+             ptr(loss), ptr(gw), ptr(gt) if table else None)
+        return float(loss[0]), gw, gt
+
+    def train(self, samples, steps=1):
+        pos, wo, nrm, rough, om, w, pb, n = self._s(samples)
+        loss = np.zeros(max(steps, 1), np.float32)
+        call("rtpg_mlp_train", self.h, ptr(pos), ptr(wo), ptr(nrm), ptr(rough), ptr(om), ptr(w), ptr(pb), n, steps,
+             ptr(loss))
+        return loss[:steps]
+
+    def train_recorded(self, steps=1):
+        loss = np.zeros(max(steps, 1), np.float32)
+        used = np.zeros(1, np.uint32)
+        call("rtpg_mlp_train_recorded", self.h, steps, ptr(loss), ptr(used, C.c_uint32))
+        return loss[:steps], int(used[0])
+
+    def export(self):
+        w = np.zeros(MLP_WEIGHTS, np.float32)
+        t = np.zeros(MLP_LEVELS * (1 << self.log2_table) * 2, np.float32)
+        call("rtpg_mlp_export", self.h, ptr(w), ptr(t))
+        return w, t
+
+    def import_(self, weights=None, table=None):
+        w = _f32(weights) if weights is not None else None
+        t = _f32(table) if table is not None else None
+        call("rtpg_mlp_import", self.h, ptr(w) if w is not None else None, ptr(t) if t is not None else None)
+
+
+def render_mlp_guided(ctx: Context, mlp: MLPGuide, budget, base_flags=FLAG_NEE | FLAG_MIS | FLAG_RR, max_depth=8,
+                      rr_depth=3, seed=0, train_passes=8, pass_spp=1, steps_per_pass=4, max_paths=0, on_pass=None):
+    """Online schedule: train_passes passes of pass_spp samples record training
+    samples (guided by the network once it has been trained at least once),
+    each followed by steps_per_pass Adam steps; the rest of the budget is one
+    final guided pass.  Every pass accumulates into the image.  Returns the
+    number of samples traced."""
+    traced = 0
+    trained = False
+    for k in range(train_passes):
+        if traced + pass_spp > budget:
+            break
+        flags = base_flags | FLAG_USE_MLP | (FLAG_GUIDE_SAMPLE if trained else 0)
+        ctx.render(spp=pass_spp, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed,
+                   sample_offset=traced, mlp=mlp, max_paths=max_paths)
+        traced += pass_spp
+        loss, used = mlp.train_recorded(steps_per_pass)
+        trained = trained or used > 0
+        if on_pass is not None:
+            on_pass(k, pass_spp, traced, loss, used)
+    rest = budget - traced
+    if rest > 0:
+        flags = base_flags | FLAG_USE_MLP | FLAG_FINAL | (FLAG_GUIDE_SAMPLE if trained else 0)
+        ctx.render(spp=rest, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed, sample_offset=traced,
+                   mlp=mlp, max_paths=max_paths)
+        traced += rest
+        if on_pass is not None:
+            on_pass(-1, rest, traced, None, 0)
     return traced

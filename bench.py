@@ -1,18 +1,20 @@
-"""Benchmark: SD-tree-guided vs plain path tracing on one MI355X.
+"""Benchmark: guided (SD-tree, learned) vs plain path tracing on one MI355X.
 
 Workload (the guided-room configuration): room with a half-open dividing wall,
 a small ceiling light (0.5% of the ceiling) hidden behind it, a 1024 x 512
 fBm heightfield floor (~1.05M triangles); Lambertian albedo U[0.2,0.8] per
 triangle with 20% GGX (roughness U[0.05,0.5]); 1280 x 720, path depth 10, NEE
-with MIS and Russian roulette; SD-tree training passes of 1, 2, 4, ... spp
+with MIS and Russian roulette; SD-tree training passes of 1, 2, 4, ..., 64 spp
 (split threshold c = 12000 * sqrt(2^k), flux threshold 0.01, DTree depth 20)
-up to a 64-spp budget.
+and a final pass, 256 spp in all.
 
-One step = one complete guided render of the budget (all training passes,
-recording, refinement and the final pass).  `value` is Msamples/s (camera
-paths per second) of the guided render.  The line also carries the plain path
-tracer's rate, the relMSE of both images at equal wall-clock time against a
-high-spp reference, the roofline of the dominant kernel and the CPU baseline.
+One step = one complete SD-tree-guided render of the budget (all training
+passes, recording, refinement and the final pass).  `value` is Msamples/s
+(camera paths per second) of that render.  The line also carries the plain
+path tracer's rate, the relMSE at equal wall-clock time of the SD-tree-guided
+and the learned-guide renders against plain path tracing (reference: a
+high-spp plain render), the roofline of the dominant kernel and the CPU
+baseline.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -52,8 +54,11 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--budget", type=int, default=64)
-    ap.add_argument("--ref-spp", type=int, default=1024)
+    ap.add_argument("--budget", type=int, default=256,
+                    help="samples per pixel of one guided render: SD-tree passes 1, 2, 4, .., 64 and the final pass")
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--mlp-passes", type=int, default=8)
+    ap.add_argument("--mlp-steps", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--small", action="store_true", help="reduced size for a quick functional check")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -68,8 +73,8 @@ def main(argv=None):
         return 2
 
     from rtlib_amd import scenes
-    from rtlib_amd.tracer import (FLAG_MIS, FLAG_NEE, FLAG_RR, Context, RTSTreeController, RTSTreeWrapper,
-                                  render_guided)
+    from rtlib_amd.tracer import (FLAG_MIS, FLAG_NEE, FLAG_RR, Context, MLPGuide, RTSTreeController, RTSTreeWrapper,
+                                  render_guided, render_mlp_guided)
 
     W, H = (320, 180) if a.small else (1280, 720)
     grid = (256, 128) if a.small else (1024, 512)
@@ -160,13 +165,32 @@ def main(argv=None):
     dt_plain, _ = timed(lambda: ctx.render(spp=a.budget, flags=base, max_depth=depth, seed=77))
     plain_rate = a.budget * npix / dt_plain / 1e6
     st_plain = ctx.render_stats()
+    plain_budget_img = ctx.image()
     quality = None
     if a.ref_spp > 0:
-        # equal time: as many plain samples as the guided step's wall clock allows
-        spp_eq = max(1, int(round(t_step * plain_rate * 1e6 / npix)))
-        ctx.clear()
-        ctx.render(spp=spp_eq, flags=base, max_depth=depth, seed=78)
-        plain_img = ctx.image()
+        # learned guide on the same budget: a.mlp_passes recording passes of 1
+        # spp, each followed by a.mlp_steps Adam steps, then one final guided pass
+        mlp = MLPGuide(ctx, lo, hi, seed=1)
+
+        def mlp_step():
+            mlp.reset(1)
+            ctx.clear()
+            return render_mlp_guided(ctx, mlp, a.budget, base, max_depth=depth, seed=11,
+                                     train_passes=a.mlp_passes, steps_per_pass=a.mlp_steps)
+
+        timed(mlp_step)
+        t_mlp, _ = timed(mlp_step)
+        mlp_img = ctx.image()
+        log(f"learned guide step: {t_mlp * 1e3:.1f} ms ({a.budget} spp)")
+
+        def plain_equal_time(t, seed):
+            spp = max(1, int(round(t * plain_rate * 1e6 / npix)))
+            ctx.clear()
+            ctx.render(spp=spp, flags=base, max_depth=depth, seed=seed)
+            return spp, ctx.image()
+
+        spp_tree, plain_tree = plain_equal_time(t_step, 78)
+        spp_mlp, plain_mlp = plain_equal_time(t_mlp, 79)
         # reference image (plain, many samples, independent seed)
         t0 = time.time()
         ctx.clear()
@@ -177,8 +201,16 @@ def main(argv=None):
             done += k
         ref = ctx.image()
         log(f"reference {a.ref_spp} spp in {time.time() - t0:.1f} s")
-        quality = {"guided": round(rel_mse(guided_img, ref), 5), "plain": round(rel_mse(plain_img, ref), 5),
-                   "plain_spp": spp_eq, "guided_spp": a.budget, "reference_spp": a.ref_spp}
+        quality = {
+            "reference_spp": a.ref_spp,
+            "plain_at_budget": {"spp": a.budget, "relmse": round(rel_mse(plain_budget_img, ref), 5)},
+            "sdtree": {"ms": round(t_step * 1e3, 1), "relmse": round(rel_mse(guided_img, ref), 5),
+                       "plain_spp": spp_tree, "plain_relmse": round(rel_mse(plain_tree, ref), 5)},
+            "mlp": {"ms": round(t_mlp * 1e3, 1), "relmse": round(rel_mse(mlp_img, ref), 5),
+                    "plain_spp": spp_mlp, "plain_relmse": round(rel_mse(plain_mlp, ref), 5),
+                    "train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps},
+        }
+        mlp.close()
 
     # ---- roofline of the dominant kernel (k_extend): algorithmic bytes of
     # step 0 over its event-timed k_extend time
@@ -215,7 +247,8 @@ def main(argv=None):
         "dtype": "fp32",
         "data": "synthetic (procedural scene, seeded)",
         "config": {"workload": "guided room: 1.05M-tri fBm heightfield, hidden light, 1280x720, depth 10, "
-                               "SD-tree passes 1,2,4.. to 64 spp" if not a.small else "guided room (small)",
+                               f"SD-tree passes 1,2,4..64 spp + final pass, {a.budget} spp budget" if not a.small
+                   else "guided room (small)",
                    "width": W, "height": H, "triangles": int(sc["idx"].shape[0]), "budget_spp": a.budget,
                    "max_depth": depth, "parallelism": "single GPU"},
         "plain_msamples_per_s": round(plain_rate, 2),

@@ -102,6 +102,10 @@ int rtpg_frame_resize(rtpg_context ctx, uint32_t width, uint32_t height);
 int rtpg_accum_clear(rtpg_context ctx);
 /* out: 3*W*H floats (sum of samples, row 0 = bottom), spp: samples accumulated. */
 int rtpg_accum_download(rtpg_context ctx, float* out, uint32_t* spp);
+/* Display frame of the accumulated image: 4*W*H bytes RGBA (row 0 = bottom),
+ * x = sum/spp, x/(1+x), gamma v <= 0.00174 ? 32v : v^(1/2.2), min(int(255.99 g),
+ * 255) -- the frame the reference's raygen writes (Test/cuda/SimpleTrace.cu:91-101). */
+int rtpg_frame_tonemap(rtpg_context ctx, uint8_t* rgba);
 
 /* --- render -----------------------------------------------------------------
  * Replaces the Params H2D copy + TracerData::Launch -> optixLaunch

@@ -616,14 +616,18 @@ static float dtree_mean(float sum, float sw) { /* DTreeT::GetMean :444-448 */
   return factor * sum;
 }
 
-/* DTreeNodeT::Sample :228-277 + clamp of DTreeT::Sample :477-482 */
+/* DTreeNodeT::Sample :228-277 + clamp of DTreeT::Sample :477-482; *pdf: the
+ * sphere density of the drawn point accumulated along the same descent, as
+ * DTreeNodeT::SampleAndPdf :178-227 does (4 s_i / sum per level, 1 / (4 pi)). */
 static void dtree_sample1(const float* sums, const uint16_t* child, float sum, float sw, pcg32* r, float* ox,
-                          float* oy) {
+                          float* oy, float* pdf) {
   if (dtree_mean(sum, sw) <= 0.0f) {
     *ox = rnd(r);
     *oy = rnd(r);
+    *pdf = 1.0f * INV_2PI_F / 2.0f;
     return;
   }
+  float result = 1.0f;
   uint32_t node = 0;
   float rx = 0.0f, ry = 0.0f, size = 1.0f;
   for (;;) {
@@ -651,6 +655,7 @@ static void dtree_sample1(const float* sums, const uint16_t* child, float sum, f
     rx = rx + size * ox0;
     ry = ry + size * oy0;
     size *= 0.5f;
+    result = s[idx] > 0.0f ? result * (4.0f * s[idx] / total) : 0.0f;
     uint16_t c = child[4 * node + idx];
     if (c == 0 || s[idx] <= 0.0f) {
       float a = rnd(r);
@@ -663,6 +668,7 @@ static void dtree_sample1(const float* sums, const uint16_t* child, float sum, f
   }
   *ox = fminf(fmaxf(rx, 0.0f), 1.0f);
   *oy = fminf(fmaxf(ry, 0.0f), 1.0f);
+  *pdf = result * INV_2PI_F / 2.0f;
 }
 
 /* DTreeNodeT::Pdf :278-316 times 1/(4 pi) (DTreeT::Pdf :484-489) */
@@ -692,8 +698,9 @@ void orc_sdtree_sample(const uint32_t* s_base, const float* s_sum, const float* 
   for (uint32_t i = 0; i < n; ++i) {
     pcg32 r = pcg32_seed(seed, i);
     uint32_t c = cells[i];
+    float pdf;
     dtree_sample1(s_sums + 4 * (size_t)s_base[c], s_child + 4 * (size_t)s_base[c], s_sum[c], s_sw[c], &r,
-                  &out[2 * i], &out[2 * i + 1]);
+                  &out[2 * i], &out[2 * i + 1], &pdf);
   }
 }
 
@@ -1393,11 +1400,12 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
             }
           }
           v3 dir;
+          float tree_pdf = -1.0f; /* density of a direction drawn from the tree */
           if (guided) {
             float us = rnd(&r);
             if (us < alpha) {
               float gu, gv;
-              dtree_sample1(gs, gc, g->s_sum[cell], g->s_sw[cell], &r, &gu, &gv);
+              dtree_sample1(gs, gc, g->s_sum[cell], g->s_sw[cell], &r, &gu, &gv, &tree_pdf);
               dir = nrm(canonical_to_dir(gu, gv));
             } else {
               dir = bsdf_sample(type, albedo, param, spec, n, wo, &r);
@@ -1409,7 +1417,7 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
           wo_pdf = be.pdf;
           if (guided) {
             dir_to_canonical(dir, &vx.cu, &vx.cv);
-            float gp = dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], vx.cu, vx.cv);
+            float gp = tree_pdf >= 0.0f ? tree_pdf : dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], vx.cu, vx.cv);
             wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
           } else if (record && cell != INVALID) {
             dir_to_canonical(dir, &vx.cu, &vx.cv);

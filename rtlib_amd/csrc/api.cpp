@@ -287,6 +287,19 @@ int rtpg_accum_download(rtpg_context ctx, float* out, uint32_t* spp) {
   });
 }
 
+int rtpg_frame_tonemap(rtpg_context ctx, uint8_t* rgba) {
+  return guarded([&] {
+    require(ctx && ctx->accum.p && rgba, "NULL argument or frame not sized");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    uint32_t npix = ctx->width * ctx->height;
+    DBuf<uchar4> frame;
+    frame.alloc(npix);
+    float inv = ctx->accum_spp ? 1.0f / (float)ctx->accum_spp : 0.0f;
+    RTPG_HIP(launch_tonemap(ctx->accum.p, npix, inv, frame.p, ctx->stream));
+    frame.download(reinterpret_cast<uchar4*>(rgba), npix, ctx->stream);
+  });
+}
+
 }  // extern "C"
 
 namespace rtpg {

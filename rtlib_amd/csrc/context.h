@@ -77,6 +77,9 @@ struct rtpg_sdtree_t {
   };
   std::vector<SNode> snodes;
   rtpg::DBuf<uint32_t> d_stree;
+  rtpg::DBuf<uint32_t> d_jump;  // lookup table of the first 3 * kJumpBits levels
+  static constexpr uint32_t kJumpBits = 5;
+  void upload_stree();
   // device DTree arrays (owned)
   struct Owned {
     uint32_t n = 0, b_nodes = 0, s_nodes = 0;

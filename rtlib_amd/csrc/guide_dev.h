@@ -15,11 +15,33 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 // The split axis of a node is its depth mod 3 (root x, SubDivide :1240).
 // Unlike the reference the child index is clamped to {0,1}, so a point on the
 // upper face of the box cannot index past the pair.
+//
+// The first 3G levels are resolved with one load from a table over the 2^G
+// cubed grid of the box (entry: the word of the node reached after 3G
+// levels, or the leaf met before).  The G child indices taken along an axis
+// are the bits of clamp(floor(q 2^G), 0, 2^G - 1), and G doublings of q minus
+// those bits leave exactly q 2^G - cell (power-of-two scaling and the
+// subtraction of an integer in [q 2^G - 1, q 2^G] are exact in float), so the
+// descent continues from the same node with the same coordinates.
 __device__ __forceinline__ uint32_t stree_lookup(const SdTreeView& g, V3 p) {
   float qx = (p.x - g.aabb_min[0]) / g.aabb_size;
   float qy = (p.y - g.aabb_min[1]) / g.aabb_size;
   float qz = (p.z - g.aabb_min[2]) / g.aabb_size;
-  uint32_t w = g.stree[0];
+  uint32_t w;
+  if (g.jump) {
+    const uint32_t n = 1u << g.jump_bits;
+    const float sn = (float)n;
+    float gx = qx * sn, gy = qy * sn, gz = qz * sn;
+    float cx = fminf(fmaxf(floorf(gx), 0.0f), sn - 1.0f);
+    float cy = fminf(fmaxf(floorf(gy), 0.0f), sn - 1.0f);
+    float cz = fminf(fmaxf(floorf(gz), 0.0f), sn - 1.0f);
+    w = g.jump[((uint32_t)cz * n + (uint32_t)cy) * n + (uint32_t)cx];
+    qx = gx - cx;
+    qy = gy - cy;
+    qz = gz - cz;
+  } else {
+    w = g.stree[0];
+  }
   uint32_t axis = 0;
   while (!(w & kLeafBit)) {
     float& q = axis == 0 ? qx : (axis == 1 ? qy : qz);
@@ -93,16 +115,20 @@ __device__ __forceinline__ float dtree_pdf(const SdTreeView& g, uint32_t cell, f
 }
 
 // DTreeNodeT::Sample (:228-277) with the clamp of DTreeT::Sample (:477-482):
-// canonical direction drawn from the sampling tree of DTree `cell`.
+// canonical direction drawn from the sampling tree of DTree `cell`.  pdf: the
+// sphere density of the drawn point as DTreeNodeT::SampleAndPdf (:178-227)
+// accumulates it along the same descent (4 s_i / sum per level, 1 / (4 pi)).
 __device__ __forceinline__ void dtree_sample(const SdTreeView& g, uint32_t cell, Pcg32& rng,
-                                             float& ox, float& oy) {
+                                             float& ox, float& oy, float& pdf) {
 #pragma clang fp contract(off)
   float4 inf = g.info[cell];
   if (dtree_mean(inf.y, inf.z) <= 0.0f) {
     ox = rnd(rng);
     oy = rnd(rng);
+    pdf = 1.0f * kInv2Pi / 2.0f;
     return;
   }
+  float result = 1.0f;
   uint32_t base = fbits(inf.x);
   uint32_t node = 0;
   float rx = 0.0f, ry = 0.0f, size = 1.0f;
@@ -133,8 +159,10 @@ __device__ __forceinline__ void dtree_sample(const SdTreeView& g, uint32_t cell,
     rx = rx + t0;
     ry = ry + t1;
     size *= 0.5f;
+    float si = f4c(s, idx);
+    result = si > 0.0f ? result * (4.0f * si / total) : 0.0f;
     uint32_t ch = dnode_child(g.samp_child[base + node], idx);
-    if (ch == 0 || f4c(s, idx) <= 0.0f) {
+    if (ch == 0 || si <= 0.0f) {
       float a = rnd(rng);
       float b = rnd(rng);
       float t2 = size * a;
@@ -147,6 +175,7 @@ __device__ __forceinline__ void dtree_sample(const SdTreeView& g, uint32_t cell,
   }
   ox = fminf(fmaxf(rx, 0.0f), 1.0f);
   oy = fminf(fmaxf(ry, 0.0f), 1.0f);
+  pdf = result * kInv2Pi / 2.0f;
 }
 
 // ---------------------------------------------------------------- hash grid

@@ -578,6 +578,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
           }
           // ---- direction sampling: BSDF or guide mixture (SimpleGuide.cu:359-377)
           V3 dir;
+          // density of a direction drawn from the SD-tree, from the sampling descent
+          float tree_pdf = -1.0f;
           if (guided) {
             float us = rnd(rng);
             if (us < alpha) {
@@ -586,7 +588,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
               } else {
                 float gu, gv;
                 if (kGuide == 1) {
-                  dtree_sample(sd, vcell, rng, gu, gv);
+                  dtree_sample(sd, vcell, rng, gu, gv, tree_pdf);
                 } else {
                   float gp;
                   mq_sample(mq, vcell, rng, gu, gv, gp);
@@ -607,7 +609,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
               gp = mix_pdf(mx, dir);
             } else {
               dir_to_canonical(dir, vcu, vcv);
-              gp = kGuide == 1 ? dtree_pdf(sd, vcell, vcu, vcv) : mq_pdf(mq, vcell, vcu, vcv);
+              if (kGuide == 1) {
+                gp = tree_pdf >= 0.0f ? tree_pdf : dtree_pdf(sd, vcell, vcu, vcv);
+              } else {
+                gp = mq_pdf(mq, vcell, vcu, vcv);
+              }
             }
             wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
           } else if (record && vcell != kInvalidCell) {
@@ -739,6 +745,30 @@ __global__ void k_accum(PathSoA ps, uint32_t npix, uint32_t spp, float* accum) {
   accum[3 * pix + 0] += sx;
   accum[3 * pix + 1] += sy;
   accum[3 * pix + 2] += sz;
+}
+
+// Display frame of the accumulated image (SimpleTrace.cu:91-101): x = sum /
+// spp, x / (1 + x), the reference's gamma approximation (Math.h:621-627:
+// v <= 0.00174 ? 32 v : v^(1/2.2)), then min(int(255.99 g), 255); alpha 255.
+__global__ void k_tonemap(const float* accum, uint32_t npix, float inv_spp, uchar4* frame) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  unsigned char c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float x = accum[3 * i + k] * inv_spp;
+    x = x / (1.0f + x);
+    float g = (x <= 0.00174f) ? (32.0f * x) : powf(x, 1.0f / 2.2f);
+    int v = (int)(g * 255.99f);
+    c[k] = (unsigned char)(v < 255 ? (v > 0 ? v : 0) : 255);
+  }
+  frame[i] = make_uchar4(c[0], c[1], c[2], 255);
+}
+
+hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, uchar4* frame, hipStream_t s) {
+  if (npix == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tonemap, dim3((npix + 255u) / 256u), dim3(256), 0, s, accum, npix, inv_spp, frame);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ guide records

@@ -249,7 +249,8 @@ __global__ void k_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n,
   if (i >= n) return;
   Pcg32 rng = pcg32_seed(seed, i);
   float u, w;
-  dtree_sample(v, cells[i], rng, u, w);
+  float pdf;
+  dtree_sample(v, cells[i], rng, u, w, pdf);
   out[2 * i] = u;
   out[2 * i + 1] = w;
 }

@@ -96,9 +96,35 @@ static std::vector<uint32_t> encode_stree(const std::vector<rtpg_sdtree_t::SNode
   return e;
 }
 
+// Node words after 3G levels of the descent for every cell of the 2^G cubed
+// grid (child index along an axis = bit G-1-level of the cell coordinate).
+static std::vector<uint32_t> build_jump(const std::vector<rtpg_sdtree_t::SNode>& sn, uint32_t G) {
+  const uint32_t n = 1u << G;
+  std::vector<uint32_t> jt((size_t)n * n * n);
+  for (uint32_t z = 0; z < n; ++z)
+    for (uint32_t y = 0; y < n; ++y)
+      for (uint32_t x = 0; x < n; ++x) {
+        uint32_t node = 0, c[3] = {x, y, z};
+        for (uint32_t k = 0; k < 3 * G && !sn[node].leaf; ++k) {
+          uint32_t bit = (c[k % 3] >> (G - 1 - k / 3)) & 1u;
+          node = sn[node].child + bit;
+        }
+        jt[((size_t)z * n + y) * n + x] = sn[node].leaf ? (0x80000000u | sn[node].dtree) : sn[node].child;
+      }
+  return jt;
+}
+
+void rtpg_sdtree_t::upload_stree() {
+  hipStream_t s = ctx->stream;
+  d_stree.upload(encode_stree(snodes), s);
+  d_jump.upload(build_jump(snodes, kJumpBits), s);
+}
+
 SdTreeView rtpg_sdtree_t::view() {
   SdTreeView v{};
   v.stree = d_stree.p;
+  v.jump = d_jump.p;
+  v.jump_bits = kJumpBits;
   for (int k = 0; k < 3; ++k) v.aabb_min[k] = aabb_min[k];
   v.aabb_size = size;
   v.info = dt.info.p;
@@ -123,7 +149,7 @@ void rtpg_sdtree_t::publish() { RTPG_HIP(launch_dtree_publish(dt.view(), ctx->st
 void rtpg_sdtree_t::clear() {
   hipStream_t s = ctx->stream;
   snodes.assign(1, SNode{true, 0u, 0u});
-  d_stree.upload(encode_stree(snodes), s);
+  upload_stree();
   alloc_per_dtree(dt, 1);
   std::vector<uint32_t> zero_u{0u}, one_u{1u};
   std::vector<float> zero_f{0.0f};
@@ -312,7 +338,7 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
   d_err.download(&err, 1, s);
   if (err) throw Error(RTPG_ERR_LIMIT, "DTree reset stack bound exceeded");
   snodes.swap(ns);
-  d_stree.upload(encode_stree(snodes), s);
+  upload_stree();
   dt = std::move(nd);
   publish();
   RTPG_HIP(hipStreamSynchronize(s));
@@ -551,7 +577,7 @@ int rtpg_sdtree_import(rtpg_sdtree tree, uint32_t n_stree, const uint32_t* stree
     o.s_child.upload(reinterpret_cast<const uint2*>(s_child), st, s);
     RTPG_HIP(hipStreamSynchronize(s));
     tree->snodes.swap(sn);
-    tree->d_stree.upload(encode_stree(tree->snodes), s);
+    tree->upload_stree();
     tree->dt = std::move(o);
     tree->publish();
     RTPG_HIP(hipStreamSynchronize(s));

@@ -55,6 +55,8 @@ struct Camera {
 // SD-tree, device side (flattened; see sdtree.hip).
 struct SdTreeView {
   const uint32_t* stree;   // bit 31 set: leaf, low bits DTree index; else first child
+  const uint32_t* jump;    // 2^(3 jump_bits) node words after 3 jump_bits levels (or null)
+  uint32_t jump_bits;
   float aabb_min[3];
   float aabb_size;         // cube edge
   const float4* info;      // per DTree: (samp_base bits, samp_sum, samp_sw, build_base bits)

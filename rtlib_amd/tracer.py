@@ -125,6 +125,12 @@ class Context:
         a, spp = self.accum()
         return a / max(spp, 1)
 
+    def frame(self):
+        """Tonemapped RGBA8 display frame (H x W x 4, row 0 at the bottom)."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        call("rtpg_frame_tonemap", self.h, ptr(out, C.c_uint8))
+        return out
+
     # -- render
     def render(self, spp=1, flags=FLAG_NEE | FLAG_MIS | FLAG_RR, max_depth=8, rr_depth=3, num_candidates=32,
                seed=0, sample_offset=0, bg=(0.0, 0.0, 0.0), fraction=0.3, sdtree=None, hashgrid=None,

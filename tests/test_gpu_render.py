@@ -117,3 +117,15 @@ def test_guided_controller_is_unbiased(ctx):
     assert n == 64
     assert tree.Sizes()["dtrees"] > 1
     assert abs(img.mean() - plain.mean()) / plain.mean() < 0.02
+
+
+def test_frame_tonemap_matches_reference_formula(ctx):
+    sc = scenes.cornell(32, 32)
+    img = _gpu(ctx, sc, 8, FLAG_NEE | FLAG_MIS | FLAG_RR, seed=1).astype(np.float32)
+    fr = ctx.frame()
+    x = img / (np.float32(1.0) + img)
+    g = np.where(x <= np.float32(0.00174), np.float32(32.0) * x, np.power(x, np.float32(1.0 / 2.2)))
+    want = np.minimum((g * np.float32(255.99)).astype(np.int32), 255)
+    # the device scales by 1/spp and uses its own powf: at most one step apart
+    assert np.abs(fr[..., :3].astype(np.int32) - want).max() <= 1
+    assert np.all(fr[..., 3] == 255) and fr[..., :3].max() > 100

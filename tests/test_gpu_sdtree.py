@@ -104,3 +104,41 @@ def test_build_of_empty_tree_and_uniform_reset(ctx, orc):
     st = tree.Download()
     # an empty sampling tree resets to the complete depth-4 tree (85 nodes)
     assert st["b_cnt"].tolist() == [85]
+
+
+def test_lookup_deep_tree_and_box_edges(ctx, orc):
+    # A spatial tree refined 27 levels deep towards one point (deeper than the
+    # device's table of the first levels), looked up at random points, at
+    # points on the power-of-two cell faces, near the refined point and
+    # outside the box: identical leaves to the oracle's plain descent.
+    depth = 27
+    target = np.array([0.3712, 0.8123, 0.0917])
+    words = np.zeros(1 + 2 * depth, np.uint32)
+    cur, leaf = 0, 0
+    for k in range(depth):
+        first = 1 + 2 * k
+        words[cur] = first
+        bit = int(np.floor(target[k % 3] * 2.0 ** (k // 3 + 1))) & 1
+        words[first + 1 - bit] = 0x80000000 | leaf
+        leaf += 1
+        cur = first + bit
+    words[cur] = 0x80000000 | leaf
+    leaf += 1
+    st = {"stree": words, "b_cnt": np.ones(leaf, np.uint32), "s_cnt": np.ones(leaf, np.uint32),
+          "scal": np.zeros((6, leaf), np.float32), "b_sums": np.zeros((leaf, 4), np.float32),
+          "b_child": np.zeros((leaf, 4), np.uint16), "s_sums": np.zeros((leaf, 4), np.float32),
+          "s_child": np.zeros((leaf, 4), np.uint16)}
+    tree = _tree(ctx)
+    tree.Upload(st)
+    rng = np.random.default_rng(5)
+    q = [rng.uniform(0, 1, (2000, 3)),
+         target + rng.normal(0, 1, (4000, 3)) * 10.0 ** rng.uniform(-7, -0.5, (4000, 1)),
+         rng.integers(0, 65, (2000, 3)) / 64.0,
+         rng.integers(0, 33, (500, 3)) / 32.0,
+         rng.uniform(-0.3, 1.3, (2000, 3)),
+         np.array([[0, 0, 0], [1, 1, 1], [0.5, 0.5, 0.5], [1, 0, 1]], np.float64)]
+    pos = (np.concatenate(q) * SIZE + LO).astype(np.float32)
+    got = tree.Lookup(pos)
+    want = orc.stree_lookup(words, LO, SIZE, pos)
+    assert np.array_equal(got, want)
+    assert len(np.unique(want)) == depth + 1  # every leaf is reached

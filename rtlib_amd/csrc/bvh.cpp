@@ -180,9 +180,9 @@ struct Collapser {
       kids[best] = bin[k].left;
       kids[nk++] = bin[k].right;
     }
-    uint32_t n_inner = 0;
-    for (int i = 0; i < nk; ++i) n_inner += bin[kids[i]].leaf() ? 0u : 1u;
-    uint32_t child_stack = stack_here + (n_inner > 0 ? n_inner - 1 : 0);
+    // The traversal keeps every hit child but the nearest on its stack, leaves
+    // included, so below this node at most nk - 1 more entries are stacked.
+    uint32_t child_stack = stack_here + (uint32_t)(nk - 1);
     res.max_stack = std::max(res.max_stack, child_stack);
     int32_t child_ref[4];
     uint32_t child_cnt[4];

@@ -34,7 +34,7 @@ struct PathSoA {
   uint32_t* count;   // live paths per depth [max_depth + 2]
   uint32_t* scount;  // shadow rays per depth [max_depth + 2]
   float4* shadow;    // 3 x float4 per shadow ray
-  uint32_t* deep;    // traversal spill stack, (cap - 16) x n
+  uint32_t* deep;    // traversal spill stack: (entry, distance) pairs, 2 x (cap - 16) x n words
   uint32_t* err;     // sticky error bits
   // guide vertices, [depth][pid]
   float4* vA;        // throughput after the bounce, woPdf

@@ -72,14 +72,19 @@ __device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tm
   return true;
 }
 
+// Traversal stack entries: (node or leaf, entry distance of its box).  An
+// inner node is its index; a leaf is kLeafEntry | (count - 1) << 27 | first.
+constexpr uint32_t kLeafEntry = 0x80000000u;
+
 struct StackRef {
-  uint32_t* lds;     // this lane's column of the LDS short stack
-  uint32_t* deep;    // this ray's column of the spill stack (stride = deep_stride)
+  uint2* lds;        // this lane's column of the LDS short stack
+  uint2* deep;       // this ray's column of the spill stack (stride = deep_stride)
   uint32_t deep_stride;
   uint32_t cap;      // total capacity
 };
 
-__device__ __forceinline__ void stack_push(const StackRef& s, uint32_t& sp, uint32_t v) {
+__device__ __forceinline__ void stack_push(const StackRef& s, uint32_t& sp, uint32_t e, float t) {
+  uint2 v = make_uint2(e, fbits(t));
   if (sp < kShortStack) {
     s.lds[sp * kBlock] = v;
   } else {
@@ -87,7 +92,7 @@ __device__ __forceinline__ void stack_push(const StackRef& s, uint32_t& sp, uint
   }
   ++sp;
 }
-__device__ __forceinline__ uint32_t stack_pop(const StackRef& s, uint32_t& sp) {
+__device__ __forceinline__ uint2 stack_pop(const StackRef& s, uint32_t& sp) {
   --sp;
   return sp < kShortStack ? s.lds[sp * kBlock] : s.deep[(sp - kShortStack) * s.deep_stride];
 }
@@ -95,13 +100,17 @@ __device__ __forceinline__ uint32_t stack_pop(const StackRef& s, uint32_t& sp) {
 #define RTPG_CS(a, b)                         \
   if (tk[b] < tk[a]) {                        \
     float tt = tk[a]; tk[a] = tk[b]; tk[b] = tt; \
-    int32_t nn = nk[a]; nk[a] = nk[b]; nk[b] = nn; \
+    uint32_t nn = nk[a]; nk[a] = nk[b]; nk[b] = nn; \
   }
 
 // BVH4 traversal: closest hit (kAny = false) or any hit (kAny = true).
-// Leaf children are intersected when their parent is visited; inner children
-// are visited nearest-first, the others pushed far-to-near.
-// kCount adds the nodes fetched and triangles tested to cnt[0], cnt[1]
+// Two phases per round, so that a wave tests triangles together: the inner
+// phase visits nodes nearest-first until the lane's next entry is a leaf (or
+// nothing is left), pushing the other hit children, inner nodes and leaves
+// alike, far-to-near with their entry distance; the leaf phase tests that
+// leaf's triangles.  A popped entry whose box is entered beyond the best hit
+// (with the same 2-ulp slack as the box test) is dropped without a fetch.
+// kCount adds the nodes fetched and triangles tested to wcnt[0], wcnt[1]
 // (the algorithmic-byte accounting of the roofline).
 template <bool kAny, bool kCount = false>
 __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax, const StackRef& st,
@@ -116,65 +125,89 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
   int32_t best = -1;
   float bu = 0.0f, bv = 0.0f;
   uint32_t sp = 0;
-  int32_t node = 0;
-  for (;;) {
-    if (kCount) ++wcnt[0];
-    const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
-    float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    int4 ch = reinterpret_cast<const int4*>(np)[6];
-    uint4 cn = reinterpret_cast<const uint4*>(np)[7];
-    float tk[4];
-    int32_t nk[4];
-    int n_inner = 0;
+  uint32_t cur = 0;  // the root
+  bool live = true;
+  while (live) {
+    // ---- inner phase
+    while (!(cur & kLeafEntry)) {
+      if (kCount) ++wcnt[0];
+      const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
+      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      int4 ch = reinterpret_cast<const int4*>(np)[6];
+      uint4 cn = reinterpret_cast<const uint4*>(np)[7];
+      float tk[4];
+      uint32_t nk[4];
+      int n_hit = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float lox = f4c(lx, i), hix = f4c(hx, i), loy = f4c(ly, i), hiy = f4c(hy, i);
-      float loz = f4c(lz, i), hiz = f4c(hz, i);
-      float t0x = lox * inv.x - oi.x, t1x = hix * inv.x - oi.x;
-      float t0y = loy * inv.y - oi.y, t1y = hiy * inv.y - oi.y;
-      float t0z = loz * inv.z - oi.z, t1z = hiz * inv.z - oi.z;
-      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
-      bool hit = tn <= tf * 1.00000024f;
-      int32_t c = i == 0 ? ch.x : (i == 1 ? ch.y : (i == 2 ? ch.z : ch.w));
-      uint32_t cnt = i == 0 ? cn.x : (i == 1 ? cn.y : (i == 2 ? cn.z : cn.w));
-      tk[i] = 3.0e38f;
-      nk[i] = 0;
-      if (hit) {
-        if (c < 0) {
-          uint32_t first = (uint32_t)(~c);
-          for (uint32_t k = 0; k < cnt; ++k) {
-            if (kCount) ++wcnt[1];
-            if (isect_tri(sc.tris + first + k, o, d, tmin, tbest, bu, bv)) {
-              best = (int32_t)(first + k);
-              if (kAny) {
-                t_out = tbest;
-                u_out = bu;
-                v_out = bv;
-                tri_out = best;
-                return true;
-              }
-            }
+      for (int i = 0; i < 4; ++i) {
+        float lox = f4c(lx, i), hix = f4c(hx, i), loy = f4c(ly, i), hiy = f4c(hy, i);
+        float loz = f4c(lz, i), hiz = f4c(hz, i);
+        float t0x = lox * inv.x - oi.x, t1x = hix * inv.x - oi.x;
+        float t0y = loy * inv.y - oi.y, t1y = hiy * inv.y - oi.y;
+        float t0z = loz * inv.z - oi.z, t1z = hiz * inv.z - oi.z;
+        float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
+        bool hit = tn <= tf * 1.00000024f;
+        int32_t c = i == 0 ? ch.x : (i == 1 ? ch.y : (i == 2 ? ch.z : ch.w));
+        uint32_t cnt = i == 0 ? cn.x : (i == 1 ? cn.y : (i == 2 ? cn.z : cn.w));
+        tk[i] = hit ? tn : 3.0e38f;
+        nk[i] = c >= 0 ? (uint32_t)c : (kLeafEntry | ((cnt - 1u) << 27) | (uint32_t)(~c));
+        n_hit += hit ? 1 : 0;
+      }
+      if (n_hit == 0) {
+        // pop the next entry that can still hold a closer hit
+        bool found = false;
+        while (sp > 0) {
+          uint2 e = stack_pop(st, sp);
+          if (bitsf(e.y) <= tbest * 1.00000024f) {
+            cur = e.x;
+            found = true;
+            break;
           }
-        } else {
-          tk[i] = tn;
-          nk[i] = c;
-          ++n_inner;
+        }
+        if (!found) {
+          live = false;
+          break;
+        }
+        continue;
+      }
+      RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
+      cur = nk[0];
+      if (sp + (uint32_t)(n_hit - 1) > st.cap) {
+        atomicOr(err, 1u);  // stack bound of the build violated: stop this ray
+        live = false;
+        break;
+      }
+      for (int k = n_hit - 1; k >= 1; --k) stack_push(st, sp, nk[k], tk[k]);
+    }
+    if (!live) break;
+    // ---- leaf phase
+    {
+      uint32_t first = cur & 0x07FFFFFFu, cnt = ((cur >> 27) & 0xFu) + 1u;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        if (kCount) ++wcnt[1];
+        if (isect_tri(sc.tris + first + k, o, d, tmin, tbest, bu, bv)) {
+          best = (int32_t)(first + k);
+          if (kAny) {
+            t_out = tbest;
+            u_out = bu;
+            v_out = bv;
+            tri_out = best;
+            return true;
+          }
         }
       }
     }
-    if (n_inner == 0) {
-      if (sp == 0) break;
-      node = (int32_t)stack_pop(st, sp);
-      continue;
+    bool found = false;
+    while (sp > 0) {
+      uint2 e = stack_pop(st, sp);
+      if (bitsf(e.y) <= tbest * 1.00000024f) {
+        cur = e.x;
+        found = true;
+        break;
+      }
     }
-    RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
-    node = nk[0];
-    if (sp + (uint32_t)(n_inner - 1) > st.cap) {
-      atomicOr(err, 1u);  // stack bound of the build violated: stop this ray
-      break;
-    }
-    for (int k = n_inner - 1; k >= 1; --k) stack_push(st, sp, (uint32_t)nk[k]);
+    live = found;
   }
   t_out = tbest;
   u_out = bu;
@@ -184,11 +217,11 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
 }
 #undef RTPG_CS
 
-__device__ __forceinline__ StackRef make_stack(uint32_t (*lds)[kBlock], uint32_t* deep, uint32_t deep_stride,
+__device__ __forceinline__ StackRef make_stack(uint2 (*lds)[kBlock], uint32_t* deep, uint32_t deep_stride,
                                                uint32_t slot, uint32_t cap) {
   StackRef s;
   s.lds = &lds[0][threadIdx.x];
-  s.deep = deep ? deep + slot : nullptr;
+  s.deep = deep ? reinterpret_cast<uint2*>(deep) + slot : nullptr;
   s.deep_stride = deep_stride;
   s.cap = cap;
   return s;
@@ -198,7 +231,7 @@ __device__ __forceinline__ StackRef make_stack(uint32_t (*lds)[kBlock], uint32_t
 __global__ void __launch_bounds__(kBlock) k_trace_rays(SceneView sc, const float4* rays, uint32_t n,
                                                        float4* hits, uint32_t* occl, uint32_t any,
                                                        uint32_t* deep, uint32_t* err) {
-  __shared__ uint32_t lds[kShortStack][kBlock];
+  __shared__ uint2 lds[kShortStack][kBlock];
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   float4 a = rays[2 * i], b = rays[2 * i + 1];
@@ -252,7 +285,7 @@ __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t a, 
 
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock) k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
-  __shared__ uint32_t lds[kShortStack][kBlock];
+  __shared__ uint2 lds[kShortStack][kBlock];
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t n_act = ps.count[depth];
   uint32_t cnt[2] = {0u, 0u};
@@ -690,7 +723,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
 
 template <bool kCount, bool kAdj = false>
 __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
-  __shared__ uint32_t lds[kShortStack][kBlock];
+  __shared__ uint2 lds[kShortStack][kBlock];
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t n_sh = ps.scount[depth];
   uint32_t cnt[2] = {0u, 0u};

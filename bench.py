@@ -56,7 +56,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--budget", type=int, default=256,
                     help="samples per pixel of one guided render: SD-tree passes 1, 2, 4, .., 64 and the final pass")
-    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--ref-spp", type=int, default=16384)
     ap.add_argument("--mlp-passes", type=int, default=8)
     ap.add_argument("--mlp-steps", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

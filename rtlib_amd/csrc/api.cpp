@@ -125,7 +125,7 @@ int rtpg_scene_upload(rtpg_context ctx, const float* verts, uint32_t nv, const u
     int threads = (int)std::max(1u, std::thread::hardware_concurrency());
     require(nt < (1u << 27), "at most 2^27 - 1 triangles (leaf entries hold a 27-bit first index)");
     BvhBuildResult b = build_bvh4(verts, nv, idx, nt, std::min(threads, 16));
-    if (b.max_stack > 16u + 112u) throw Error(RTPG_ERR_LIMIT, "BVH too deep for the traversal stack");
+    if (b.max_stack > 128u) throw Error(RTPG_ERR_LIMIT, "BVH too deep for the traversal stack");
     std::vector<float4> shade(nt), specv(nt);
     std::vector<uint32_t> info(nt);
     std::vector<uint32_t> slot_of(nt);
@@ -325,7 +325,7 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
   c->p_shadow.reserve(3 * (size_t)n);
   c->p_count.reserve(2 * (size_t)(max_depth + 2));
   // spill stack: (entry, distance) pairs beyond the 16 LDS entries
-  if (c->max_stack > 16u) c->p_deep.reserve(2 * (size_t)(c->max_stack - 16u) * n);
+  if (c->max_stack > kShortStack) c->p_deep.reserve(2 * (size_t)(c->max_stack - kShortStack) * n);
   if (guide) {
     c->p_vA.reserve((size_t)max_depth * n);
     c->p_vB.reserve((size_t)max_depth * n);
@@ -358,7 +358,7 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
   ps.count = ctx->p_count.p;
   ps.scount = ctx->p_count.p + (max_depth + 2u);
   ps.shadow = ctx->p_shadow.p;
-  ps.deep = ctx->max_stack > 16u ? ctx->p_deep.p : nullptr;
+  ps.deep = ctx->max_stack > kShortStack ? ctx->p_deep.p : nullptr;
   ps.err = ctx->p_err.p;
   ps.vA = guide ? ctx->p_vA.p : nullptr;
   ps.vB = guide ? ctx->p_vB.p : nullptr;
@@ -610,9 +610,9 @@ static int trace_common(rtpg_context ctx, const float* rays, uint32_t n, float* 
     DBuf<uint32_t> d_occ, d_deep;
     d_rays.upload(reinterpret_cast<const float4*>(rays), 2 * (size_t)n, s);
     if (any) d_occ.alloc(n); else d_hits.alloc(n);
-    if (ctx->max_stack > 16u) d_deep.alloc(2 * (size_t)(ctx->max_stack - 16u) * n);
+    if (ctx->max_stack > kShortStack) d_deep.alloc(2 * (size_t)(ctx->max_stack - kShortStack) * n);
     RTPG_HIP(launch_trace_rays(ctx->scene_view(), d_rays.p, n, d_hits.p, d_occ.p, any,
-                               ctx->max_stack > 16u ? d_deep.p : nullptr, ctx->p_err.p, s));
+                               ctx->max_stack > kShortStack ? d_deep.p : nullptr, ctx->p_err.p, s));
     if (any) d_occ.download(occl, n, s); else d_hits.download(reinterpret_cast<float4*>(hits), n, s);
     uint32_t err = 0;
     ctx->p_err.download(&err, 1, s);

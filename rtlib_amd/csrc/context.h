@@ -87,6 +87,7 @@ struct rtpg_sdtree_t {
     rtpg::DBuf<float> b_sw, b_sum, b_area, s_sw, s_sum, s_area;
     rtpg::DBuf<float4> info, b_sums, s_sums;
     rtpg::DBuf<uint2> b_child, s_child;
+    rtpg::DBuf<uint4> s_node;  // interleaved sampling nodes (written by publish)
     rtpg::DTreeArrays view();
   } dt;
   rtpg::DBuf<uint32_t> d_err;

@@ -98,7 +98,8 @@ __device__ __forceinline__ float dtree_pdf(const SdTreeView& g, uint32_t cell, f
   uint32_t node = 0;
   int idx = dtree_child_idx(px, py);
   for (;;) {
-    float4 s = g.samp_sums[base + node];
+    uint4 r0 = g.samp_node[2u * (base + node)], r1 = g.samp_node[2u * (base + node) + 1u];
+    float4 s = make_float4(bitsf(r0.x), bitsf(r0.y), bitsf(r0.z), bitsf(r0.w));
     float total = s.x + s.y + s.z + s.w;
     float si = f4c(s, idx);
     if (si <= 0.0f) {
@@ -106,7 +107,7 @@ __device__ __forceinline__ float dtree_pdf(const SdTreeView& g, uint32_t cell, f
       break;
     }
     result *= 4.0f * si / total;
-    uint32_t ch = dnode_child(g.samp_child[base + node], idx);
+    uint32_t ch = dnode_child(make_uint2(r1.x, r1.y), idx);
     if (ch == 0) break;
     node = ch;
     idx = dtree_child_idx(px, py);
@@ -133,7 +134,8 @@ __device__ __forceinline__ void dtree_sample(const SdTreeView& g, uint32_t cell,
   uint32_t node = 0;
   float rx = 0.0f, ry = 0.0f, size = 1.0f;
   for (;;) {
-    float4 s = g.samp_sums[base + node];
+    uint4 r0 = g.samp_node[2u * (base + node)], r1 = g.samp_node[2u * (base + node) + 1u];
+    float4 s = make_float4(bitsf(r0.x), bitsf(r0.y), bitsf(r0.z), bitsf(r0.w));
     int idx = 0;
     float partial = s.x + s.z;
     float total = s.x + s.y + s.z + s.w;
@@ -161,7 +163,7 @@ __device__ __forceinline__ void dtree_sample(const SdTreeView& g, uint32_t cell,
     size *= 0.5f;
     float si = f4c(s, idx);
     result = si > 0.0f ? result * (4.0f * si / total) : 0.0f;
-    uint32_t ch = dnode_child(g.samp_child[base + node], idx);
+    uint32_t ch = dnode_child(make_uint2(r1.x, r1.y), idx);
     if (ch == 0 || si <= 0.0f) {
       float a = rnd(rng);
       float b = rnd(rng);

@@ -9,6 +9,8 @@
 namespace rtpg {
 
 constexpr float kRayTmin = 1e-4f;   // SimpleTrace.cu:76
+// Traversal stack entries kept in LDS per lane; deeper entries spill to HBM.
+constexpr uint32_t kShortStack = 8;
 constexpr float kRayTmax = 1e20f;
 
 // Path meta word (ray_d.w): depth in bits 0..7, flags above.

@@ -20,7 +20,6 @@
 namespace rtpg {
 
 constexpr uint32_t kBlock = 128;
-constexpr uint32_t kShortStack = 16;
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -387,7 +386,9 @@ __device__ __forceinline__ void block_append2(uint32_t* ca, bool pa, uint32_t* c
   sb = s_b[kShadeWaves] + s_b[wave] + bb;
 }
 
-template <int kGuide, bool kAdj = false>
+// kRis instantiates the RIS light selection; without it the kernel needs
+// fewer registers and runs at a higher occupancy.
+template <int kGuide, bool kAdj = false, bool kRis = false>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd,
                                                        HashGridView hg, MortonView mq, uint32_t depth) {
   uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
@@ -416,7 +417,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
     float cos_out = 0.0f;
     int32_t tri = (int32_t)fbits(hit.w);
     const bool nee = (rp.flags & kFlagNee) && sc.n_lights > 0;
-    const bool ris = nee && (rp.flags & kFlagRis);
+    const bool ris = kRis && nee && (rp.flags & kFlagRis);
     const bool mis = nee && !ris && (rp.flags & kFlagMis);
     uint32_t vcell = kInvalidCell;
     float vcu = 0.0f, vcv = 0.0f;
@@ -850,7 +851,7 @@ __device__ __forceinline__ bool commit_value(const PathSoA& ps, uint32_t pid, ui
 // leaves near the quadtree roots do not serialise on HBM atomics.
 constexpr int kMaxCommitVtx = 16;
 constexpr uint32_t kApplyBlock = 256;
-constexpr uint32_t kApplyPerThread = 16;
+constexpr uint32_t kApplyPerThread = 128;
 constexpr uint32_t kLeafTab = 4096;
 constexpr uint32_t kCellTab = 1024;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
@@ -1167,25 +1168,33 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     mark(0, d, 1);
     mark(1, d, 0);
     const bool adjoint = ps.grad_alb != nullptr;
+    const bool ris = (rp.flags & kFlagRis) != 0;
     const uint32_t nbs = blocks_for(ps.n, kShadeBlock);
+#define RTPG_SHADE(G, A)                                                                                      \
+  if (ris) {                                                                                                  \
+    hipLaunchKernelGGL((k_shade<G, A, true>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);  \
+  } else {                                                                                                    \
+    hipLaunchKernelGGL((k_shade<G, A, false>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d); \
+  }
     if (adjoint) {
       // both passes of the adjoint run this one instantiation, so the replay
-      // reproduces the primal paths operation for operation
-      hipLaunchKernelGGL((k_shade<0, true>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      // reproduces the primal paths operation for operation (RIS is refused)
+      hipLaunchKernelGGL((k_shade<0, true, false>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
     } else if (guide == 1) {
-      hipLaunchKernelGGL(k_shade<1>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      RTPG_SHADE(1, false)
     } else if (guide == 2) {
-      hipLaunchKernelGGL(k_shade<2>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      RTPG_SHADE(2, false)
     } else if (guide == 3) {
       // the learned guide's inference pass is timed with the shade class
       if (rp.flags & kFlagGuideSample) {
         uint32_t ni = std::min(blocks_for(ps.n, 64u), kInferGrid);
         hipLaunchKernelGGL(k_mlp_infer, dim3(ni), dim3(kInferBlock), 0, s, sc, ps, ml, d);
       }
-      hipLaunchKernelGGL(k_shade<3>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      RTPG_SHADE(3, false)
     } else {
-      hipLaunchKernelGGL(k_shade<0>, dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      RTPG_SHADE(0, false)
     }
+#undef RTPG_SHADE
     mark(1, d, 1);
     mark(2, d, 0);
     if (adjoint) {

@@ -41,10 +41,12 @@ struct DTreeArrays {
   uint2* b_child = nullptr;
   float4* s_sums = nullptr;
   uint2* s_child = nullptr;
+  uint4* s_node = nullptr;      // 2 per sampling node: sums bits, (children, 0, 0)
 };
 
 // Kernel launchers (sdtree.hip).
 hipError_t launch_dtree_build(DTreeArrays a, float* area_tmp, hipStream_t s);
+// Per-DTree tracer view (info) and the interleaved copy of the sampling nodes.
 hipError_t launch_dtree_publish(DTreeArrays a, hipStream_t s);
 // Reset of the building trees of `dst` from the sampling trees of `src`
 // (per new DTree j: source map[j]); pass 0 counts nodes into dst.b_cnt,

@@ -67,6 +67,16 @@ __global__ void k_dtree_publish(DTreeArrays a) {
   a.info[j] = make_float4(bitsf(a.s_base[j]), a.s_sum[j], a.s_sw[j], bitsf(a.b_base[j]));
 }
 
+// One 32-byte record per sampling node, so a descent step touches one line.
+__global__ void k_dtree_interleave(DTreeArrays a) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.s_nodes) return;
+  float4 s = a.s_sums[i];
+  uint2 c = a.s_child[i];
+  a.s_node[2 * i] = make_uint4(fbits(s.x), fbits(s.y), fbits(s.z), fbits(s.w));
+  a.s_node[2 * i + 1] = make_uint4(c.x, c.y, 0u, 0u);
+}
+
 // RTDTreeT::Reset (OPX7UtilsPathGuiding.h:888-947), one work-item per DTree.
 // The DFS keeps the reference's LIFO order, so nodes get the reference's
 // numbers.  A stack entry either copies a node of the source sampling tree
@@ -276,6 +286,8 @@ hipError_t launch_dtree_build(DTreeArrays a, float* area_tmp, hipStream_t s) {
 hipError_t launch_dtree_publish(DTreeArrays a, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_dtree_publish, dim3(nblk(a.n, 256)), dim3(256), 0, s, a);
+  if (a.s_nodes && a.s_node)
+    hipLaunchKernelGGL(k_dtree_interleave, dim3(nblk(a.s_nodes, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_dtree_reset(DTreeArrays src, DTreeArrays dst, const uint32_t* map, uint32_t max_depth,

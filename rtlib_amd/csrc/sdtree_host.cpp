@@ -70,6 +70,7 @@ DTreeArrays rtpg_sdtree_t::Owned::view() {
   a.b_child = b_child.p;
   a.s_sums = s_sums.p;
   a.s_child = s_child.p;
+  a.s_node = s_node.p;
   return a;
 }
 
@@ -133,6 +134,7 @@ SdTreeView rtpg_sdtree_t::view() {
   v.build_child = dt.b_child.p;
   v.samp_sums = dt.s_sums.p;
   v.samp_child = dt.s_child.p;
+  v.samp_node = dt.s_node.p;
   v.fraction = fraction;
   v.n_dtrees = dt.n;
   return v;
@@ -142,7 +144,10 @@ namespace rtpg {
 SdTreeView sdtree_view_of(rtpg_sdtree_t* t) { return t->view(); }
 }  // namespace rtpg
 
-void rtpg_sdtree_t::publish() { RTPG_HIP(launch_dtree_publish(dt.view(), ctx->stream)); }
+void rtpg_sdtree_t::publish() {
+  dt.s_node.reserve(2 * (size_t)std::max(dt.s_nodes, 1u));
+  RTPG_HIP(launch_dtree_publish(dt.view(), ctx->stream));
+}
 
 // RTSTreeWrapperT::Clear (OPX7UtilsPathGuiding.h:1634-1636): a single leaf
 // whose DTrees (building, sampling) are one empty node each (RTDTreeT ctor :879-886).

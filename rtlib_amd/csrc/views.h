@@ -65,6 +65,8 @@ struct SdTreeView {
   const uint2* build_child;
   const float4* samp_sums;
   const uint2* samp_child;
+  // sampling nodes interleaved for the tracer: per node (sums), (children, 0, 0)
+  const uint4* samp_node;
   float fraction;
   uint32_t n_dtrees;
 };

@@ -73,8 +73,9 @@ def main(argv=None):
         return 2
 
     from rtlib_amd import scenes
-    from rtlib_amd.tracer import (FLAG_MIS, FLAG_NEE, FLAG_RR, Context, MLPGuide, RTSTreeController, RTSTreeWrapper,
-                                  render_guided, render_mlp_guided)
+    from rtlib_amd.tracer import (FLAG_MIS, FLAG_NEE, FLAG_RR, Context, DoubleBufferedHashGrid3Buffer, MLPGuide,
+                                  RTMortonQuadTreeController, RTMortonQuadTreeWrapper, RTSTreeController,
+                                  RTSTreeWrapper, render_guided, render_mlp_guided)
 
     W, H = (320, 180) if a.small else (1280, 720)
     grid = (256, 128) if a.small else (1024, 512)
@@ -189,8 +190,27 @@ def main(argv=None):
             ctx.render(spp=spp, flags=base, max_depth=depth, seed=seed)
             return spp, ctx.image()
 
+        # hash-grid variant: 30-bit Morton cell keys, 2^20 slots, 8 x 8 direction
+        # bins per cell, the reference's Morton controller (Locate first, trees
+        # built from iteration 3)
+        grid = DoubleBufferedHashGrid3Buffer(ctx, lo, hi, bounds=(256, 64, 256), slots=1 << 20, key_mode=1)
+        mq = RTMortonQuadTreeWrapper(ctx, 1 << 20, 3, 0.3)
+
+        def grid_step():
+            grid.Clear()
+            mq.Allocate()
+            ctrl = RTMortonQuadTreeController(mq, grid, a.budget, 3, 0, 0.5, 1)
+            ctx.clear()
+            return render_guided(ctx, ctrl, mq, "grid", base, max_depth=depth, seed=13, grid=grid)
+
+        timed(grid_step)
+        t_grid, _ = timed(grid_step)
+        grid_img = ctx.image()
+        log(f"hash-grid guided step: {t_grid * 1e3:.1f} ms ({a.budget} spp)")
+
         spp_tree, plain_tree = plain_equal_time(t_step, 78)
         spp_mlp, plain_mlp = plain_equal_time(t_mlp, 79)
+        spp_grid, plain_grid = plain_equal_time(t_grid, 80)
         # reference image (plain, many samples, independent seed)
         t0 = time.time()
         ctx.clear()
@@ -206,6 +226,8 @@ def main(argv=None):
             "plain_at_budget": {"spp": a.budget, "relmse": round(rel_mse(plain_budget_img, ref), 5)},
             "sdtree": {"ms": round(t_step * 1e3, 1), "relmse": round(rel_mse(guided_img, ref), 5),
                        "plain_spp": spp_tree, "plain_relmse": round(rel_mse(plain_tree, ref), 5)},
+            "hashgrid": {"ms": round(t_grid * 1e3, 1), "relmse": round(rel_mse(grid_img, ref), 5),
+                         "plain_spp": spp_grid, "plain_relmse": round(rel_mse(plain_grid, ref), 5)},
             "mlp": {"ms": round(t_mlp * 1e3, 1), "relmse": round(rel_mse(mlp_img, ref), 5),
                     "plain_spp": spp_mlp, "plain_relmse": round(rel_mse(plain_mlp, ref), 5),
                     "train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps},

@@ -129,3 +129,26 @@ def test_frame_tonemap_matches_reference_formula(ctx):
     # the device scales by 1/spp and uses its own powf: at most one step apart
     assert np.abs(fr[..., :3].astype(np.int32) - want).max() <= 1
     assert np.all(fr[..., 3] == 255) and fr[..., :3].max() > 100
+
+
+def test_hash_grid_controller_is_unbiased(ctx):
+    # Morton-coded hash grid (30-bit keys) + direction quadtrees under the
+    # reference controller schedule (Locate, Record, RecordAndSample, Sample).
+    from rtlib_amd.tracer import DoubleBufferedHashGrid3Buffer, RTMortonQuadTreeController, RTMortonQuadTreeWrapper
+
+    sc = scenes.cornell(32, 32)
+    base = FLAG_NEE | FLAG_MIS | FLAG_RR
+    plain = _gpu(ctx, sc, 256, base, seed=9)
+    info = ctx.scene_info()
+    grid = DoubleBufferedHashGrid3Buffer(ctx, info["aabb_min"] - 0.005, info["aabb_max"] + 0.005,
+                                         bounds=(64, 64, 64), slots=1 << 14, key_mode=1)
+    mq = RTMortonQuadTreeWrapper(ctx, 1 << 14, 3, 0.3)
+    ctrl = RTMortonQuadTreeController(mq, grid, 64, 3, 0, 0.5, 1)
+    ctx.clear()
+    states = []
+    n = render_guided(ctx, ctrl, mq, "grid", base, max_depth=8, seed=10, grid=grid,
+                      on_pass=lambda st, spp, traced: states.append(st))
+    img = ctx.image()
+    assert n == 64
+    assert 3 in states and 2 in states  # located cells, then sampled from the trees
+    assert abs(img.mean() - plain.mean()) / plain.mean() < 0.02

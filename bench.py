@@ -57,6 +57,8 @@ def main(argv=None):
     ap.add_argument("--budget", type=int, default=256,
                     help="samples per pixel of one guided render: SD-tree passes 1, 2, 4, .., 64 and the final pass")
     ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--no-configs", action="store_true", help="skip the Cornell / many-light / adjoint / "
+                    "learned-guide configurations")
     ap.add_argument("--mlp-passes", type=int, default=8)
     ap.add_argument("--mlp-steps", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -252,6 +254,9 @@ def main(argv=None):
                 "kernel_ms_per_step": {n: round(float(x), 2) for n, x in zip(names, per_kernel)},
                 "kernel_launches_per_step": {n: int(x) for n, x in zip(names, launches)}}
 
+    # ---- the other configurations, measured on the same context
+    configs = None if (a.small or a.no_configs) else other_configs(ctx, timed)
+
     # ---- CPU baseline: the oracle integrator on the host cores, guided final pass
     cpu = cpu_baseline(sc, tree, lo, hi, W, H, depth, a.cpu_seconds) if a.cpu_seconds > 0 else None
 
@@ -280,12 +285,121 @@ def main(argv=None):
         "sdtree": tree.Sizes(),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
     return 0
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "k_extend_guided_room_traffic.json"
+
+
+def other_configs(ctx, timed):
+    """Throughput (and, for the learned guide, equal-time relMSE) of the other
+    benchmark configurations: Cornell box plain path tracing (256^2, 64 spp,
+    depth 8, RR after 3); many-light RIS (2000 icospheres, 2.56M tris, 4096
+    emissive triangles, 1920x1080, 16 spp, depth 6, M = 32); path-replay
+    adjoint (Cornell 512^2, 8 orbit views 45 degrees apart, 16 spp, depth 6);
+    learned guide on the Cornell box against SD-tree guiding and plain path
+    tracing at equal time (64-spp budgets, 16k-spp reference)."""
+    from rtlib_amd import scenes
+    from rtlib_amd.tracer import (FLAG_MIS, FLAG_NEE, FLAG_RIS, FLAG_RR, MLPGuide, RTSTreeController,
+                                  RTSTreeWrapper, render_guided, render_mlp_guided)
+    out = {}
+    nee_mis_rr = FLAG_NEE | FLAG_MIS | FLAG_RR
+    # Cornell box, plain
+    sc = scenes.cornell(256, 256)
+    ctx.upload_scene(sc)
+    ctx.clear()
+    ctx.render(spp=64, flags=nee_mis_rr, max_depth=8, rr_depth=3, seed=1)
+    ts = []
+    for k in range(3):
+        ctx.clear()
+        dt, _ = timed(lambda: ctx.render(spp=64, flags=nee_mis_rr, max_depth=8, rr_depth=3, seed=2 + k))
+        ts.append(dt)
+    st = ctx.render_stats()
+    t = float(np.median(ts))
+    out["cornell_plain"] = {"msamples_per_s": round(256 * 256 * 64 / t / 1e6, 1), "ms": round(t * 1e3, 2),
+                            "mrays_per_s": round((st["rays"] + st["shadow_rays"]) / t / 1e6, 1)}
+    log(f"cornell: {out['cornell_plain']}")
+    # learned guide vs SD-tree vs plain on the Cornell box at equal time
+    ctx.clear()
+    for k in range(64):
+        ctx.render(spp=256, flags=nee_mis_rr, max_depth=8, seed=900, sample_offset=256 * k)
+    ref = ctx.image()
+    info = ctx.scene_info()
+    lo, hi = info["aabb_min"] - 0.005, info["aabb_max"] + 0.005
+    plain_rate = 256 * 256 * 64 / t
+    mlp = MLPGuide(ctx, lo, hi, seed=1)
+
+    def mlp_step():
+        mlp.reset(1)
+        ctx.clear()
+        return render_mlp_guided(ctx, mlp, 64, nee_mis_rr, max_depth=8, seed=21, train_passes=8, steps_per_pass=4)
+
+    timed(mlp_step)
+    t_mlp, _ = timed(mlp_step)
+    img_mlp = ctx.image()
+    mlp.close()
+    tree = RTSTreeWrapper(ctx, lo, hi, 20, 0.3)
+    tree.SetSplit(mode=1, c=12000.0)
+
+    def tree_step():
+        tree.Clear()
+        ctrl = RTSTreeController(tree, 64, 0, 0.5, 1)
+        ctx.clear()
+        return render_guided(ctx, ctrl, tree, "tree", nee_mis_rr, max_depth=8, seed=22)
+
+    timed(tree_step)
+    t_tree, _ = timed(tree_step)
+    img_tree = ctx.image()
+    res = {}
+    for name, tt, img in (("mlp", t_mlp, img_mlp), ("sdtree", t_tree, img_tree)):
+        spp = max(1, int(round(tt * plain_rate / (256 * 256))))
+        ctx.clear()
+        ctx.render(spp=spp, flags=nee_mis_rr, max_depth=8, seed=23)
+        res[name] = {"ms": round(tt * 1e3, 2), "relmse": round(rel_mse(img, ref), 6), "plain_spp": spp,
+                     "plain_relmse": round(rel_mse(ctx.image(), ref), 6)}
+    ctx.clear()
+    ctx.render(spp=64, flags=nee_mis_rr, max_depth=8, seed=24)
+    res["plain_at_budget"] = {"spp": 64, "relmse": round(rel_mse(ctx.image(), ref), 6)}
+    out["cornell_guided_equal_time"] = res
+    log(f"cornell guided: {res}")
+    # many-light RIS
+    sc = scenes.many_lights()
+    ctx.upload_scene(sc)
+    ris = FLAG_NEE | FLAG_RIS
+    ctx.clear()
+    ctx.render(spp=4, flags=ris, max_depth=6, num_candidates=32, seed=1)
+    ctx.clear()
+    dt, _ = timed(lambda: ctx.render(spp=16, flags=ris, max_depth=6, num_candidates=32, seed=2))
+    st = ctx.render_stats()
+    out["many_light_ris"] = {"msamples_per_s": round(1920 * 1080 * 16 / dt / 1e6, 1), "ms": round(dt * 1e3, 1),
+                             "mrays_per_s": round((st["rays"] + st["shadow_rays"]) / dt / 1e6, 1),
+                             "triangles": int(sc["idx"].shape[0])}
+    log(f"many-light RIS: {out['many_light_ris']}")
+    # path-replay adjoint, 8 orbit views
+    sc = scenes.cornell(512, 512)
+    ctx.upload_scene(sc)
+    rng = np.random.default_rng(0)
+    dldi = rng.uniform(-1, 1, (512, 512, 3)).astype(np.float32)
+    views = []
+    for k in range(8):
+        ang = np.radians(45.0 * k)
+        views.append(((3.8 * np.sin(ang), 1.0, 3.8 * np.cos(ang)), (0.0, 1.0, 0.0)))
+
+    def adjoint_batch():
+        for eye, look in views:
+            ctx.set_camera_look(eye, look, (0, 1, 0), 40.0, 512, 512)
+            ctx.adjoint(dldi, spp=16, flags=FLAG_NEE | FLAG_MIS, max_depth=6, seed=3)
+
+    adjoint_batch()
+    dt, _ = timed(adjoint_batch)
+    out["adjoint"] = {"ms_per_batch": round(dt * 1e3, 1),
+                      "msamples_per_s": round(8 * 512 * 512 * 16 / dt / 1e6, 1),
+                      "note": "primal + replay per view, gradients downloaded and permuted on the host"}
+    log(f"adjoint: {out['adjoint']}")
+    return out
 
 
 def _traffic(W, H):

@@ -340,10 +340,60 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& sc, Pcg32& 
 // albedo once per visit, and each emitted or light-sampled contribution scales
 // with its light's radiance.  ps.adj_mode 0 is the primal pass (stores
 // nothing extra); 1 replays the identical paths and scatters dLoss/dparam.
-__device__ __forceinline__ void adj_add_le(const PathSoA& ps, uint32_t li, V3 w, V3 c, V3 le) {
-  if (le.x > 0.0f) unsafeAtomicAdd(ps.grad_le + 3u * li + 0u, w.x * c.x / le.x);
-  if (le.y > 0.0f) unsafeAtomicAdd(ps.grad_le + 3u * li + 1u, w.y * c.y / le.y);
-  if (le.z > 0.0f) unsafeAtomicAdd(ps.grad_le + 3u * li + 2u, w.z * c.z / le.z);
+// Block-level accumulation of (key -> 3 floats) in LDS before the global
+// float atomics.  The adjoint's derivatives land on few addresses (a handful
+// of lights; the triangles a block's paths hit), whose HBM atomics would
+// otherwise serialise.  A key that finds no slot in 8 probes goes to HBM.
+constexpr uint32_t kAccEmpty = 0xFFFFFFFFu;
+template <uint32_t N>
+struct LdsAcc3 {
+  uint32_t key[N];
+  float val[N][3];
+};
+
+template <uint32_t N>
+__device__ __forceinline__ void acc_init(LdsAcc3<N>& t) {
+  for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+    t.key[i] = kAccEmpty;
+    t.val[i][0] = t.val[i][1] = t.val[i][2] = 0.0f;
+  }
+}
+
+template <uint32_t N>
+__device__ __forceinline__ void acc_add(LdsAcc3<N>& t, float* dst, uint32_t key, V3 v) {
+  uint32_t h = (key * 0x9E3779B1u) & (N - 1u);
+  for (int probe = 0; probe < 8; ++probe) {
+    uint32_t prev = atomicCAS(&t.key[h], kAccEmpty, key);
+    if (prev == kAccEmpty || prev == key) {
+      if (v.x != 0.0f) atomicAdd(&t.val[h][0], v.x);
+      if (v.y != 0.0f) atomicAdd(&t.val[h][1], v.y);
+      if (v.z != 0.0f) atomicAdd(&t.val[h][2], v.z);
+      return;
+    }
+    h = (h + 1u) & (N - 1u);
+  }
+  if (v.x != 0.0f) unsafeAtomicAdd(dst + 3u * key + 0u, v.x);
+  if (v.y != 0.0f) unsafeAtomicAdd(dst + 3u * key + 1u, v.y);
+  if (v.z != 0.0f) unsafeAtomicAdd(dst + 3u * key + 2u, v.z);
+}
+
+template <uint32_t N>
+__device__ __forceinline__ void acc_flush(LdsAcc3<N>& t, float* dst) {
+  for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+    uint32_t k = t.key[i];
+    if (k == kAccEmpty) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (t.val[i][c] != 0.0f) unsafeAtomicAdd(dst + 3u * k + (uint32_t)c, t.val[i][c]);
+  }
+}
+
+// d L / d Le of light li for a contribution c = (...) * Le, per channel.
+template <uint32_t N>
+__device__ __forceinline__ void adj_add_le(LdsAcc3<N>& t, const PathSoA& ps, uint32_t li, V3 w, V3 c, V3 le) {
+  V3 g{le.x > 0.0f ? w.x * c.x / le.x : 0.0f, le.y > 0.0f ? w.y * c.y / le.y : 0.0f,
+       le.z > 0.0f ? w.z * c.z / le.z : 0.0f};
+  acc_add(t, ps.grad_le, li, g);
 }
 
 __device__ __forceinline__ V3 adj_weight(const PathSoA& ps, uint32_t pid) {
@@ -391,6 +441,13 @@ __device__ __forceinline__ void block_append2(uint32_t* ca, bool pa, uint32_t* c
 template <int kGuide, bool kAdj = false, bool kRis = false>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd,
                                                        HashGridView hg, MortonView mq, uint32_t depth) {
+  __shared__ LdsAcc3<kAdj ? 1024u : 1u> acc_alb;
+  __shared__ LdsAcc3<kAdj ? 256u : 1u> acc_le;
+  if (kAdj) {
+    acc_init(acc_alb);
+    acc_init(acc_le);
+    __syncthreads();
+  }
   uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
   uint32_t n_act = ps.count[depth];
   bool active = i < n_act;
@@ -453,7 +510,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
             }
           }
           c = T * V3{e.x, e.y, e.z} * w;
-          if (adj) adj_add_le(ps, li, adj_w, c, V3{e.x, e.y, e.z});
+          if (adj) adj_add_le(acc_le, ps, li, adj_w, c, V3{e.x, e.y, e.z});
         }
       } else if (depth + 1u < rp.max_depth) {
         float4 sh = sc.shade[tri];
@@ -464,9 +521,10 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
           // This vertex's NEE and everything after it carry one factor of albedo.
           float4 lt = ps.adjL[pid], lc = ps.L[pid];
           V3 rest{lt.x - lc.x, lt.y - lc.y, lt.z - lc.z};
-          if (albedo.x > 0.0f) unsafeAtomicAdd(ps.grad_alb + 3u * tri + 0u, adj_w.x * rest.x / albedo.x);
-          if (albedo.y > 0.0f) unsafeAtomicAdd(ps.grad_alb + 3u * tri + 1u, adj_w.y * rest.y / albedo.y);
-          if (albedo.z > 0.0f) unsafeAtomicAdd(ps.grad_alb + 3u * tri + 2u, adj_w.z * rest.z / albedo.z);
+          acc_add(acc_alb, ps.grad_alb, (uint32_t)tri,
+                  V3{albedo.x > 0.0f ? adj_w.x * rest.x / albedo.x : 0.0f,
+                     albedo.y > 0.0f ? adj_w.y * rest.y / albedo.y : 0.0f,
+                     albedo.z > 0.0f ? adj_w.z * rest.z / albedo.z : 0.0f});
         }
         V3 spec{0.0f, 0.0f, 0.0f};
         if (type == kMatPhong && sc.spec) {
@@ -713,6 +771,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
   }
   uint32_t slot, sslot;
   block_append2(&ps.count[depth + 1u], cont, &ps.scount[depth], shadow, slot, sslot);
+  if (kAdj) {
+    __syncthreads();
+    acc_flush(acc_alb, ps.grad_alb);
+    acc_flush(acc_le, ps.grad_le);
+  }
   if (cont) ps.queue[(depth + 1u) & 1][slot] = pid;
   if (shadow) {
     float4* e = ps.shadow + 3u * (size_t)sslot;
@@ -725,6 +788,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
 template <bool kCount, bool kAdj = false>
 __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
   __shared__ uint2 lds[kShortStack][kBlock];
+  __shared__ LdsAcc3<kAdj ? 256u : 1u> acc_le;
+  if (kAdj) {
+    acc_init(acc_le);
+    __syncthreads();
+  }
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t n_sh = ps.scount[depth];
   uint32_t cnt[2] = {0u, 0u};
@@ -748,13 +816,17 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uin
     if (kAdj && ps.adj_mode != 0u) {
       uint32_t li = fbits(c.w);
       float4 le = sc.lights[4u * li + 3u];
-      adj_add_le(ps, li, adj_weight(ps, pid), V3{c.x, c.y, c.z}, V3{le.x, le.y, le.z});
+      adj_add_le(acc_le, ps, li, adj_weight(ps, pid), V3{c.x, c.y, c.z}, V3{le.x, le.y, le.z});
     }
     if (record && pid < ps.rec_limit) {
       size_t vi = (size_t)depth * ps.n + pid;
       float4 vb = ps.vB[vi];
       ps.vB[vi] = make_float4(vb.x + c.x, vb.y + c.y, vb.z + c.z, vb.w);
     }
+  }
+  if (kAdj) {
+    __syncthreads();
+    acc_flush(acc_le, ps.grad_le);
   }
 }
 

@@ -977,6 +977,20 @@ orc_scene* orc_scene_create(const float* verts, uint32_t nv, const uint32_t* idx
   return s;
 }
 
+/* Light selection probabilities per light (light order = emissive triangles
+ * of positive area in triangle order), e.g. the reference's uniform mesh then
+ * uniform triangle choice (MeshLightList::Sample, SimpleKernel.h:114-120);
+ * rebuilds the selection CDF the same way as the uniform default. */
+void orc_scene_set_light_pdf(orc_scene* s, const float* pdf) {
+  double acc = 0.0;
+  for (uint32_t k = 0; k < s->nl; ++k) {
+    s->light[15 * k + 10] = pdf[k];
+    acc += pdf[k];
+    s->cdf[k] = (float)acc;
+  }
+  if (s->nl) s->cdf[s->nl - 1] = 1.0f;
+}
+
 void orc_scene_destroy(orc_scene* s) {
   if (!s) return;
   free(s->tri);

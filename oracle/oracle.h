@@ -93,6 +93,7 @@ void orc_brute_closest(const float* verts, const uint32_t* idx, uint32_t nt, con
 
 /* ---- integrator ---- */
 typedef struct orc_scene orc_scene;
+void orc_scene_set_light_pdf(orc_scene* s, const float* pdf);
 orc_scene* orc_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* albedo,
                             const uint32_t* mat, const float* emission, const float* spec);
 void orc_scene_destroy(orc_scene* s);

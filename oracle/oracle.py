@@ -73,6 +73,7 @@ def lib():
         L.orc_split_threshold.argtypes = [C.c_int, C.c_float, C.c_int, C.c_int]
         L.orc_scene_create.restype = C.c_void_p
         L.orc_scene_create.argtypes = [f32p, C.c_uint32, u32p, C.c_uint32, f32p, u32p, f32p, f32p]
+        L.orc_scene_set_light_pdf.argtypes = [C.c_void_p, f32p]
         L.orc_scene_destroy.argtypes = [C.c_void_p]
         L.orc_render.restype = C.c_uint64
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(SdTree), f32p]
@@ -322,12 +323,28 @@ def brute_closest(verts, idx, rays):
 
 # ---- integrator
 class Scene:
-    def __init__(self, sc):
+    def __init__(self, sc, light_mesh=None):
         L = lib()
         self._keep = [_f(sc["verts"]), _u(sc["idx"]), _f(sc["albedo"]), _u(sc["mat"]), _f(sc["emission"]),
                       None if sc.get("spec") is None else _f(sc["spec"])]
         v, i, a, m, e, s = self._keep
         self.h = L.orc_scene_create(_p(v), v.shape[0], _p(i, u32p), i.shape[0], _p(a), _p(m, u32p), _p(e), _p(s))
+        if light_mesh is not None:
+            # uniform light mesh, then uniform triangle within it
+            verts, idx = sc["verts"].astype(np.float32), sc["idx"]
+            em = np.asarray(sc["emission"])
+            lt = []
+            for t in range(idx.shape[0]):
+                if em[t].sum() <= 0:
+                    continue
+                p0, p1, p2 = verts[idx[t]]
+                if 0.5 * np.linalg.norm(np.cross(p1 - p0, p2 - p0)) > 0:
+                    lt.append(t)
+            meshes = np.asarray(light_mesh)[lt]
+            uniq, counts = np.unique(meshes, return_counts=True)
+            cnt = dict(zip(uniq.tolist(), counts.tolist()))
+            pdf = np.array([1.0 / (len(uniq) * cnt[m]) for m in meshes.tolist()], np.float32)
+            L.orc_scene_set_light_pdf(C.c_void_p(self.h), _p(pdf))
 
     def __del__(self):
         try:

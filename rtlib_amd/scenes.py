@@ -105,6 +105,45 @@ def cornell(width=256, height=256, fovy=40.0):
     return b.finish(cam)
 
 
+def cornell_materials(width=64, height=64, fovy=40.0):
+    """Cornell box with the reference's other materials and light layout: the
+    tall box normalised Phong (diffuse 0.3, specular 0.5, shininess 30), the
+    short box glass (ior 1.5), and two light meshes (the ceiling quad and a
+    smaller quad on the back wall, light_mesh ids 0 (4 triangles) and 1 (2
+    triangles)) for the uniform mesh-then-triangle selection.  Returns
+    (scene, light_mesh)."""
+    b = _Builder()
+    b.quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), WHITE)
+    b.quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), WHITE)
+    b.quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), WHITE)
+    b.quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), RED)
+    b.quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), GREEN)
+    n_walls = sum(len(i) for i in b.i)
+    b.box((0.33, 0.0, 0.3), (0.6, 0.6, 0.6), -17.0, WHITE)
+    n_short = sum(len(i) for i in b.i)
+    b.box((-0.33, 0.0, -0.35), (0.6, 1.2, 0.6), 17.0, (0.3, 0.3, 0.3))
+    n_tall = sum(len(i) for i in b.i)
+    # ceiling light as two quads (4 triangles, mesh 0); back-wall light 2 triangles (mesh 1)
+    b.quad((-0.25, 1.98, -0.2), (0.0, 1.98, -0.2), (0.0, 1.98, 0.2), (-0.25, 1.98, 0.2), (0, 0, 0),
+           emission=CORNELL_LE)
+    b.quad((0.0, 1.98, -0.2), (0.25, 1.98, -0.2), (0.25, 1.98, 0.2), (0.0, 1.98, 0.2), (0, 0, 0),
+           emission=CORNELL_LE)
+    b.quad((0.5, 1.2, -0.99), (0.8, 1.2, -0.99), (0.8, 1.5, -0.99), (0.5, 1.5, -0.99), (0, 0, 0),
+           emission=(4.0, 6.0, 9.0))
+    cam = {"eye": (0.0, 1.0, 3.8), "lookat": (0.0, 1.0, 0.0), "up": (0, 1, 0), "fovy": fovy,
+           "width": width, "height": height}
+    nt = sum(len(i) for i in b.i)
+    spec = np.zeros((nt, 4), np.float32)
+    s = b.finish(cam, spec=spec)
+    s["mat"][n_walls:n_short] = MAT_GLASS
+    s["albedo"][n_walls:n_short] = (1.0, 1.0, 1.0, 1.5)
+    s["mat"][n_short:n_tall] = MAT_PHONG
+    spec[n_short:n_tall] = (0.5, 0.5, 0.5, 30.0)
+    light_mesh = np.zeros(nt, np.uint32)
+    light_mesh[nt - 2:] = 1
+    return s, light_mesh
+
+
 def _value_noise(nx, nz, octaves, seed):
     """fBm of bilinearly interpolated value noise on an nx x nz grid in [0,1]^2."""
     rng = np.random.default_rng(seed)

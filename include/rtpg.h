@@ -184,6 +184,18 @@ int rtpg_sdtree_build(rtpg_sdtree tree);
  * positions pos[3n], canonical directions canon[2n]. */
 int rtpg_sdtree_record(rtpg_sdtree tree, const float* pos, const float* canon, const float* irradiance,
                        const float* stat_weight, uint32_t n);
+/* Recording filters (reference SpatialFilter / DirectionalFilter, :31-43;
+ * TraceVertexT::Commit<sFilter, dFilter>, :806-813, 830-866): spatial and
+ * directional are 0 (Nearest) or 1 (Box).  Box filters are applied in their
+ * stochastic form: each record moves to a uniform point of the reference's box
+ * (the voxel of its STree leaf centred on the vertex; the square of edge
+ * 0.5^depth centred on its direction) and is recorded there, so every node's
+ * expected statistical weight and flux equal the reference's box-filtered
+ * values.  The draws are a function of (params.seed, pixel, sample, vertex
+ * depth) in rtpg_render, and of (seed, record index, number of earlier
+ * rtpg_sdtree_record calls since this one) in rtpg_sdtree_record.  Applies to
+ * later recording until changed. */
+int rtpg_sdtree_set_filter(rtpg_sdtree tree, int spatial, int directional, uint64_t seed);
 /* STree lookup (STreeT::GetDTreeWrapper, :639-648): DTree index per position. */
 int rtpg_sdtree_lookup(rtpg_sdtree tree, const float* pos, uint32_t n, uint32_t* dtree);
 /* Draw canonical directions from the sampling trees (DTreeNodeT::Sample,

@@ -601,12 +601,192 @@ static void dtree_record1(float* b_sw, float* b_sums, const uint16_t* b_child, u
   atomic_addf(&b_sums[4 * ((size_t)base + node) + idx], irr * sw);
 }
 
+/* ---- recording filters ------------------------------------------------
+ * Stochastic form (what the GPU library runs): the record moves to a uniform
+ * point of the reference's box and is recorded there, so each node's expected
+ * weight and flux equal the reference's box-filtered values.  Its own stream
+ * per (seed, pixel, sample, vertex depth), five draws: three for the spatial
+ * point, two for the directional one. */
+enum { FILTER_SPATIAL = 1, FILTER_DIRECTIONAL = 2 };
+
+static pcg32 filter_rng(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t depth) {
+  return pcg32_seed(mix64(seed ^ 0xD6E8FEB86659FD93ULL ^ mix64(((uint64_t)sample << 8) | depth)), (uint64_t)pixel);
+}
+
+/* STreeT::GetDTreeWrapper(p, size) :639-648 with the descent of :556-567,
+ * also returning the depth of the leaf (levels below the root). */
+static uint32_t stree_lookup_depth(const uint32_t* stree, const float* lo, float size, v3 p, uint32_t* depth) {
+  float q[3] = {(p.x - lo[0]) / size, (p.y - lo[1]) / size, (p.z - lo[2]) / size};
+  uint32_t w = stree[0], d = 0;
+  int axis = 0;
+  while (!(w & LEAF_BIT)) {
+    float pa = q[axis] * 2.0f;
+    int idx = (int)pa;
+    idx = idx < 0 ? 0 : (idx > 1 ? 1 : idx);
+    q[axis] = pa - (float)idx;
+    w = stree[w + (uint32_t)idx];
+    axis = axis == 2 ? 0 : axis + 1;
+    ++d;
+  }
+  *depth = d;
+  return w & ~LEAF_BIT;
+}
+
+/* Spatial box (TraceVertexT::Impl<SpatialFilterBox> :806-813, STreeT::Record
+ * :649-655): the box is the voxel of the leaf holding p (GetDTreeWrapper's
+ * size, halved once per level along the level's axis) centred on p; returns
+ * the leaf holding p + (u - 1/2) * voxel, INVALID outside the tree's cube. */
+static uint32_t stree_box_jitter1(const uint32_t* stree, const float* lo, float size, v3 p, float u0, float u1,
+                                  float u2) {
+  uint32_t d;
+  (void)stree_lookup_depth(stree, lo, size, p, &d);
+  float e[3] = {size, size, size};
+  for (uint32_t k = 0; k < d; ++k) e[k % 3] *= 0.5f;
+  v3 q = V(p.x + (u0 - 0.5f) * e[0], p.y + (u1 - 0.5f) * e[1], p.z + (u2 - 0.5f) * e[2]);
+  float qx = (q.x - lo[0]) / size, qy = (q.y - lo[1]) / size, qz = (q.z - lo[2]) / size;
+  if (!(qx >= 0.0f && qx < 1.0f && qy >= 0.0f && qy < 1.0f && qz >= 0.0f && qz < 1.0f)) return INVALID;
+  return stree_lookup1(stree, lo, size, q);
+}
+
+/* DTreeNodeT::GetDepth :322-333 on the building tree rooted at `base`. */
+static int dtree_depth1(const uint16_t* b_child, uint32_t base, float cu, float cv) {
+  uint32_t node = 0;
+  int depth = 1;
+  int idx = child_idx(&cu, &cv);
+  for (;;) {
+    uint16_t c = b_child[4 * ((size_t)base + node) + idx];
+    if (c == 0) break;
+    node = c;
+    idx = child_idx(&cu, &cv);
+    ++depth;
+  }
+  return depth;
+}
+
+/* Directional box (DTreeT::Impl<DirectionalFilterBox> :429-443): (u, v)
+ * moves by (r - 1/2) 0.5^depth; 0 when it leaves [0,1)^2 (flux dropped). */
+static int dtree_box_jitter1(const uint16_t* b_child, uint32_t base, float* cu, float* cv, float r0, float r1) {
+  int depth = dtree_depth1(b_child, base, *cu, *cv);
+  float e = 1.0f;
+  for (int k = 0; k < depth; ++k) e *= 0.5f;
+  *cu = *cu + (r0 - 0.5f) * e;
+  *cv = *cv + (r1 - 0.5f) * e;
+  return *cu >= 0.0f && *cu < 1.0f && *cv >= 0.0f && *cv < 1.0f;
+}
+
+/* filter: FILTER_* bits; the draws of record i come from
+ * filter_rng(seed, i, batch, 0). */
 void orc_sdtree_record(const uint32_t* stree, const float* aabb_min, float size, const uint32_t* b_base,
                        float* b_sw, float* b_sums, const uint16_t* b_child, const float* pos, const float* canon,
-                       const float* irr, const float* sw, uint32_t n) {
+                       const float* irr, const float* sw, uint32_t n, uint32_t filter, uint64_t seed,
+                       uint32_t batch) {
   for (uint32_t i = 0; i < n; ++i) {
-    uint32_t cell = stree_lookup1(stree, aabb_min, size, ld3(pos + 3 * i));
-    dtree_record1(b_sw, b_sums, b_child, b_base[cell], cell, canon[2 * i], canon[2 * i + 1], irr[i], sw[i]);
+    v3 p = ld3(pos + 3 * i);
+    uint32_t cell = stree_lookup1(stree, aabb_min, size, p);
+    float cu = canon[2 * i], cv = canon[2 * i + 1], val = irr[i];
+    if (filter) {
+      pcg32 fr = filter_rng(seed, i, batch, 0);
+      float f[5];
+      for (int k = 0; k < 5; ++k) f[k] = rnd(&fr);
+      if (filter & FILTER_SPATIAL) cell = stree_box_jitter1(stree, aabb_min, size, p, f[0], f[1], f[2]);
+      if (cell == INVALID) continue;
+      if ((filter & FILTER_DIRECTIONAL) && !dtree_box_jitter1(b_child, b_base[cell], &cu, &cv, f[3], f[4])) val = 0.0f;
+    }
+    dtree_record1(b_sw, b_sums, b_child, b_base[cell], cell, cu, cv, val, sw[i]);
+  }
+}
+
+/* Deterministic box filters exactly as the reference states them (not run on
+ * the GPU; the reference to which the stochastic form is checked in
+ * expectation).  DTreeNodeT::Record(origin, size, nodeOrigin, nodeSize,
+ * value) :126-160: every leaf slot overlapped by the square [origin,
+ * origin + size]^2 receives value times the overlap area. */
+static void dtree_record_square(float* b_sums, const uint16_t* b_child, uint32_t base, uint32_t node, float ox,
+                                float oy, float size, float nox, float noy, float nsize, float value) {
+  float cs = nsize / 2.0f;
+  for (int i = 0; i < 4; ++i) {
+    float cx = nox + ((i & 1) ? cs : 0.0f), cy = noy + ((i & 2) ? cs : 0.0f);
+    float lx = fmaxf(fminf(ox + size, cx + cs) - fmaxf(ox, cx), 0.0f);
+    float ly = fmaxf(fminf(oy + size, cy + cs) - fmaxf(oy, cy), 0.0f);
+    float w = lx * ly;
+    if (w > 0.0f) {
+      uint16_t c = b_child[4 * ((size_t)base + node) + i];
+      if (c == 0) {
+        b_sums[4 * ((size_t)base + node) + i] += value * w;
+      } else {
+        dtree_record_square(b_sums, b_child, base, c, ox, oy, size, cx, cy, cs, value);
+      }
+    }
+  }
+}
+
+/* DTreeT::Impl<dFilter>::RecordIrradiance :416-443 into DTree `cell`. */
+static void dtree_record_filtered(float* b_sw, float* b_sums, const uint16_t* b_child, uint32_t base,
+                                  uint32_t cell, float cu, float cv, float irr, float sw, int box) {
+  if (!box) {
+    dtree_record1(b_sw, b_sums, b_child, base, cell, cu, cv, irr, sw);
+    return;
+  }
+  if (!(isfinite(sw) && sw > 0.0f)) return;
+  b_sw[cell] += sw;
+  if (!(isfinite(irr) && irr > 0.0f)) return;
+  int depth = dtree_depth1(b_child, base, cu, cv);
+  float e = powf(0.5f, (float)depth);
+  dtree_record_square(b_sums, b_child, base, 0, cu - e / 2.0f, cv - e / 2.0f, e, 0.0f, 0.0f, 1.0f,
+                      irr * sw / (e * e));
+}
+
+/* STreeNodeT::Record :573-611: every leaf whose voxel [mn, mn + sz] overlaps
+ * the box [b0, b1] records with statistical weight sw times the overlap
+ * volume (sw already divided by the box volume, STreeT::Record :649-655). */
+static void stree_record_box(const uint32_t* stree, uint32_t word, int axis, const float* mn, const float* sz,
+                             const float* b0, const float* b1, const uint32_t* b_base, float* b_sw, float* b_sums,
+                             const uint16_t* b_child, float cu, float cv, float irr, float sw, int dbox) {
+  float w = 1.0f;
+  for (int k = 0; k < 3; ++k) w *= fmaxf(fminf(b1[k], mn[k] + sz[k]) - fmaxf(b0[k], mn[k]), 0.0f);
+  if (!(w > 0.0f)) return;
+  if (word & LEAF_BIT) {
+    uint32_t cell = word & ~LEAF_BIT;
+    dtree_record_filtered(b_sw, b_sums, b_child, b_base[cell], cell, cu, cv, irr, sw * w, dbox);
+    return;
+  }
+  float csz[3] = {sz[0], sz[1], sz[2]};
+  csz[axis] /= 2.0f;
+  for (int i = 0; i < 2; ++i) {
+    float cmn[3] = {mn[0], mn[1], mn[2]};
+    if (i) cmn[axis] += csz[axis];
+    stree_record_box(stree, stree[word + (uint32_t)i], axis == 2 ? 0 : axis + 1, cmn, csz, b0, b1, b_base, b_sw,
+                     b_sums, b_child, cu, cv, irr, sw, dbox);
+  }
+}
+
+/* The reference's filtered record (TraceVertexT::Impl<sFilter>::Record
+ * :796-813 -> DTreeWrapperT::Record -> DTreeT::Impl<dFilter>) of n records;
+ * spatial, directional: 0 Nearest, 1 Box. */
+void orc_sdtree_record_box(const uint32_t* stree, const float* aabb_min, float size, const uint32_t* b_base,
+                           float* b_sw, float* b_sums, const uint16_t* b_child, const float* pos,
+                           const float* canon, const float* irr, const float* sw, uint32_t n, int spatial,
+                           int directional) {
+  for (uint32_t i = 0; i < n; ++i) {
+    v3 p = ld3(pos + 3 * i);
+    uint32_t d;
+    uint32_t cell = stree_lookup_depth(stree, aabb_min, size, p, &d);
+    if (!spatial) {
+      dtree_record_filtered(b_sw, b_sums, b_child, b_base[cell], cell, canon[2 * i], canon[2 * i + 1], irr[i],
+                            sw[i], directional);
+      continue;
+    }
+    float e[3] = {size, size, size};
+    for (uint32_t k = 0; k < d; ++k) e[k % 3] /= 2.0f;
+    float pc[3] = {p.x, p.y, p.z};
+    float b0[3], b1[3], mn[3] = {aabb_min[0], aabb_min[1], aabb_min[2]}, sz[3] = {size, size, size};
+    for (int k = 0; k < 3; ++k) {
+      b0[k] = pc[k] - e[k] * 0.5f;
+      b1[k] = pc[k] + e[k] * 0.5f;
+    }
+    float volume = e[0] * e[1] * e[2];
+    stree_record_box(stree, stree[0], 0, mn, sz, b0, b1, b_base, b_sw, b_sums, b_child, canon[2 * i],
+                     canon[2 * i + 1], irr[i], sw[i] / volume, directional);
   }
 }
 
@@ -1219,6 +1399,7 @@ static lsample sample_light(const orc_scene* s, pcg32* r) {
 
 typedef struct {
   v3 T;       /* throughput after the bounce */
+  v3 pos;     /* hit point (for the spatial box filter) */
   float wo_pdf, cosv, cu, cv;
   uint32_t cell, flags;
 } vtx;
@@ -1450,6 +1631,7 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
           int ok = finite3(be.f) && be.f.x >= 0.0f && be.f.y >= 0.0f && be.f.z >= 0.0f;
           vx.flags = ok ? 0u : VTX_BAD_BSDF;
           vx.cell = cell;
+          vx.pos = pos;
           nd = dir;
           cont = max3(Tn) > 0.0f && isfinite(max3(Tn));
         }
@@ -1490,7 +1672,19 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
       ly *= cs;
       lz *= cs;
       float value = ((lx + ly + lz) / 3.0f) / v->wo_pdf;
-      dtree_record1(g->b_sw, g->b_sums, g->b_child, g->b_base[v->cell], v->cell, v->cu, v->cv, value, 1.0f);
+      uint32_t cell = v->cell;
+      float cu = v->cu, cv = v->cv;
+      if (g->filter) {
+        /* stochastic box filters, own stream per (pixel, sample, vertex depth) */
+        pcg32 fr = filter_rng(p->seed, pix, p->sample_offset + smp, dv);
+        float f[5];
+        for (int k = 0; k < 5; ++k) f[k] = rnd(&fr);
+        if (g->filter & FILTER_SPATIAL) cell = stree_box_jitter1(g->stree, g->aabb_min, g->size, v->pos, f[0], f[1], f[2]);
+        if (cell == INVALID) continue;
+        if ((g->filter & FILTER_DIRECTIONAL) && !dtree_box_jitter1(g->b_child, g->b_base[cell], &cu, &cv, f[3], f[4]))
+          value = 0.0f;
+      }
+      dtree_record1(g->b_sw, g->b_sums, g->b_child, g->b_base[cell], cell, cu, cv, value, 1.0f);
     }
   }
   *Lout = L;

@@ -78,7 +78,12 @@ void orc_stree_lookup(const uint32_t* stree, const float* aabb_min, float size, 
                       uint32_t* out);
 void orc_sdtree_record(const uint32_t* stree, const float* aabb_min, float size, const uint32_t* b_base,
                        float* b_sw, float* b_sums, const uint16_t* b_child, const float* pos, const float* canon,
-                       const float* irr, const float* sw, uint32_t n);
+                       const float* irr, const float* sw, uint32_t n, uint32_t filter, uint64_t seed,
+                       uint32_t batch);
+void orc_sdtree_record_box(const uint32_t* stree, const float* aabb_min, float size, const uint32_t* b_base,
+                           float* b_sw, float* b_sums, const uint16_t* b_child, const float* pos,
+                           const float* canon, const float* irr, const float* sw, uint32_t n, int spatial,
+                           int directional);
 void orc_sdtree_sample(const uint32_t* s_base, const float* s_sum, const float* s_sw, const float* s_sums,
                        const uint16_t* s_child, const uint32_t* cells, uint32_t n, uint64_t seed, float* out);
 void orc_sdtree_pdf(const uint32_t* s_base, const float* s_sum, const float* s_sw, const float* s_sums,
@@ -121,6 +126,7 @@ typedef struct orc_sdtree {
   float* b_sw;
   float* b_sums;
   const uint16_t* b_child;
+  uint32_t filter; /* recording filters: 1 spatial box, 2 directional box (stochastic form) */
 } orc_sdtree;
 /* accum: 3*W*H floats, added to; returns the number of radiance rays traced. */
 uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum);

@@ -38,7 +38,7 @@ class Params(C.Structure):
 class SdTree(C.Structure):
     _fields_ = [("stree", u32p), ("aabb_min", C.c_float * 3), ("size", C.c_float), ("s_base", u32p),
                 ("s_sum", f32p), ("s_sw", f32p), ("s_sums", f32p), ("s_child", u16p), ("b_base", u32p),
-                ("b_sw", f32p), ("b_sums", f32p), ("b_child", u16p)]
+                ("b_sw", f32p), ("b_sums", f32p), ("b_child", u16p), ("filter", C.c_uint32)]
 
 
 class Grid(C.Structure):
@@ -280,8 +280,10 @@ def stree_lookup(stree, aabb_min, size, pos):
     return out
 
 
-def sdtree_record(st, aabb_min, size, pos, canon, irr, sw):
-    """Sequential record into a copy of the state's building trees."""
+def sdtree_record(st, aabb_min, size, pos, canon, irr, sw, filter=0, seed=0, batch=0):
+    """Sequential record into a copy of the state's building trees.  filter:
+    bit 0 spatial box, bit 1 directional box, in the stochastic form (draws of
+    record i from the filter stream (seed, i, batch, 0))."""
     out = {k: np.array(v, copy=True) for k, v in st.items()}
     pos = _f(pos, (-1, 3))
     b_sw = np.ascontiguousarray(out["scal"][0], np.float32).copy()
@@ -289,7 +291,26 @@ def sdtree_record(st, aabb_min, size, pos, canon, irr, sw):
     lo = _f(aabb_min, (3,))
     lib().orc_sdtree_record(_p(_u(out["stree"]), u32p), _p(lo), C.c_float(size), _p(_bases(out["b_cnt"]), u32p),
                             _p(b_sw), _p(bs), _p(np.ascontiguousarray(out["b_child"], np.uint16), u16p), _p(pos),
-                            _p(_f(canon)), _p(_f(irr)), _p(_f(sw)), pos.shape[0])
+                            _p(_f(canon)), _p(_f(irr)), _p(_f(sw)), C.c_uint32(pos.shape[0]), C.c_uint32(filter),
+                            C.c_uint64(seed), C.c_uint32(batch))
+    out["scal"][0] = b_sw
+    out["b_sums"] = bs
+    return out
+
+
+def sdtree_record_box(st, aabb_min, size, pos, canon, irr, sw, spatial, directional):
+    """The reference's deterministic filtered record (spatial, directional:
+    0 Nearest, 1 Box) into a copy of the state's building trees."""
+    out = {k: np.array(v, copy=True) for k, v in st.items()}
+    pos = _f(pos, (-1, 3))
+    b_sw = np.ascontiguousarray(out["scal"][0], np.float32).copy()
+    bs = np.ascontiguousarray(out["b_sums"], np.float32)
+    lo = _f(aabb_min, (3,))
+    lib().orc_sdtree_record_box(_p(_u(out["stree"]), u32p), _p(lo), C.c_float(size),
+                                _p(_bases(out["b_cnt"]), u32p), _p(b_sw), _p(bs),
+                                _p(np.ascontiguousarray(out["b_child"], np.uint16), u16p), _p(pos), _p(_f(canon)),
+                                _p(_f(irr)), _p(_f(sw)), C.c_uint32(pos.shape[0]), C.c_int(spatial),
+                                C.c_int(directional))
     out["scal"][0] = b_sw
     out["b_sums"] = bs
     return out
@@ -399,7 +420,7 @@ class Guide:
     """Binds a state dict (exported layout) to the integrator: sampling trees
     read, building trees recorded into (in place, in `state`)."""
 
-    def __init__(self, state, aabb_min, size):
+    def __init__(self, state, aabb_min, size, filter=0):
         self.state = state
         st = state
         self.s_base = _bases(st["s_cnt"])
@@ -426,4 +447,5 @@ class Guide:
         g.b_sw = self.scal[0].ctypes.data_as(f32p)
         g.b_sums = _p(self.b_sums)
         g.b_child = _p(self.b_child, u16p)
+        g.filter = filter
         self.struct = g

@@ -104,6 +104,13 @@ RTPG_HD Pcg32 path_rng(uint64_t seed, uint32_t pixel, uint32_t sample) {
   return pcg32_seed(mix64(seed ^ mix64((uint64_t)sample)), (uint64_t)pixel);
 }
 
+// Stream of the recording filters for the vertex at `depth` of path (pixel,
+// sample): separate from the path's own stream, so switching the filters on
+// leaves every path unchanged.
+RTPG_HD Pcg32 filter_rng(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t depth) {
+  return pcg32_seed(mix64(seed ^ 0xD6E8FEB86659FD93ULL ^ mix64(((uint64_t)sample << 8) | depth)), (uint64_t)pixel);
+}
+
 // 23 random mantissa bits -> [0,1): the reference's to_uniform_float.
 RTPG_HD float to_uniform_float(uint32_t v) {
   uint32_t bits = (v >> 9) | 0x3F800000u;

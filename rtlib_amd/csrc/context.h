@@ -78,7 +78,13 @@ struct rtpg_sdtree_t {
   std::vector<SNode> snodes;
   rtpg::DBuf<uint32_t> d_stree;
   rtpg::DBuf<uint32_t> d_jump;  // lookup table of the first 3 * kJumpBits levels
+  rtpg::DBuf<uint32_t> d_leaf_depth;  // per DTree: depth of its STree leaf
   static constexpr uint32_t kJumpBits = 5;
+  // recording filters (RecordFilter bits); rtpg_sdtree_record draws from
+  // filter_rng(filter_seed, i, record_batch, 0), record_batch counting its calls
+  uint32_t filter = 0;
+  uint64_t filter_seed = 0;
+  uint32_t record_batch = 0;
   void upload_stree();
   // device DTree arrays (owned)
   struct Owned {

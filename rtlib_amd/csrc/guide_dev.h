@@ -55,6 +55,34 @@ __device__ __forceinline__ uint32_t stree_lookup(const SdTreeView& g, V3 p) {
   return w & ~kLeafBit;
 }
 
+// Stochastic form of the spatial box filter (SpatialFilterBox: TraceVertexT
+// ::Impl<Box>::Record :806-813 -> STreeT::Record :649-655 -> STreeNodeT::Record
+// :573-611).  The reference gives every leaf overlapped by a box the record's
+// statistical weight times w_j / volume (w_j: overlap volume), the box being
+// the voxel of the leaf that holds the vertex, centred on the vertex.  Here
+// one point is drawn uniformly in that box (u0..u2 in [0,1)) and the whole
+// record goes to the leaf holding the point: leaf j receives it with
+// probability w_j / volume, so each leaf's expected weight and flux are the
+// reference's, for one extra lookup instead of a subtree walk.  A point
+// outside the tree's cube returns kInvalidCell (the record is dropped; the
+// reference's overlaps there are empty).  A leaf at depth D (split axis =
+// depth mod 3) has been halved ceil(D/3) times along x, ceil((D-1)/3) along y
+// and floor(D/3) along z.
+__device__ __forceinline__ uint32_t stree_box_jitter(const SdTreeView& g, V3 p, uint32_t cell, float u0, float u1,
+                                                     float u2) {
+#pragma clang fp contract(off)
+  uint32_t d = g.leaf_depth[cell];
+  float ex = ldexpf(g.aabb_size, -(int)((d + 2u) / 3u));
+  float ey = ldexpf(g.aabb_size, -(int)((d + 1u) / 3u));
+  float ez = ldexpf(g.aabb_size, -(int)(d / 3u));
+  V3 q{p.x + (u0 - 0.5f) * ex, p.y + (u1 - 0.5f) * ey, p.z + (u2 - 0.5f) * ez};
+  float qx = (q.x - g.aabb_min[0]) / g.aabb_size;
+  float qy = (q.y - g.aabb_min[1]) / g.aabb_size;
+  float qz = (q.z - g.aabb_min[2]) / g.aabb_size;
+  bool inside = qx >= 0.0f && qx < 1.0f && qy >= 0.0f && qy < 1.0f && qz >= 0.0f && qz < 1.0f;
+  return inside ? stree_lookup(g, q) : kInvalidCell;
+}
+
 RTPG_HD uint32_t dnode_child(uint2 c, int idx) {
   uint32_t w = idx < 2 ? c.x : c.y;
   return (w >> ((idx & 1) * 16)) & 0xFFFFu;
@@ -85,6 +113,36 @@ RTPG_HD float dtree_mean(float sum, float sw) {
   if (sw <= 0.0f) return 0.0f;
   float factor = kInv2Pi / (2.0f * sw);
   return factor * sum;
+}
+
+// Stochastic form of the directional box filter (DirectionalFilterBox:
+// DTreeT::Impl<Box>::RecordIrradiance :429-443 -> DTreeNodeT::Record
+// :126-169).  The reference spreads the flux over the leaf slots overlapped by
+// a square of edge 0.5^depth centred on (u, v), depth being that of the slot
+// holding (u, v) (DTreeNodeT::GetDepth :322-333), each slot taking its share
+// of the square's area.  Here (u, v) moves to a uniform point of the square
+// (r0, r1 in [0,1)), so each slot's expected flux is the reference's.  Returns
+// false when the point leaves [0,1)^2: the flux is then dropped, the
+// statistical weight still counts (as in the reference).
+__device__ __forceinline__ bool dtree_box_jitter(const SdTreeView& g, uint32_t cell, float& u, float& v, float r0,
+                                                 float r1) {
+#pragma clang fp contract(off)
+  uint32_t base = fbits(g.info[cell].w);
+  float pu = u, pv = v;
+  int depth = 1;
+  uint32_t node = 0;
+  int idx = dtree_child_idx(pu, pv);
+  for (;;) {
+    uint32_t ch = dnode_child(g.build_child[base + node], idx);
+    if (ch == 0) break;
+    node = ch;
+    idx = dtree_child_idx(pu, pv);
+    ++depth;
+  }
+  float e = ldexpf(1.0f, -depth);
+  u = u + (r0 - 0.5f) * e;
+  v = v + (r1 - 0.5f) * e;
+  return u >= 0.0f && u < 1.0f && v >= 0.0f && v < 1.0f;
 }
 
 // DTreeNodeT::Pdf (:278-316) times 1/(4 pi) as in DTreeT::Pdf (:484-489):

@@ -753,6 +753,18 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
       ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
       if (kGuide != 3) {
         bool has_vertex = cont && vcell != kInvalidCell;
+        if (kGuide == 1 && sd.filter && has_vertex) {
+          // stochastic box filters: the record moves to a random point of the
+          // reference's boxes (own stream); u < 0 marks a dropped flux
+          uint32_t npix = rp.width * rp.height;
+          Pcg32 fr = filter_rng(rp.seed, pid % npix, rp.sample_offset + pid / npix, depth);
+          float f0 = rnd(fr), f1 = rnd(fr), f2 = rnd(fr), f3 = rnd(fr), f4 = rnd(fr);
+          if (sd.filter & kFilterSpatialBox) vcell = stree_box_jitter(sd, p, vcell, f0, f1, f2);
+          has_vertex = vcell != kInvalidCell;
+          if (has_vertex && (sd.filter & kFilterDirectionalBox) && !dtree_box_jitter(sd, vcell, vcu, vcv, f3, f4)) {
+            vcu = -1.0f;
+          }
+        }
         ps.vC[vi] = make_float4(vcu, vcv, bitsf(has_vertex ? vcell : kInvalidCell),
                                 bitsf(vflags | (has_vertex ? kVtxValid : 0u)));
       } else if (pid < ps.mlp_rec && (!cont || (vflags & kVtxDelta))) {
@@ -878,42 +890,6 @@ hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, ucha
 }
 
 // ------------------------------------------------------------------ guide records
-// TraceVertexT::Commit (OPX7UtilsPathGuiding.h:830-866): the radiance that
-// reached vertex d is the sum of the contributions of the later hits, in hit
-// order; divided by the vertex throughput (per-channel guard T*woPdf > 1e-4),
-// times |cos|, averaged over RGB, divided by woPdf.
-__device__ __forceinline__ bool commit_value(const PathSoA& ps, uint32_t pid, uint32_t d, uint32_t nh,
-                                             uint32_t& cell, float& cu, float& cv, float& value) {
-  size_t vi = (size_t)d * ps.n + pid;
-  float4 vc = ps.vC[vi];
-  uint32_t fl = fbits(vc.w);
-  cell = fbits(vc.z);
-  if (!(fl & kVtxValid) || (fl & kVtxDelta) || cell == kInvalidCell) return false;
-  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-  for (uint32_t k = d + 1u; k < nh; ++k) {
-    float4 b = ps.vB[(size_t)k * ps.n + pid];
-    rx += b.x;
-    ry += b.y;
-    rz += b.z;
-  }
-  float4 va = ps.vA[vi];
-  float wo_pdf = va.w;
-  bool ok = isfinite(rx) && rx >= 0.0f && isfinite(ry) && ry >= 0.0f && isfinite(rz) && rz >= 0.0f;
-  if (wo_pdf <= 0.0f || !ok || (fl & kVtxBadBsdf)) return false;
-  float lx = va.x * wo_pdf > 1e-4f ? rx / va.x : 0.0f;
-  float ly = va.y * wo_pdf > 1e-4f ? ry / va.y : 0.0f;
-  float lz = va.z * wo_pdf > 1e-4f ? rz / va.z : 0.0f;
-  float cosv = fabsf(ps.vB[vi].w);
-  lx *= cosv;
-  ly *= cosv;
-  lz *= cosv;
-  float avg = (lx + ly + lz) / 3.0f;
-  value = avg / wo_pdf;
-  cu = vc.x;
-  cv = vc.y;
-  return true;
-}
-
 // Recording runs in two stages.  Stage 1 (one work-item per path) turns the
 // path's vertices into compact records (DTree or hash slot, canonical u, v,
 // value), appended with one atomic per wave.  Stage 2 applies a contiguous run
@@ -1039,7 +1015,8 @@ __global__ void __launch_bounds__(kApplyBlock) k_commit_apply(const float4* recs
     float cu = rec.y, cv = rec.z;
     if (kKind == 1) {
       if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) unsafeAtomicAdd(sd.build_sw + cell, 1.0f);
-      if (!(isfinite(value) && value > 0.0f)) continue;
+      // cu < 0: flux dropped by the directional box filter
+      if (!(isfinite(value) && value > 0.0f) || cu < 0.0f) continue;
       uint32_t base = fbits(sd.info[cell].w);
       uint32_t node = 0;
       int idx = dtree_child_idx(cu, cv);

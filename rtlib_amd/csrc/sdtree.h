@@ -59,7 +59,7 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, 
 
 // Test/driver kernels on an SdTreeView.
 hipError_t launch_sdtree_record(SdTreeView v, const float* pos, const float* canon, const float* irr,
-                                const float* sw, uint32_t n, hipStream_t s);
+                                const float* sw, uint32_t n, uint64_t seed, uint32_t batch, hipStream_t s);
 hipError_t launch_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n, uint64_t seed, float* out_canon,
                                 hipStream_t s);
 hipError_t launch_sdtree_pdf(SdTreeView v, const uint32_t* cells, const float* canon, uint32_t n, float* out_pdf,

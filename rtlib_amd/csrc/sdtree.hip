@@ -230,18 +230,29 @@ __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* tile_sums)
 }
 
 // DTreeT::Impl<Nearest>::RecordIrradiance (:419-427) at the DTree found by
-// the STree lookup of `pos`.
+// the STree lookup of `pos`; with v.filter set, the stochastic box filters
+// (stree_box_jitter, dtree_box_jitter) move the record first, drawing five
+// numbers from filter_rng(seed, i, batch, 0).
 __global__ void k_sdtree_record(SdTreeView v, const float* pos, const float* canon, const float* irr,
-                                const float* sw, uint32_t n) {
+                                const float* sw, uint32_t n, uint64_t seed, uint32_t batch) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t cell = stree_lookup(v, V3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+  V3 p{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  uint32_t cell = stree_lookup(v, p);
+  float cu = canon[2 * i], cv = canon[2 * i + 1];
+  bool flux = true;
+  if (v.filter) {
+    Pcg32 fr = filter_rng(seed, i, batch, 0u);
+    float f0 = rnd(fr), f1 = rnd(fr), f2 = rnd(fr), f3 = rnd(fr), f4 = rnd(fr);
+    if (v.filter & kFilterSpatialBox) cell = stree_box_jitter(v, p, cell, f0, f1, f2);
+    if (cell == kInvalidCell) return;
+    if (v.filter & kFilterDirectionalBox) flux = dtree_box_jitter(v, cell, cu, cv, f3, f4);
+  }
   float w = sw[i];
   if (!(isfinite(w) && w > 0.0f)) return;
   unsafeAtomicAdd(v.build_sw + cell, w);
   float val = irr[i];
-  if (!(isfinite(val) && val > 0.0f)) return;
-  float cu = canon[2 * i], cv = canon[2 * i + 1];
+  if (!(isfinite(val) && val > 0.0f) || !flux) return;
   uint32_t base = fbits(v.info[cell].w);
   uint32_t node = 0;
   int idx = dtree_child_idx(cu, cv);
@@ -316,9 +327,9 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, 
   return e;
 }
 hipError_t launch_sdtree_record(SdTreeView v, const float* pos, const float* canon, const float* irr,
-                                const float* sw, uint32_t n, hipStream_t s) {
+                                const float* sw, uint32_t n, uint64_t seed, uint32_t batch, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sdtree_record, dim3(nblk(n, 256)), dim3(256), 0, s, v, pos, canon, irr, sw, n);
+  hipLaunchKernelGGL(k_sdtree_record, dim3(nblk(n, 256)), dim3(256), 0, s, v, pos, canon, irr, sw, n, seed, batch);
   return hipGetLastError();
 }
 hipError_t launch_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n, uint64_t seed, float* out,

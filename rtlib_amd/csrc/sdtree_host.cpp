@@ -5,6 +5,8 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/rtpg.h"
 #include "context.h"
@@ -115,10 +117,30 @@ static std::vector<uint32_t> build_jump(const std::vector<rtpg_sdtree_t::SNode>&
   return jt;
 }
 
+// Depth of every DTree's leaf (root = 0), for the voxel size of the spatial
+// box filter.
+static std::vector<uint32_t> leaf_depths(const std::vector<rtpg_sdtree_t::SNode>& sn) {
+  std::vector<uint32_t> out;
+  std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 0u}};  // (node, depth)
+  while (!stack.empty()) {
+    auto [i, d] = stack.back();
+    stack.pop_back();
+    if (sn[i].leaf) {
+      if (sn[i].dtree >= out.size()) out.resize((size_t)sn[i].dtree + 1u, 0u);
+      out[sn[i].dtree] = d;
+    } else {
+      stack.push_back({sn[i].child, d + 1u});
+      stack.push_back({sn[i].child + 1u, d + 1u});
+    }
+  }
+  return out;
+}
+
 void rtpg_sdtree_t::upload_stree() {
   hipStream_t s = ctx->stream;
   d_stree.upload(encode_stree(snodes), s);
   d_jump.upload(build_jump(snodes, kJumpBits), s);
+  d_leaf_depth.upload(leaf_depths(snodes), s);
 }
 
 SdTreeView rtpg_sdtree_t::view() {
@@ -137,6 +159,8 @@ SdTreeView rtpg_sdtree_t::view() {
   v.samp_node = dt.s_node.p;
   v.fraction = fraction;
   v.n_dtrees = dt.n;
+  v.filter = filter;
+  v.leaf_depth = d_leaf_depth.p;
   return v;
 }
 
@@ -436,8 +460,20 @@ int rtpg_sdtree_record(rtpg_sdtree tree, const float* pos, const float* canon, c
     dc.upload(canon, 2 * (size_t)n, s);
     di.upload(irradiance, n, s);
     dw.upload(stat_weight, n, s);
-    RTPG_HIP(launch_sdtree_record(tree->view(), dp.p, dc.p, di.p, dw.p, n, s));
+    RTPG_HIP(launch_sdtree_record(tree->view(), dp.p, dc.p, di.p, dw.p, n, tree->filter_seed, tree->record_batch++,
+                                  s));
     RTPG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int rtpg_sdtree_set_filter(rtpg_sdtree tree, int spatial, int directional, uint64_t seed) {
+  return guarded([&] {
+    require(tree != nullptr, "tree is NULL");
+    require((spatial == 0 || spatial == 1) && (directional == 0 || directional == 1),
+            "filters are 0 (nearest) or 1 (box)");
+    tree->filter = (spatial ? kFilterSpatialBox : 0u) | (directional ? kFilterDirectionalBox : 0u);
+    tree->filter_seed = seed;
+    tree->record_batch = 0;
   });
 }
 

@@ -69,6 +69,15 @@ struct SdTreeView {
   const uint4* samp_node;
   float fraction;
   uint32_t n_dtrees;
+  // recording filters: bit 0 spatial box, bit 1 directional box; per DTree the
+  // depth of its STree leaf (root = 0), for the voxel size of the spatial box
+  uint32_t filter;
+  const uint32_t* leaf_depth;
+};
+
+enum RecordFilter : uint32_t {
+  kFilterSpatialBox = 1u,      // reference SpatialFilterBox
+  kFilterDirectionalBox = 2u,  // reference DirectionalFilterBox
 };
 
 // Hash grid + Morton quadtrees (reference OPX7UtilsGrid.h / OPX7UtilsMorton.h).

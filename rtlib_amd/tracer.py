@@ -248,6 +248,11 @@ class RTSTreeWrapper:
     def SetSplit(self, mode=0, c=4000.0, max_mb=2000, subdiv_th=0.01):
         call("rtpg_sdtree_set_split", self.h, mode, c, max_mb, subdiv_th)
 
+    def SetFilter(self, spatial=0, directional=0, seed=0):
+        """Recording filters, the reference's SpatialFilter / DirectionalFilter
+        (OPX7UtilsPathGuiding.h:31-43): 0 Nearest, 1 Box (stochastic form)."""
+        call("rtpg_sdtree_set_filter", self.h, spatial, directional, seed)
+
     def Reset(self, iteration, samples_per_pass):
         call("rtpg_sdtree_reset", self.h, iteration, samples_per_pass)
 

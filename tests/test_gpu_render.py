@@ -75,8 +75,10 @@ def test_chunked_launch_equals_single_launch(ctx):
     assert np.array_equal(a, b) or np.max(np.abs(a - b)) < 1e-5
 
 
-def test_guided_render_matches_cpu_with_same_tree(ctx, orc):
-    # A trained tree drives both integrators: sampling from it and recording into it.
+@pytest.mark.parametrize("filt", [0, 3])
+def test_guided_render_matches_cpu_with_same_tree(ctx, orc, filt):
+    # A trained tree drives both integrators: sampling from it and recording
+    # into it, with nearest or stochastic box recording filters.
     sc = scenes.cornell(32, 32)
     ctx.upload_scene(sc)
     info = ctx.scene_info()
@@ -89,18 +91,23 @@ def test_guided_render_matches_cpu_with_same_tree(ctx, orc):
     tree.Build()
     tree.Reset(1, 4)
     state = tree.Download()
+    tree.SetFilter(spatial=filt & 1, directional=filt >> 1)
     ctx.clear()
     ctx.render(spp=8, flags=flags | FLAG_GUIDE_SAMPLE, seed=6, sdtree=tree, fraction=0.3)
     g = ctx.image()
     after = tree.Download()
     size = float(np.max(hi - lo))
-    guide = orc.Guide({k: np.array(v, copy=True) for k, v in state.items()}, lo, size)
+    guide = orc.Guide({k: np.array(v, copy=True) for k, v in state.items()}, lo, size, filter=filt)
     c = _cpu(orc, sc, 8, flags | FLAG_GUIDE_SAMPLE, seed=6, fraction=0.3, guide=guide)
     assert _rel_mse(g, c) < 2e-3, _rel_mse(g, c)
     # recorded statistical weights agree exactly (each vertex adds 1.0) up to
     # the few paths that diverged
     sw_g, sw_c = after["scal"][0], guide.scal[0]
     assert abs(sw_g.sum() - sw_c.sum()) / sw_c.sum() < 2e-3
+    # and land in the same DTrees (the spatial filter moves them by the same draws)
+    assert np.abs(sw_g - sw_c).sum() / sw_c.sum() < 5e-3
+    fl_g, fl_c = after["b_sums"].sum(), guide.b_sums.sum()
+    assert abs(fl_g - fl_c) / fl_c < 5e-2
 
 
 def test_guided_controller_is_unbiased(ctx):

@@ -80,6 +80,31 @@ def test_lookup_sample_pdf_match_oracle(ctx, orc):
     assert np.array_equal(tree.Pdf(cells, canon), orc.sdtree_pdf(st, cells, canon))
 
 
+@pytest.mark.parametrize("filt", [1, 2, 3])
+def test_box_filtered_record_matches_oracle(ctx, orc, filt):
+    # stochastic box filters (bit 0 spatial, bit 1 directional): the device and
+    # the oracle draw the same points from the filter stream, so the recorded
+    # trees are identical; two calls check the per-call batch index
+    rng = np.random.default_rng(5 + filt)
+    tree = _tree(ctx)
+    tree.SetSplit(mode=1, c=300.0)
+    st = tree.Download()
+    for it in range(3):
+        if it:
+            tree.Reset(it, 1)
+            st = orc.sdtree_reset(st, it, 1, mode=1, c=300.0)
+        st = _pass(tree, orc, st, rng, 6000)
+    tree.Reset(3, 1)
+    st = orc.sdtree_reset(st, 3, 1, mode=1, c=300.0)
+    assert tree.Sizes()["dtrees"] > 8
+    tree.SetFilter(spatial=filt & 1, directional=filt >> 1, seed=99)
+    for batch in range(2):
+        pos, canon, irr, sw = _records(rng, 20000)
+        tree.Record(pos, canon, irr, sw)
+        st = orc.sdtree_record(st, LO, SIZE, pos, canon, irr, sw, filter=filt, seed=99, batch=batch)
+        _assert_state_equal(tree.Download(), st)
+
+
 def test_import_export_round_trip_and_validation(ctx, orc):
     rng = np.random.default_rng(2)
     tree = _tree(ctx)

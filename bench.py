@@ -157,6 +157,9 @@ def main(argv=None):
 
     guided_step(1, on_pass=acc_work)
     ctx.set_profile(False, False)
+    # the cost of the per-kernel events: step 0 again without them
+    dt_noev, _ = timed(lambda: guided_step(1))
+    log(f"step 0 without kernel events: {dt_noev * 1e3:.1f} ms")
 
     t_step = float(np.median(times))
     samples = a.budget * npix
@@ -210,7 +213,17 @@ def main(argv=None):
         grid_img = ctx.image()
         log(f"hash-grid guided step: {t_grid * 1e3:.1f} ms ({a.budget} spp)")
 
+        # SD-tree with the stochastic box recording filters (spatial + directional)
+        tree.SetFilter(1, 1, 17)
+        timed(lambda: guided_step(90))
+        t_box, _ = timed(lambda: guided_step(91))
+        box_img = ctx.image()
+        box_sizes = tree.Sizes()
+        tree.SetFilter(0, 0)
+        log(f"SD-tree guided step, box filters: {t_box * 1e3:.1f} ms ({a.budget} spp)")
+
         spp_tree, plain_tree = plain_equal_time(t_step, 78)
+        spp_box, plain_box = plain_equal_time(t_box, 81)
         spp_mlp, plain_mlp = plain_equal_time(t_mlp, 79)
         spp_grid, plain_grid = plain_equal_time(t_grid, 80)
         # reference image (plain, many samples, independent seed)
@@ -228,6 +241,9 @@ def main(argv=None):
             "plain_at_budget": {"spp": a.budget, "relmse": round(rel_mse(plain_budget_img, ref), 5)},
             "sdtree": {"ms": round(t_step * 1e3, 1), "relmse": round(rel_mse(guided_img, ref), 5),
                        "plain_spp": spp_tree, "plain_relmse": round(rel_mse(plain_tree, ref), 5)},
+            "sdtree_box_filters": {"ms": round(t_box * 1e3, 1), "relmse": round(rel_mse(box_img, ref), 5),
+                                   "plain_spp": spp_box, "plain_relmse": round(rel_mse(plain_box, ref), 5),
+                                   "dtrees": box_sizes["dtrees"]},
             "hashgrid": {"ms": round(t_grid * 1e3, 1), "relmse": round(rel_mse(grid_img, ref), 5),
                          "plain_spp": spp_grid, "plain_relmse": round(rel_mse(plain_grid, ref), 5)},
             "mlp": {"ms": round(t_mlp * 1e3, 1), "relmse": round(rel_mse(mlp_img, ref), 5),
@@ -268,6 +284,7 @@ def main(argv=None):
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": round(t_step * 1e3, 2),
+        "ms_per_step_without_kernel_events": round(dt_noev * 1e3, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

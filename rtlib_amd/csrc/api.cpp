@@ -520,9 +520,33 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     ctx->p_stats.zero(4, s);
     const bool timing = (ctx->prof_mode & 1u) != 0;
     const bool counting = (ctx->prof_mode & 2u) != 0;
+    // events of up to kEvChunks chunks are read back together, so timing adds
+    // one host synchronisation per kEvChunks chunks
+    constexpr uint32_t kEvChunks = 8;
+    const size_t ev_stride = (size_t)kTimedKernels * kMaxDepthEv * 2;
+    uint32_t ev_pending = 0;
+    auto drain_events = [&] {
+      if (ev_pending == 0) return;
+      RTPG_HIP(hipStreamSynchronize(s));
+      for (uint32_t c = 0; c < ev_pending; ++c) {
+        const hipEvent_t* ev = ctx->events.data() + c * ev_stride;
+        for (int k = 0; k < kTimedKernels; ++k) {
+          uint32_t nd = (k <= 2) ? prm->max_depth : 1u;
+          for (uint32_t d = 0; d < nd; ++d) {
+            float ms = 0.0f;
+            RTPG_HIP(hipEventElapsedTime(&ms, ev[((size_t)k * kMaxDepthEv + d) * 2],
+                                         ev[((size_t)k * kMaxDepthEv + d) * 2 + 1]));
+            ctx->kernel_ms[k] += ms;
+          }
+        }
+      }
+      ev_pending = 0;
+    };
     if (timing && ctx->events.empty()) {
-      ctx->events.resize((size_t)kTimedKernels * kMaxDepthEv * 2);
-      for (hipEvent_t& e : ctx->events) RTPG_HIP(hipEventCreate(&e));
+      ctx->events.resize(kEvChunks * ev_stride);
+      // timing-only events: no system-scope fence (cache writeback and
+      // invalidate) at each record, which would slow the kernels being timed
+      for (hipEvent_t& e : ctx->events) RTPG_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     }
     for (double& m : ctx->kernel_ms) m = 0.0;
     for (uint64_t& l : ctx->kernel_launches) l = 0;
@@ -536,28 +560,18 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       uint32_t n = cs * npix;
       PathSoA ps = make_soa(ctx, n, prm->max_depth, guide != 0, counting, guide == 3 ? prm->mlp : nullptr);
       RenderParams rp = make_rp(ctx, prm, cs, done);
+      if (timing && ev_pending == kEvChunks) drain_events();
       RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, ml, guide, ctx->accum.p, s,
-                                   timing ? ctx->events.data() : nullptr));
+                                   timing ? ctx->events.data() + (ev_pending++) * ev_stride : nullptr));
       RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));
       for (int k = 0; k < 3; ++k) ctx->kernel_launches[k] += prm->max_depth;
       ctx->kernel_launches[3] += 1;
       if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL) && !(guide == 2 && (prm->flags & RTPG_FLAG_LOCATE)))
         ctx->kernel_launches[4] += guide == 3 ? 1 : 2;
       if (guide == 3 && (prm->flags & RTPG_FLAG_GUIDE_SAMPLE)) ctx->kernel_launches[1] += prm->max_depth;
-      if (timing) {
-        RTPG_HIP(hipStreamSynchronize(s));
-        for (int k = 0; k < kTimedKernels; ++k) {
-          uint32_t nd = (k <= 2) ? prm->max_depth : 1u;
-          for (uint32_t d = 0; d < nd; ++d) {
-            float ms = 0.0f;
-            RTPG_HIP(hipEventElapsedTime(&ms, ctx->events[((size_t)k * kMaxDepthEv + d) * 2],
-                                         ctx->events[((size_t)k * kMaxDepthEv + d) * 2 + 1]));
-            ctx->kernel_ms[k] += ms;
-          }
-        }
-      }
       done += cs;
     }
+    if (timing) drain_events();
     if (counting) {
       unsigned long long w[4];
       ctx->p_work.download(w, 4, s);

@@ -110,6 +110,24 @@ def test_guided_render_matches_cpu_with_same_tree(ctx, orc, filt):
     assert abs(fl_g - fl_c) / fl_c < 5e-2
 
 
+def test_kernel_timing_and_work_counts(ctx):
+    # 16 chunks: the per-kernel events are read back in two batches of 8
+    sc = scenes.cornell(64, 64)
+    flags = FLAG_NEE | FLAG_MIS | FLAG_RR
+    plain = _gpu(ctx, sc, 32, flags, max_depth=6, seed=1, max_paths=64 * 64 * 2)
+    ctx.set_profile(time_kernels=True, count_work=True)
+    try:
+        prof = _gpu(ctx, sc, 32, flags, max_depth=6, seed=1, max_paths=64 * 64 * 2)
+        ms, work, nl = ctx.kernel_stats()
+    finally:
+        ctx.set_profile(False, False)
+    assert np.array_equal(plain, prof)
+    assert nl["extend"] == 6 * 16 and nl["shade"] == 6 * 16 and nl["accumulate"] == 16
+    assert all(ms[k] > 0 for k in ("extend", "shade", "shadow", "accumulate")), ms
+    st = ctx.render_stats()
+    assert work["extend_nodes"] >= st["rays"] and work["extend_tris"] > 0 and work["shadow_nodes"] > 0
+
+
 def test_guided_controller_is_unbiased(ctx):
     sc = scenes.cornell(32, 32)
     base = FLAG_NEE | FLAG_MIS | FLAG_RR

@@ -178,6 +178,15 @@ int rtpg_sdtree_set_split(rtpg_sdtree tree, int mode, float c, int max_mb, float
 /* RTSTreeWrapperT::Reset(iter, spp) (:1645-1656): refine the spatial tree from
  * the recorded weights, then rebuild every building DTree from its sampling tree. */
 int rtpg_sdtree_reset(rtpg_sdtree tree, int iter, int samples_per_pass);
+/* Octree spatial refinement, the reference's STree2 variant (RTSTreeWrapper2T /
+ * RTSTree2T::Refine, SubDivide, :2062-2077, 2108-2142; device side STree2T,
+ * :662-783): a leaf whose weight exceeds the threshold splits into 8 children
+ * of half the edge, each with 1/8 of the weight (instead of 2 with 1/2).  The
+ * octree is stored as three binary levels per split, so lookup, sampling and
+ * the recording filters are those of the binary tree.  For the reference's
+ * RTSTreeWrapper2T::Reset also set rtpg_sdtree_set_split(tree, 0, 4000, 4000,
+ * 0.01).  Takes effect at the next rtpg_sdtree_reset. */
+int rtpg_sdtree_set_octree(rtpg_sdtree tree, int enable);
 /* RTSTreeWrapperT::Build (:1657-1735): sums, areas, sampling := building. */
 int rtpg_sdtree_build(rtpg_sdtree tree);
 /* Device record of n samples (DTreeT::RecordIrradiance, :419-427) at

@@ -481,10 +481,13 @@ uint64_t orc_split_threshold(int mode, float c, int iter, int spp) {
   return (uint64_t)((double)c * sqrt(pow(2.0, iter)));
 }
 
-/* RTSTreeT::Refine :1275-1311 with ShallSplit :1270-1274 and SubDivide :1231-1247 */
+/* RTSTreeT::Refine :1275-1311 with ShallSplit :1270-1274 and SubDivide
+ * :1231-1247; octree != 0: RTSTree2T::Refine :2108-2142 with SubDivide
+ * :2062-2077 (8 children, weight / 8), the octree node stored as three binary
+ * levels (pairs at n, n+2..n+5, leaves n+6..n+13). */
 uint32_t orc_stree_refine(const uint32_t* stree, uint32_t n_nodes, const float* b_sw, const uint32_t* b_cnt,
                           const uint32_t* s_cnt, uint64_t th, int max_mb, uint32_t* out, uint32_t cap, uint32_t* map,
-                          uint32_t* n_dtrees_out) {
+                          uint32_t* n_dtrees_out, int octree) {
   typedef struct {
     int leaf;
     uint32_t child;
@@ -513,7 +516,31 @@ uint32_t orc_stree_refine(const uint32_t* stree, uint32_t n_nodes, const float* 
     st[sp++] = 0;
     while (sp > 0) {
       uint32_t idx = st[--sp];
-      if (h[idx].leaf && (uint64_t)n < 0xFFFFFFFFull - 1 && h[idx].sw > (float)th) {
+      if (octree && h[idx].leaf && (uint64_t)n < 0xFFFFFFFFull - 1 && h[idx].sw > (float)th) {
+        if (n + 14 > cap) {
+          free(st);
+          free(h);
+          return 0;
+        }
+        float sw8 = h[idx].sw / 8.0f;
+        int64_t src = h[idx].src;
+        h[idx].leaf = 0;
+        h[idx].child = n;
+        h[idx].src = -1;
+        for (uint32_t k = 0; k < 6; ++k) { /* the x pair's y pairs, then their z pairs */
+          h[n + k].leaf = 0;
+          h[n + k].child = n + 2 + 2 * k;
+          h[n + k].src = -1;
+          h[n + k].sw = 0.0f;
+        }
+        for (uint32_t k = 6; k < 14; ++k) {
+          h[n + k].leaf = 1;
+          h[n + k].child = 0;
+          h[n + k].sw = sw8;
+          h[n + k].src = src;
+        }
+        n += 14;
+      } else if (h[idx].leaf && (uint64_t)n < 0xFFFFFFFFull - 1 && h[idx].sw > (float)th) {
         if (n + 2 > cap) {
           free(st);
           free(h);

@@ -72,7 +72,7 @@ uint32_t orc_dtree_reset(const float* src_sums, const uint16_t* src_child, float
  * the DTree it copies.  Returns the number of nodes, or 0 if cap is too small. */
 uint32_t orc_stree_refine(const uint32_t* stree, uint32_t n_nodes, const float* b_sw, const uint32_t* b_cnt,
                           const uint32_t* s_cnt, uint64_t th, int max_mb, uint32_t* out, uint32_t cap,
-                          uint32_t* map, uint32_t* n_dtrees_out);
+                          uint32_t* map, uint32_t* n_dtrees_out, int octree);
 uint64_t orc_split_threshold(int mode, float c, int iter, int spp);
 void orc_stree_lookup(const uint32_t* stree, const float* aabb_min, float size, const float* pos, uint32_t n,
                       uint32_t* out);

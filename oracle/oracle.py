@@ -68,7 +68,7 @@ def lib():
         L.orc_dtree_reset.argtypes = [f32p, u16p, C.c_float, C.c_uint32, C.c_float, u16p, u32p]
         L.orc_stree_refine.restype = C.c_uint32
         L.orc_stree_refine.argtypes = [u32p, C.c_uint32, f32p, u32p, u32p, C.c_uint64, C.c_int, u32p, C.c_uint32,
-                                       u32p, u32p]
+                                       u32p, u32p, C.c_int]
         L.orc_split_threshold.restype = C.c_uint64
         L.orc_split_threshold.argtypes = [C.c_int, C.c_float, C.c_int, C.c_int]
         L.orc_scene_create.restype = C.c_void_p
@@ -218,8 +218,9 @@ def sdtree_build(st):
     return out
 
 
-def sdtree_reset(st, iteration, spp, mode=0, c=4000.0, max_mb=2000, th=0.01, max_depth=20):
-    """RTSTreeWrapperT::Reset(iter, spp) on a state dict; returns a new dict."""
+def sdtree_reset(st, iteration, spp, mode=0, c=4000.0, max_mb=2000, th=0.01, max_depth=20, octree=False):
+    """RTSTreeWrapperT::Reset(iter, spp) on a state dict (octree: the
+    RTSTree2T refinement); returns a new dict."""
     if iteration <= 0:
         return {k: np.array(v, copy=True) for k, v in st.items()}
     L = lib()
@@ -234,7 +235,7 @@ def sdtree_reset(st, iteration, spp, mode=0, c=4000.0, max_mb=2000, th=0.01, max
         nd = C.c_uint32()
         nn = L.orc_stree_refine(_p(stree, u32p), stree.shape[0], _p(b_sw), _p(_u(st["b_cnt"]), u32p),
                                 _p(_u(st["s_cnt"]), u32p), C.c_uint64(thr), max_mb, _p(out, u32p), cap,
-                                _p(mp, u32p), C.byref(nd))
+                                _p(mp, u32p), C.byref(nd), 1 if octree else 0)
         if nn:
             break
         cap *= 4

@@ -97,6 +97,7 @@ SIGNATURES = {
     "rtpg_sdtree_build": [c_vp],
     "rtpg_sdtree_record": [c_vp, c_fp, c_fp, c_fp, c_fp, C.c_uint32],
     "rtpg_sdtree_set_filter": [c_vp, C.c_int, C.c_int, C.c_uint64],
+    "rtpg_sdtree_set_octree": [c_vp, C.c_int],
     "rtpg_sdtree_lookup": [c_vp, c_fp, C.c_uint32, c_u32p],
     "rtpg_sdtree_sample": [c_vp, c_u32p, C.c_uint32, C.c_uint64, c_fp],
     "rtpg_sdtree_pdf": [c_vp, c_u32p, c_fp, C.c_uint32, c_fp],

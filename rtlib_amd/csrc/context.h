@@ -69,6 +69,7 @@ struct rtpg_sdtree_t {
   float split_c = 4000.0f;
   int max_mb = 2000;
   float subdiv_th = 0.01f;
+  bool octree = false;  // refine splits a leaf into 8 (RTSTree2T) instead of 2
   // Host mirror of the spatial tree topology: leaf flag, first child / DTree.
   struct SNode {
     bool leaf;

@@ -295,7 +295,7 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
       st.pop_back();
       if (hn[idx].leaf) {
         bool split = hn.size() < (size_t)std::numeric_limits<uint32_t>::max() - 1 && hn[idx].sw > (float)th;
-        if (split) {
+        if (split && !octree) {
           size_t c = hn.size();
           hn.resize(c + 2);
           HN& cur = hn[idx];
@@ -311,6 +311,35 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
           cur.leaf = false;
           cur.src = -1;
           cur.sw = 0.0f;
+        } else if (split) {
+          // RTSTree2T::SubDivide (:2062-2077): eight children of half the
+          // edge along every axis, each with an eighth of the weight, here as
+          // three binary levels below the leaf (pairs at c, c+2..c+5, and the
+          // eight leaves at c+6..c+13), so lookups and filters are unchanged
+          size_t c = hn.size();
+          hn.resize(c + 14);
+          const float sw8 = hn[idx].sw / 8.0f;
+          const int64_t src = hn[idx].src;
+          const uint8_t ax = hn[idx].axis;
+          auto make_inner = [&](size_t i, size_t first, uint8_t axis) {
+            hn[i].leaf = false;
+            hn[i].axis = axis;
+            hn[i].child[0] = (uint32_t)first;
+            hn[i].child[1] = (uint32_t)first + 1u;
+            hn[i].sw = 0.0f;
+            hn[i].src = -1;
+          };
+          make_inner(idx, c, ax);
+          for (size_t k = 0; k < 2; ++k) make_inner(c + k, c + 2 + 2 * k, (uint8_t)((ax + 1) % 3));
+          for (size_t k = 0; k < 4; ++k) make_inner(c + 2 + k, c + 6 + 2 * k, (uint8_t)((ax + 2) % 3));
+          for (size_t k = 0; k < 8; ++k) {
+            HN& ch = hn[c + 6 + k];
+            ch.leaf = true;
+            ch.axis = ax;
+            ch.sw = sw8;
+            ch.src = src;
+            ch.child[0] = ch.child[1] = 0;
+          }
         }
       }
       if (!hn[idx].leaf) {
@@ -463,6 +492,14 @@ int rtpg_sdtree_record(rtpg_sdtree tree, const float* pos, const float* canon, c
     RTPG_HIP(launch_sdtree_record(tree->view(), dp.p, dc.p, di.p, dw.p, n, tree->filter_seed, tree->record_batch++,
                                   s));
     RTPG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int rtpg_sdtree_set_octree(rtpg_sdtree tree, int enable) {
+  return guarded([&] {
+    require(tree != nullptr, "tree is NULL");
+    require(enable == 0 || enable == 1, "enable is 0 or 1");
+    tree->octree = enable != 0;
   });
 }
 

@@ -248,6 +248,10 @@ class RTSTreeWrapper:
     def SetSplit(self, mode=0, c=4000.0, max_mb=2000, subdiv_th=0.01):
         call("rtpg_sdtree_set_split", self.h, mode, c, max_mb, subdiv_th)
 
+    def SetOctree(self, enable=True):
+        """Refine by 8-way splits (the reference's STree2 variant)."""
+        call("rtpg_sdtree_set_octree", self.h, 1 if enable else 0)
+
     def SetFilter(self, spatial=0, directional=0, seed=0):
         """Recording filters, the reference's SpatialFilter / DirectionalFilter
         (OPX7UtilsPathGuiding.h:31-43): 0 Nearest, 1 Box (stochastic form)."""
@@ -313,6 +317,18 @@ class RTSTreeWrapper:
         sch = np.ascontiguousarray(d["s_child"], np.uint16)
         call("rtpg_sdtree_import", self.h, st.shape[0], ptr(st, u32), bc.shape[0], ptr(bc, u32), ptr(sc, u32),
              ptr(scal), ptr(bs), ptr(bch, u16), ptr(ss), ptr(sch, u16))
+
+
+
+class RTSTreeWrapper2(RTSTreeWrapper):
+    """RTSTreeWrapper2T (OPX7UtilsPathGuiding.h:2156-2432): the octree spatial
+    tree, with its Reset's fixed split rule (sqrt(2^iter spp / 4) * 4000,
+    4000 MB guard, DTree threshold 0.01) and its fraction 0.5."""
+
+    def __init__(self, ctx: Context, aabb_min, aabb_max, max_dtree_depth=20):
+        super().__init__(ctx, aabb_min, aabb_max, max_dtree_depth, 0.5)
+        self.SetSplit(mode=0, c=4000.0, max_mb=4000, subdiv_th=0.01)
+        self.SetOctree(True)
 
 
 class _Ctrl:

@@ -10,7 +10,7 @@ import pytest
 
 from rtlib_amd import scenes
 from rtlib_amd.tracer import (FLAG_GUIDE_SAMPLE, FLAG_MIS, FLAG_NEE, FLAG_RIS, FLAG_RR, FLAG_USE_TREE, Context,
-                              RTSTreeController, RTSTreeWrapper, render_guided)
+                              RTSTreeController, RTSTreeWrapper, RTSTreeWrapper2, render_guided)
 
 pytestmark = pytest.mark.gpu
 
@@ -128,14 +128,20 @@ def test_kernel_timing_and_work_counts(ctx):
     assert work["extend_nodes"] >= st["rays"] and work["extend_tris"] > 0 and work["shadow_nodes"] > 0
 
 
-def test_guided_controller_is_unbiased(ctx):
+@pytest.mark.parametrize("octree", [False, True])
+def test_guided_controller_is_unbiased(ctx, octree):
     sc = scenes.cornell(32, 32)
     base = FLAG_NEE | FLAG_MIS | FLAG_RR
     plain = _gpu(ctx, sc, 256, base, seed=9)
     ctx.clear()
     info = ctx.scene_info()
-    tree = RTSTreeWrapper(ctx, info["aabb_min"] - 0.005, info["aabb_max"] + 0.005, 20, 0.3)
-    tree.SetSplit(mode=1, c=2000.0)
+    lo, hi = info["aabb_min"] - 0.005, info["aabb_max"] + 0.005
+    if octree:
+        tree = RTSTreeWrapper2(ctx, lo, hi)
+        tree.SetSplit(mode=0, c=500.0, max_mb=4000)
+    else:
+        tree = RTSTreeWrapper(ctx, lo, hi, 20, 0.3)
+        tree.SetSplit(mode=1, c=2000.0)
     ctrl = RTSTreeController(tree, 64, 0, 0.5, 1)
     n = render_guided(ctx, ctrl, tree, "tree", base, max_depth=8, seed=10)
     img = ctx.image()

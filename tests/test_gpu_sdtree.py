@@ -61,6 +61,28 @@ def test_lifecycle_record_build_reset(ctx, orc):
     assert sz["dtrees"] > 8 and sz["sampling_nodes"] > 100
 
 
+def test_octree_lifecycle_matches_oracle(ctx, orc):
+    # the STree2 variant: 8-way spatial splits, the reference's fixed rule
+    # (mode 0, c = 4000) scaled down by c so that a few thousand records split
+    rng = np.random.default_rng(3)
+    tree = _tree(ctx)
+    tree.SetSplit(mode=0, c=60.0)
+    tree.SetOctree(True)
+    st = tree.Download()
+    spp = 1
+    for it in range(4):
+        if it > 0:
+            tree.Reset(it, spp)
+            st = orc.sdtree_reset(st, it, spp, mode=0, c=60.0, octree=True)
+            _assert_state_equal(tree.Download(), st)
+        st = _pass(tree, orc, st, rng, 5000 * (it + 1))
+        spp *= 2
+    n = tree.Sizes()["dtrees"]
+    assert n > 8 and (n - 1) % 7 == 0
+    pos = rng.uniform(0, 1, (4000, 3)).astype(np.float32) * np.array([4, 2, 3], np.float32)
+    assert np.array_equal(tree.Lookup(pos), orc.stree_lookup(st["stree"], LO, SIZE, pos))
+
+
 def test_lookup_sample_pdf_match_oracle(ctx, orc):
     rng = np.random.default_rng(1)
     tree = _tree(ctx)

@@ -376,13 +376,38 @@ def test_stree_refine_split_counts(orc):
         n = orc.lib().orc_stree_refine(orc._p(np.array([0x80000000], np.uint32), orc.u32p), 1,
                                        orc._p(np.array([w], np.float32)), orc._p(np.array([1], np.uint32), orc.u32p),
                                        orc._p(np.array([1], np.uint32), orc.u32p), C.c_uint64(th), 2000,
-                                       orc._p(out, orc.u32p), 1024, orc._p(mp, orc.u32p), C.byref(nd))
+                                       orc._p(out, orc.u32p), 1024, orc._p(mp, orc.u32p), C.byref(nd), 0)
         assert nd.value == leaves and n == 2 * leaves - 1
         assert np.all(mp[:leaves] == 0)
         # every inner node's children are a consecutive pair after it
         for i in range(n):
             if not out[i] & 0x80000000:
                 assert out[i] > i and out[i] + 1 < n
+
+
+def test_stree_octree_refine_matches_uniform_grid(orc):
+    # 8-way splits (the STree2 variant): w = 1000, th = 100 -> 8 leaves of 125
+    # split again -> 64 leaves of 15.6: the 4 x 4 x 4 grid, one DTree per cell,
+    # every leaf three binary levels per octree level (depth 6)
+    import ctypes as C
+
+    for w, th, leaves in [(1000.0, 100, 64), (100.5, 100, 8), (99.0, 100, 1)]:
+        out = np.zeros(4096, np.uint32)
+        mp = np.zeros(4096, np.uint32)
+        nd = C.c_uint32()
+        n = orc.lib().orc_stree_refine(orc._p(np.array([0x80000000], np.uint32), orc.u32p), 1,
+                                       orc._p(np.array([w], np.float32)), orc._p(np.array([1], np.uint32), orc.u32p),
+                                       orc._p(np.array([1], np.uint32), orc.u32p), C.c_uint64(th), 2000,
+                                       orc._p(out, orc.u32p), 4096, orc._p(mp, orc.u32p), C.byref(nd), 1)
+        assert nd.value == leaves and n == 1 + 14 * (leaves - 1) // 7
+        k = round(leaves ** (1 / 3))
+        c = (np.stack(np.meshgrid(*[np.arange(k)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) / k
+        got = orc.stree_lookup(out[:n], (0, 0, 0), 1.0, c.astype(np.float32))
+        assert sorted(got.tolist()) == list(range(leaves))
+        # jittered points stay in the DTree of their grid cell
+        rng = np.random.default_rng(0)
+        jit = (c + rng.uniform(-0.45, 0.45, c.shape) / k).astype(np.float32)
+        assert np.array_equal(orc.stree_lookup(out[:n], (0, 0, 0), 1.0, jit), got)
 
 
 def test_stree_lookup_axis_cycle(orc):

@@ -8,7 +8,7 @@ more tightly than their Monte Carlo error.  Tolerances are stated per test.
 import numpy as np
 import pytest
 
-from rtlib_amd import scenes
+from rtlib_amd import scenefile, scenes
 from rtlib_amd.tracer import (FLAG_GUIDE_SAMPLE, FLAG_MIS, FLAG_NEE, FLAG_RIS, FLAG_RR, FLAG_USE_TREE, Context,
                               RTSTreeController, RTSTreeWrapper, RTSTreeWrapper2, render_guided)
 
@@ -218,3 +218,35 @@ def test_phong_glass_and_mesh_light_selection_match_cpu(ctx, orc):
         e_same, e_indep = _rel_mse(g, c), _rel_mse(other, c)
         assert e_same < 0.5 * e_indep, (flags, e_same, e_indep)
         assert abs(g.mean() - c.mean()) / c.mean() < 0.01 and abs(other.mean() - c.mean()) / c.mean() < 0.02
+
+
+def test_scene_loaded_from_files_renders_like_the_procedural_scene(ctx, orc, tmp_path):
+    # the materials Cornell box written as OBJ/MTL + scene JSON and loaded back
+    # (camera from the JSON's CameraController): GPU vs oracle on the loaded
+    # arrays, and the same image as the procedural scene up to Monte Carlo error
+    sc, lm = scenes.cornell_materials(48, 48)
+    scenefile.save_obj(sc, lm, str(tmp_path / "cornell.obj"))
+    sj = scenefile.default_scene_json(str(tmp_path / "cornell.obj"))
+    sj["CameraController"] = {"Position": [0.0, 1.0, 3.8], "Yaw": -90.0, "Pitch": 0.0, "Zoom": 40.0}
+    sj["Config"].update(Width=48, Height=48)
+    s2, lm2, _ = scenefile.load_scene(sj)
+    flags = FLAG_NEE | FLAG_MIS | FLAG_RR
+    kw = dict(max_depth=8, rr_depth=3, seed=21)
+    ctx.upload_scene(s2, light_mode=1, light_mesh=lm2)
+    ctx.clear()
+    ctx.render(spp=256, flags=flags, **kw)
+    loaded = ctx.image()
+    cam = s2["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 1.0)
+    acc, _ = orc.render(orc.Scene(s2, light_mesh=lm2), uvw, 48, 48, 256, flags, **kw)
+    c = acc / 256
+    ctx.clear()
+    ctx.render(spp=256, flags=flags, **dict(kw, seed=77))
+    other = ctx.image()
+    assert _rel_mse(loaded, c) < 0.5 * _rel_mse(other, c)
+    ctx.upload_scene(sc, light_mode=1, light_mesh=lm)
+    ctx.clear()
+    ctx.render(spp=256, flags=flags, **dict(kw, seed=22))
+    proc = ctx.image()
+    assert abs(loaded.mean() - proc.mean()) / proc.mean() < 0.01
+    assert _rel_mse(loaded, proc) < 2.0 * _rel_mse(other, loaded) + 1e-4

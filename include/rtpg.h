@@ -129,6 +129,22 @@ typedef struct rtpg_render_params {
   rtpg_mlp mlp;            /* learned guide for RTPG_FLAG_USE_MLP, else NULL */
 } rtpg_render_params;
 int rtpg_render(rtpg_context ctx, const rtpg_render_params* params);
+/* Debug frames: the reference's debug programs (__raygen__debug,
+ * __closesthit__debug, __miss__debug, SimpleTrace.cu:104-213, 631-734) and
+ * DebugFrameType (SimpleKernel.h:160-171).  One camera ray per pixel, jittered
+ * as sample params->sample_offset of params->seed, closest hit, then per
+ * frame_type: 0 face normal, 1 hit distance, 2 diffuse colour, 3 specular
+ * colour, 4 emission, 5 Phong shininess, 6 refraction index, 7 grid value
+ * (root weight of the hit cell's building Morton tree; needs RTPG_FLAG_USE_GRID,
+ * params->hashgrid, params->mqtree, else 0), 8 world position (no reference
+ * counterpart).  value: 4 floats per pixel, the raw quantity (xyz) and the hit
+ * primitive's original index as uint32 bits (0xFFFFFFFF on a miss); rgba: the
+ * reference's display bytes (normal (n+1)/2, distance t/(1+t), shininess and
+ * index x/(1+x), then linear_to_gamma x 255.99 clamped).  Either may be NULL.
+ * A miss reads normal (1,0,0), distance 1e20, zeros otherwise.  Pixels in
+ * accumulation-buffer order. */
+int rtpg_render_debug(rtpg_context ctx, const rtpg_render_params* params, uint32_t frame_type, float* value,
+                      uint8_t* rgba);
 /* Last render's counters: [0] paths, [1] radiance rays, [2] shadow rays. */
 int rtpg_render_stats(rtpg_context ctx, uint64_t stats[3]);
 /* Profiling of later renders (no reference counterpart; the reference only

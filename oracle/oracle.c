@@ -1799,6 +1799,53 @@ static void* worker(void* arg) {
   return NULL;
 }
 
+/* Debug frames: __raygen__debug / __closesthit__debug / __miss__debug
+ * (SimpleTrace.cu:104-213, 631-734) for DebugFrameType (SimpleKernel.h:160-171)
+ * 0 normal, 1 distance, 2 diffuse, 3 specular, 4 emission, 5 shininess, 6
+ * refraction index, and 8 world position (7, the grid value, is left to the
+ * caller): per pixel the raw quantity and the hit triangle (-1 on a miss) of
+ * the camera ray of sample p->sample_offset, traced in [1e-4, 1e20].  A miss
+ * gives normal (1,0,0), distance 1e20, zeros otherwise. */
+void orc_debug_frame(const orc_scene* s, const orc_params* p, uint32_t type, float* out3, int32_t* tri_out) {
+  uint32_t npix = p->width * p->height;
+  for (uint32_t pix = 0; pix < npix; ++pix) {
+    pcg32 r = path_rng(p->seed, pix, p->sample_offset);
+    uint32_t x = pix % p->width, y = pix / p->width;
+    float jx = rnd(&r), jy = rnd(&r);
+    float dx = 2.0f * ((float)x + jx) / (float)p->width - 1.0f;
+    float dy = 2.0f * ((float)y + jy) / (float)p->height - 1.0f;
+    v3 o = ld3(p->eye);
+    v3 d = nrm(add(add(scl(ld3(p->u), dx), scl(ld3(p->v), dy)), ld3(p->w)));
+    float t = 0.0f;
+    uint64_t rays = 0;
+    int64_t tri = trace(s, o, d, 1e-4f, 1e20f, 0, &t, &rays);
+    v3 q = V(0, 0, 0);
+    if (tri < 0) {
+      if (type == 0) q = V(1, 0, 0);
+      if (type == 1) q = V(1e20f, 1e20f, 1e20f);
+    } else {
+      const float* tq = s->tri + 9 * tri;
+      const float* al = s->albedo + 4 * tri;
+      const float* sp = s->spec ? s->spec + 4 * tri : NULL;
+      uint32_t mt = s->mat[tri];
+      int light = s->light_of[tri] >= 0;
+      if (type == 0) q = nrm(cross(ld3(tq + 3), ld3(tq + 6)));
+      if (type == 1) q = V(t, t, t);
+      if (type == 2 && mt != M_GLASS && !light) q = ld3(al);
+      if (type == 3 && mt == M_GLASS) q = ld3(al);
+      if (type == 3 && mt == M_PHONG && sp) q = ld3(sp);
+      if (type == 4 && light) q = ld3(s->light + 15 * s->light_of[tri] + 11);
+      if (type == 5 && mt == M_PHONG && sp) q = V(sp[3], sp[3], sp[3]);
+      if (type == 6 && mt == M_GLASS) q = V(al[3], al[3], al[3]);
+      if (type == 8) q = add(o, scl(d, t));
+    }
+    out3[3 * pix] = q.x;
+    out3[3 * pix + 1] = q.y;
+    out3[3 * pix + 2] = q.z;
+    tri_out[pix] = (int32_t)tri;
+  }
+}
+
 uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum) {
   job j;
   j.s = s;

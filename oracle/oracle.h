@@ -128,6 +128,9 @@ typedef struct orc_sdtree {
   const uint16_t* b_child;
   uint32_t filter; /* recording filters: 1 spatial box, 2 directional box (stochastic form) */
 } orc_sdtree;
+/* Debug frame of type 0-6 or 8 (raw quantity per pixel, 3 floats) and the
+ * hit triangle per pixel (-1: miss). */
+void orc_debug_frame(const orc_scene* s, const orc_params* p, uint32_t type, float* out3, int32_t* tri_out);
 /* accum: 3*W*H floats, added to; returns the number of radiance rays traced. */
 uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum);
 /* FP64-accumulated gradient of sum_pixels dldi . image w.r.t. per-triangle

@@ -75,6 +75,7 @@ def lib():
         L.orc_scene_create.argtypes = [f32p, C.c_uint32, u32p, C.c_uint32, f32p, u32p, f32p, f32p]
         L.orc_scene_set_light_pdf.argtypes = [C.c_void_p, f32p]
         L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_debug_frame.argtypes = [C.c_void_p, C.POINTER(Params), C.c_uint32, f32p, C.POINTER(C.c_int32)]
         L.orc_render.restype = C.c_uint64
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(SdTree), f32p]
         L.orc_adjoint.argtypes = [C.c_void_p, C.POINTER(Params), f32p, C.POINTER(C.c_double),
@@ -394,6 +395,18 @@ def render(scene: Scene, camera_uvw, width, height, spp, flags, max_depth=8, rr_
         gp = C.byref(guide.struct)
     rays = lib().orc_render(scene.h, C.byref(p), gp, _p(accum))
     return accum, rays
+
+
+def debug_frame(scene: Scene, camera_uvw, width, height, frame_type, seed=0, sample_offset=0):
+    """Debug frame (orc_debug_frame): (value H x W x 3, triangle H x W, -1 on a miss)."""
+    p = Params()
+    p.width, p.height, p.seed, p.sample_offset = width, height, seed, sample_offset
+    for k in range(3):
+        p.eye[k], p.u[k], p.v[k], p.w[k] = (float(x[k]) for x in camera_uvw)
+    val = np.zeros((height, width, 3), np.float32)
+    tri = np.zeros((height, width), np.int32)
+    lib().orc_debug_frame(scene.h, C.byref(p), frame_type, _p(val), tri.ctypes.data_as(C.POINTER(C.c_int32)))
+    return val, tri
 
 
 def adjoint(scene: Scene, camera_uvw, width, height, spp, flags, dldi, n_lights, max_depth=6, seed=0,

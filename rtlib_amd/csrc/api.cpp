@@ -634,6 +634,37 @@ static int trace_common(rtpg_context ctx, const float* rays, uint32_t n, float* 
   });
 }
 
+extern "C" int rtpg_render_debug(rtpg_context ctx, const rtpg_render_params* prm, uint32_t frame_type,
+                                 float* value, uint8_t* rgba) {
+  return guarded([&] {
+    require(ctx && prm && (value || rgba), "NULL argument");
+    require(ctx->n_tris > 0 && ctx->accum.p != nullptr, "no scene or frame");
+    require(frame_type <= 8u, "frame_type must be in [0, 8]");
+    const bool use_grid = frame_type == 7u && (prm->flags & RTPG_FLAG_USE_GRID) != 0;
+    require(!use_grid || (prm->hashgrid != nullptr && prm->mqtree != nullptr), "USE_GRID needs hashgrid + mqtree");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    uint32_t npix = ctx->width * ctx->height;
+    HashGridView hg{};
+    MortonView mq{};
+    if (use_grid) guide_views(prm, ctx, hg, mq);
+    DBuf<float4> d_val;
+    DBuf<uchar4> d_frame;
+    DBuf<uint32_t> d_deep;
+    if (value) d_val.alloc(npix);
+    if (rgba) d_frame.alloc(npix);
+    if (ctx->max_stack > kShortStack) d_deep.alloc(2 * (size_t)(ctx->max_stack - kShortStack) * npix);
+    ctx->p_err.zero(1, s);
+    RTPG_HIP(launch_debug(make_rp(ctx, prm, 1, 0), ctx->cam, ctx->scene_view(), hg, mq, frame_type, use_grid,
+                          d_val.p, d_frame.p, ctx->max_stack > kShortStack ? d_deep.p : nullptr, ctx->p_err.p, s));
+    if (value) d_val.download(reinterpret_cast<float4*>(value), npix, s);
+    if (rgba) d_frame.download(reinterpret_cast<uchar4*>(rgba), npix, s);
+    uint32_t err = 0;
+    ctx->p_err.download(&err, 1, s);
+    if (err) throw Error(RTPG_ERR_LIMIT, "traversal stack bound exceeded");
+  });
+}
+
 extern "C" int rtpg_trace_closest(rtpg_context ctx, const float* rays, uint32_t n, float* hits) {
   return trace_common(ctx, rays, n, hits, nullptr, false);
 }

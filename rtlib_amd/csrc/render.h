@@ -78,6 +78,10 @@ hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n
 
 hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, uchar4* frame, hipStream_t s);
 
+hipError_t launch_debug(const RenderParams& rp, const Camera& cam, const SceneView& sc, const HashGridView& hg,
+                        const MortonView& mq, uint32_t type, bool use_grid, float4* val, uchar4* frame,
+                        uint32_t* deep, uint32_t* err, hipStream_t s);
+
 hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
                                const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, const MlpView& ml,
                                int guide, float* accum, hipStream_t s, hipEvent_t* ev = nullptr);

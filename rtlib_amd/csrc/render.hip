@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_rays(SceneView sc, const float
   } else {
     traverse<false>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, st, err, t, u, v, tri);
     uint32_t prim = tri >= 0 ? fbits(sc.tris[tri].v0[3]) : 0xFFFFFFFFu;
-    hits[i] = make_float4(tri >= 0 ? t : INFINITY, u, v, bitsf(prim));
+    reinterpret_cast<uint4*>(hits)[i] = make_uint4(fbits(tri >= 0 ? t : INFINITY), fbits(u), fbits(v), prim);
   }
 }
 
@@ -765,8 +765,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
             vcu = -1.0f;
           }
         }
-        ps.vC[vi] = make_float4(vcu, vcv, bitsf(has_vertex ? vcell : kInvalidCell),
-                                bitsf(vflags | (has_vertex ? kVtxValid : 0u)));
+        // integer store: the all-ones sentinel is a NaN pattern as a float
+        reinterpret_cast<uint4*>(ps.vC)[vi] = make_uint4(fbits(vcu), fbits(vcv), has_vertex ? vcell : kInvalidCell,
+                                                         vflags | (has_vertex ? kVtxValid : 0u));
       } else if (pid < ps.mlp_rec && (!cont || (vflags & kVtxDelta))) {
         // no training vertex here (light, miss, delta or absorbed path)
         size_t st = (size_t)rp.max_depth * ps.mlp_rec;
@@ -883,6 +884,121 @@ __global__ void k_tonemap(const float* accum, uint32_t npix, float inv_spp, ucha
   frame[i] = make_uchar4(c[0], c[1], c[2], 255);
 }
 
+// ------------------------------------------------------------------ debug frames
+// __raygen__debug / __closesthit__debug / __miss__debug (SimpleTrace.cu:104-213,
+// 631-673, 714-734): one camera ray per pixel, jittered by the first two numbers
+// of the stream of sample rp.sample_offset (as k_camera), closest hit in
+// [1e-4, 1e20], and per pixel one quantity (DebugFrameType, SimpleKernel.h:
+// 160-171; 8 = world position has no reference counterpart).  val: the raw
+// quantity and the hit primitive (original index bits, ~0 on a miss); frame:
+// the reference's display mapping (normal (n+1)/2, depth t/(1+t), shininess
+// s/(1+s), index r/(1+r)), linear_to_gamma, x 255.99, clamped.  A miss gives
+// normal (1,0,0), depth 1e20 and zeros.
+__device__ __forceinline__ unsigned char debug_byte(float x) {
+  float g = (x <= 0.00174f) ? (32.0f * x) : powf(x, 1.0f / 2.2f);
+  int v = (int)(g * 255.99f);
+  return (unsigned char)(v < 255 ? (v > 0 ? v : 0) : 255);
+}
+
+__global__ void __launch_bounds__(kBlock) k_debug(RenderParams rp, Camera cam, SceneView sc, HashGridView hg,
+                                                  MortonView mq, uint32_t type, uint32_t use_grid, float4* val,
+                                                  uchar4* frame, uint32_t* deep, uint32_t* err) {
+  __shared__ uint2 lds[kShortStack][kBlock];
+  uint32_t npix = rp.width * rp.height;
+  uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+  if (pix >= npix) return;
+  uint32_t x = pix % rp.width, y = pix / rp.width;
+  Pcg32 rng = path_rng(rp.seed, pix, rp.sample_offset);
+  float jx = rnd(rng), jy = rnd(rng);
+  float dx = 2.0f * ((float)x + jx) / (float)rp.width - 1.0f;
+  float dy = 2.0f * ((float)y + jy) / (float)rp.height - 1.0f;
+  V3 U{cam.u[0], cam.u[1], cam.u[2]}, Vv{cam.v[0], cam.v[1], cam.v[2]}, W{cam.w[0], cam.w[1], cam.w[2]};
+  V3 d = normalize(dx * U + dy * Vv + W);
+  V3 o{cam.eye[0], cam.eye[1], cam.eye[2]};
+  StackRef st = make_stack(lds, deep, npix, pix, sc.max_stack);
+  float t, u, v;
+  int32_t tri;
+  traverse<false>(sc, o, d, 1e-4f, 1e20f, st, err, t, u, v, tri);
+  V3 q{0.0f, 0.0f, 0.0f};
+  uint32_t prim = 0xFFFFFFFFu;
+  if (tri < 0) {
+    if (type == 0) q = V3{1.0f, 0.0f, 0.0f};
+    if (type == 1) q = V3{1e20f, 1e20f, 1e20f};
+  } else {
+    const float4* tq = reinterpret_cast<const float4*>(sc.tris + tri);
+    prim = fbits(tq[0].w);
+    V3 p = o + d * t;
+    float4 sh = sc.shade[tri];
+    uint32_t info = sc.info[tri];
+    uint32_t mt = tri_type(info);
+    switch (type) {
+      case 0:
+        q = normalize(cross(load3(tq[1]), load3(tq[2])));
+        break;
+      case 1:
+        q = V3{t, t, t};
+        break;
+      case 2:
+        if (mt != kMatGlass && !tri_is_light(info)) q = V3{sh.x, sh.y, sh.z};
+        break;
+      case 3:
+        if (mt == kMatGlass) {
+          q = V3{sh.x, sh.y, sh.z};
+        } else if (mt == kMatPhong && sc.spec) {
+          float4 s4 = sc.spec[tri];
+          q = V3{s4.x, s4.y, s4.z};
+        }
+        break;
+      case 4:
+        if (tri_is_light(info)) {
+          float4 e = sc.lights[4u * tri_light(info) + 3u];
+          q = V3{e.x, e.y, e.z};
+        }
+        break;
+      case 5:
+        if (mt == kMatPhong && sc.spec) q = V3{sc.spec[tri].w, sc.spec[tri].w, sc.spec[tri].w};
+        break;
+      case 6:
+        if (mt == kMatGlass) q = V3{sh.w, sh.w, sh.w};
+        break;
+      case 7:
+        if (use_grid) {
+          // root weight of the building direction tree of the hit cell
+          uint32_t ix, iy, iz, bucket, csum;
+          grid_cell(hg, p, ix, iy, iz);
+          grid_hash(hg, ix, iy, iz, bucket, csum);
+          uint32_t cell = grid_find_prv(hg, bucket, csum);
+          if (cell != kInvalidCell) {
+            float w = mq.build[(size_t)mq.per_cell * cell];
+            q = V3{w, w, w};
+          }
+        }
+        break;
+      default:
+        q = p;
+        break;
+    }
+  }
+  // integer store: the miss marker (all ones) is a NaN pattern as a float
+  if (val) reinterpret_cast<uint4*>(val)[pix] = make_uint4(fbits(q.x), fbits(q.y), fbits(q.z), prim);
+  if (frame) {
+    V3 c = q;
+    if (type == 0) c = (q + V3{1.0f, 1.0f, 1.0f}) * 0.5f;
+    if (type == 1 || type == 5 || type == 6) c = V3{q.x / (1.0f + q.x), q.y / (1.0f + q.y), q.z / (1.0f + q.z)};
+    frame[pix] = make_uchar4(debug_byte(c.x), debug_byte(c.y), debug_byte(c.z), 255);
+  }
+}
+
+hipError_t launch_debug(const RenderParams& rp, const Camera& cam, const SceneView& sc, const HashGridView& hg,
+                        const MortonView& mq, uint32_t type, bool use_grid, float4* val, uchar4* frame,
+                        uint32_t* deep, uint32_t* err, hipStream_t s) {
+  uint32_t npix = rp.width * rp.height;
+  if (npix == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_debug, dim3((npix + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, rp, cam, sc, hg, mq, type,
+                     use_grid ? 1u : 0u, val, frame, deep, err);
+  return hipGetLastError();
+}
+
 hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, uchar4* frame, hipStream_t s) {
   if (npix == 0) return hipSuccess;
   hipLaunchKernelGGL(k_tonemap, dim3((npix + 255u) / 256u), dim3(256), 0, s, accum, npix, inv_spp, frame);
@@ -957,7 +1073,8 @@ __global__ void __launch_bounds__(kBlock) k_commit_emit(PathSoA ps, uint32_t max
     // fixed slot per (vertex, path): no counter to contend on; stage 2 reads
     // slot (d, pid) only for d < nhit[pid] - 1, all of which are written here
     if (in && (uint32_t)d < max_vtx) {
-      recs[(size_t)d * ps.n + pid] = make_float4(bitsf(ok ? cell : kInvalidCell), cu, cv, value);
+      reinterpret_cast<uint4*>(recs)[(size_t)d * ps.n + pid] =
+          make_uint4(ok ? cell : kInvalidCell, fbits(cu), fbits(cv), fbits(value));
     }
   }
 }

@@ -152,6 +152,24 @@ class Context:
         p.mlp = mlp.h if mlp is not None else None
         call("rtpg_render", self.h, C.byref(p))
 
+    # DebugFrameType (SimpleKernel.h:160-171), plus 8 = world position
+    DEBUG_NORMAL, DEBUG_DEPTH, DEBUG_DIFFUSE, DEBUG_SPECULAR, DEBUG_EMISSION = 0, 1, 2, 3, 4
+    DEBUG_SHINNESS, DEBUG_REFR_INDEX, DEBUG_GRID_VALUE, DEBUG_POSITION = 5, 6, 7, 8
+
+    def render_debug(self, frame_type, seed=0, sample_offset=0, hashgrid=None, mqtree=None):
+        """Debug frame: (value H x W x 3 float32, primitive H x W uint32, rgba
+        H x W x 4 uint8).  The grid value needs hashgrid and mqtree."""
+        p = _lib.RenderParams()
+        p.seed = seed
+        p.sample_offset = sample_offset
+        if hashgrid is not None and mqtree is not None:
+            p.flags = FLAG_USE_GRID
+            p.hashgrid, p.mqtree = hashgrid.h, mqtree.h
+        val = np.zeros((self.height, self.width, 4), np.float32)
+        rgba = np.zeros((self.height, self.width, 4), np.uint8)
+        call("rtpg_render_debug", self.h, C.byref(p), frame_type, ptr(val), ptr(rgba, C.c_uint8))
+        return val[..., :3].copy(), val[..., 3].view(np.uint32).copy(), rgba
+
     def render_stats(self):
         s = np.zeros(3, np.uint64)
         call("rtpg_render_stats", self.h, ptr(s, C.c_uint64))

@@ -10,7 +10,8 @@ import pytest
 
 from rtlib_amd import scenefile, scenes
 from rtlib_amd.tracer import (FLAG_GUIDE_SAMPLE, FLAG_MIS, FLAG_NEE, FLAG_RIS, FLAG_RR, FLAG_USE_TREE, Context,
-                              RTSTreeController, RTSTreeWrapper, RTSTreeWrapper2, render_guided)
+                              DoubleBufferedHashGrid3Buffer, RTMortonQuadTreeWrapper, RTSTreeController,
+                              RTSTreeWrapper, RTSTreeWrapper2, render_guided)
 
 pytestmark = pytest.mark.gpu
 
@@ -250,3 +251,73 @@ def test_scene_loaded_from_files_renders_like_the_procedural_scene(ctx, orc, tmp
     proc = ctx.image()
     assert abs(loaded.mean() - proc.mean()) / proc.mean() < 0.01
     assert _rel_mse(loaded, proc) < 2.0 * _rel_mse(other, loaded) + 1e-4
+
+
+def _gamma_bytes(c):
+    c = np.asarray(c, np.float64)
+    g = np.where(c <= 0.00174, 32.0 * c, np.power(np.maximum(c, 0.0), 1.0 / 2.2))
+    return np.clip((g * 255.99).astype(np.int64), 0, 255)
+
+
+def test_debug_frames_match_cpu(ctx, orc):
+    # every DebugFrameType against the oracle's debug programs on the materials
+    # Cornell box (Phong, glass, two light meshes); rows of identical primitives
+    sc, lm = scenes.cornell_materials(40, 32)
+    cam = sc["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 40 / 32)
+    ctx.upload_scene(sc, light_mode=1, light_mesh=lm)
+    ctx.resize(40, 32)
+    ctx.set_camera_look(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 40, 32)
+    S = orc.Scene(sc, light_mesh=lm)
+    for ft in (0, 1, 2, 3, 4, 5, 6, 8):
+        val, prim, rgba = ctx.render_debug(ft, seed=5, sample_offset=3)
+        ref, tri = orc.debug_frame(S, uvw, 40, 32, ft, seed=5, sample_offset=3)
+        same = prim.astype(np.int64) == np.where(tri < 0, 0xFFFFFFFF, tri.astype(np.int64))
+        bad = np.argwhere(~same)[:8]
+        assert same.mean() > 0.995, (ft, same.mean(), [(tuple(b), int(prim[tuple(b)]), int(tri[tuple(b)])) for b in bad])
+        assert (tri < 0).sum() > 0 and (tri >= 0).sum() > 0
+        if ft in (0, 8):
+            assert np.allclose(val[same], ref[same], rtol=1e-5, atol=1e-5), ft
+        elif ft == 1:
+            assert np.allclose(val[same], ref[same], rtol=1e-5), ft
+        else:
+            assert np.array_equal(val[same], ref[same]), ft
+        # display bytes: the reference's mapping of the raw values
+        c = val.astype(np.float64)
+        if ft == 0:
+            c = (c + 1.0) / 2.0
+        elif ft in (1, 5, 6):
+            c = c / (1.0 + c)
+        assert np.abs(rgba[..., :3].astype(np.int64) - _gamma_bytes(c)).max() <= 1, ft
+        assert np.all(rgba[..., 3] == 255)
+    # every material kind shows up where expected
+    d, prim, _ = ctx.render_debug(ctx.DEBUG_REFR_INDEX, seed=5)
+    assert np.any(d[..., 0] == np.float32(1.5))
+    e, _, _ = ctx.render_debug(ctx.DEBUG_EMISSION, seed=5)
+    assert np.any(e[..., 0] == 17.0)
+
+
+def test_debug_grid_value_is_the_root_weight_of_the_hit_cell(ctx):
+    sc = scenes.cornell(32, 32)
+    ctx.upload_scene(sc)
+    info = ctx.scene_info()
+    lo, hi = info["aabb_min"] - 0.005, info["aabb_max"] + 0.005
+    grid = DoubleBufferedHashGrid3Buffer(ctx, lo, hi, bounds=(16, 16, 16), slots=4096)
+    mq = RTMortonQuadTreeWrapper(ctx, 4096, 2, 0.3)
+    mq.Allocate()
+    pos, prim, _ = ctx.render_debug(ctx.DEBUG_POSITION, seed=1)
+    hit = prim != 0xFFFFFFFF
+    grid.Find(pos[hit], insert=True)
+    grid.Update()
+    w = np.zeros((4096, mq.per_cell), np.float32)
+    w[:, 0] = np.arange(4096, dtype=np.float32) + 1.0
+    mq.Upload(w, which=0)
+    val, prim2, _ = ctx.render_debug(ctx.DEBUG_GRID_VALUE, seed=1, hashgrid=grid, mqtree=mq)
+    assert np.array_equal(prim2, prim)
+    slots = grid.Find(pos[hit], insert=False)
+    want = np.where(slots == 0xFFFFFFFF, 0.0, slots.astype(np.float32) + 1.0)
+    assert np.all(slots != 0xFFFFFFFF)
+    assert np.array_equal(val[hit][:, 0], want) and np.all(val[~hit] == 0)
+    # without the grid the value is 0
+    v0, _, _ = ctx.render_debug(ctx.DEBUG_GRID_VALUE, seed=1)
+    assert np.all(v0 == 0)

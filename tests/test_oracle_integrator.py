@@ -81,3 +81,32 @@ def test_oracle_adjoint_matches_finite_differences(orc):
     want = loss(s2) - loss(sc)
     got = float((gl * le[None, :]).sum())
     assert abs(want - got) <= 1e-3 * abs(want)
+
+
+def test_debug_frame_closed_forms(orc):
+    # a 4 x 4 Phong quad at z = -2 facing +z, camera at the origin looking down -z:
+    # per pixel t = 2 / -d_z, normal +z, the material's fields; rays past the
+    # quad's edge miss (normal (1,0,0), distance 1e20)
+    b = scenes._Builder()
+    b.quad((-2, -2, -2), (2, -2, -2), (2, 2, -2), (-2, 2, -2), (0.2, 0.3, 0.4), mat=scenes.MAT_PHONG)
+    cam = {"eye": (0, 0, 0), "lookat": (0, 0, -1), "up": (0, 1, 0), "fovy": 100.0, "width": 24, "height": 16}
+    nt = 2
+    sc = b.finish(cam, spec=np.tile(np.array([[0.5, 0.6, 0.7, 30.0]], np.float32), (nt, 1)))
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 24 / 16)
+    S = orc.Scene(sc)
+    pos, tri = orc.debug_frame(S, uvw, 24, 16, 8, seed=3)
+    hit = tri >= 0
+    assert 0 < hit.sum() < hit.size  # some rays pass the quad's edge
+    d = pos[hit] / np.linalg.norm(pos[hit], axis=1, keepdims=True)
+    t, _ = orc.debug_frame(S, uvw, 24, 16, 1, seed=3)
+    assert np.allclose(t[hit][:, 0], 2.0 / -d[:, 2], rtol=1e-5)
+    assert np.allclose(pos[hit][:, 2], -2.0, atol=1e-5)
+    assert np.all(t[~hit] == 1e20)
+    nrm, _ = orc.debug_frame(S, uvw, 24, 16, 0, seed=3)
+    assert np.allclose(nrm[hit], (0, 0, 1), atol=1e-7) and np.all(nrm[~hit] == (1, 0, 0))
+    for ft, want in ((2, (0.2, 0.3, 0.4)), (3, (0.5, 0.6, 0.7)), (4, (0, 0, 0)), (5, (30, 30, 30)), (6, (0, 0, 0))):
+        v, _ = orc.debug_frame(S, uvw, 24, 16, ft, seed=3)
+        assert np.all(v[hit] == np.float32(want)) and np.all(v[~hit] == 0), ft
+    # the jitter is the first two numbers of the pixel's stream: another seed moves the rays
+    pos2, _ = orc.debug_frame(S, uvw, 24, 16, 8, seed=4)
+    assert not np.array_equal(pos2, pos)

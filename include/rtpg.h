@@ -331,6 +331,7 @@ typedef struct rtpg_mlp_config {
   uint32_t capacity;     /* recorded samples kept between trainings (2^20) */
   uint32_t record_paths; /* paths per render chunk that record samples (2^16) */
   uint64_t seed;         /* initialisation */
+  uint32_t guide_depth;  /* vertices guided and recorded: depth < guide_depth (0 = all) */
 } rtpg_mlp_config;
 /* cfg = NULL takes the defaults above.  The position box is made cubic. */
 int rtpg_mlp_create(rtpg_context ctx, const float aabb_min[3], const float aabb_max[3], const rtpg_mlp_config* cfg,

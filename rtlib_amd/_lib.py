@@ -50,6 +50,7 @@ class MlpConfig(C.Structure):
         ("capacity", C.c_uint32),
         ("record_paths", C.c_uint32),
         ("seed", C.c_uint64),
+        ("guide_depth", C.c_uint32),
     ]
 
 

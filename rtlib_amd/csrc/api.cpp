@@ -373,6 +373,7 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
     ps.mixc = ctx->p_mixc.p;
     ps.mv = ctx->p_mv.p;
     ps.mlp_rec = mlp_rec;
+    ps.mlp_guide_depth = mlp->cfg.guide_depth ? mlp->cfg.guide_depth : 0xFFFFFFFFu;
     ps.mlp_out = mlp->samples.p;
     ps.mlp_count = mlp->count.p;
     ps.mlp_cap = mlp->cfg.capacity;

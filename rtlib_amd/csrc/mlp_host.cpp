@@ -48,6 +48,7 @@ rtpg_mlp_config default_config() {
   c.capacity = 1u << 20;
   c.record_paths = 1u << 16;
   c.seed = 0;
+  c.guide_depth = 0;
   return c;
 }
 

@@ -65,6 +65,7 @@ struct PathSoA {
   float* mixc;
   float4* mv;
   uint32_t mlp_rec;
+  uint32_t mlp_guide_depth;  // vertices at depth < this are guided and recorded
   float4* mlp_out;
   uint32_t* mlp_count;
   uint32_t mlp_cap;

@@ -590,7 +590,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneVie
           // learned guide: the mixture the inference pass wrote for this queue slot
           const float4* mx = nullptr;
           float csel = 0.0f;
-          if (kGuide == 3 && (rp.flags & kFlagGuideSample)) {
+          if (kGuide == 3 && (rp.flags & kFlagGuideSample) && depth < ps.mlp_guide_depth) {
             mx = ps.mix + (size_t)i * kMlpMixF4;
             csel = ps.mixc[i];
           }
@@ -1241,7 +1241,7 @@ __global__ void __launch_bounds__(kBlock) k_mlp_emit(PathSoA ps, uint32_t max_de
     float w = 0.0f;
     size_t mi = (size_t)d * ps.mlp_rec + pid;
     float4 mc{0, 0, 0, 0};
-    if ((uint32_t)d < nv) {
+    if ((uint32_t)d < nv && (uint32_t)d < ps.mlp_guide_depth) {
       mc = ps.mv[2 * st + mi];
       if (fbits(mc.w) & kVtxValid) {
         float4 va = ps.vA[(size_t)d * ps.n + pid];
@@ -1352,7 +1352,7 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
       RTPG_SHADE(2, false)
     } else if (guide == 3) {
       // the learned guide's inference pass is timed with the shade class
-      if (rp.flags & kFlagGuideSample) {
+      if ((rp.flags & kFlagGuideSample) && d < ps.mlp_guide_depth) {
         uint32_t ni = std::min(blocks_for(ps.n, 64u), kInferGrid);
         hipLaunchKernelGGL(k_mlp_infer, dim3(ni), dim3(kInferBlock), 0, s, sc, ps, ml, d);
       }

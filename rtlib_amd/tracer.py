@@ -571,9 +571,10 @@ class MLPGuide:
     pdf_bsdf (n)."""
 
     def __init__(self, ctx: Context, aabb_min, aabb_max, log2_table=19, base_res=16.0, max_res=2048.0, lr=1e-2,
-                 beta1=0.9, beta2=0.99, eps=1e-15, batch=1 << 18, capacity=1 << 20, record_paths=1 << 16, seed=0):
+                 beta1=0.9, beta2=0.99, eps=1e-15, batch=1 << 18, capacity=1 << 20, record_paths=1 << 16, seed=0,
+                 guide_depth=0):
         cfg = _lib.MlpConfig(log2_table, base_res, max_res, lr, beta1, beta2, eps, batch, capacity, record_paths,
-                             seed)
+                             seed, guide_depth)
         h = C.c_void_p()
         call("rtpg_mlp_create", ctx.h, ptr(_f32(aabb_min)), ptr(_f32(aabb_max)), C.byref(cfg), C.byref(h))
         self.h = h.value

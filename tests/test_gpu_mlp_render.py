@@ -66,3 +66,21 @@ def test_mlp_recording_without_sampling_is_plain_path_tracing(ctx):
     p = ctx.image()
     assert abs(g.mean() - p.mean()) / p.mean() < 0.02
     mlp.close()
+
+
+def test_mlp_guide_depth_limits_guided_bounces(ctx):
+    # guide_depth = 1: only camera-ray hits are guided (and recorded); deeper
+    # vertices sample the BSDF.  The estimator stays unbiased.
+    sc = scenes.cornell(32, 32)
+    ctx.upload_scene(sc)
+    info = ctx.scene_info()
+    ctx.clear()
+    ctx.render(spp=512, flags=BASE, max_depth=8, seed=3)
+    plain = ctx.image()
+    mlp = MLPGuide(ctx, info["aabb_min"], info["aabb_max"], seed=0, guide_depth=1)
+    ctx.clear()
+    n = render_mlp_guided(ctx, mlp, 512, BASE, max_depth=8, seed=4, train_passes=4, steps_per_pass=4)
+    g = ctx.image()
+    assert n == 512 and mlp.info()["steps"] == 16
+    assert abs(g.mean() - plain.mean()) / plain.mean() < 0.02
+    mlp.close()

@@ -125,7 +125,7 @@ typedef struct rtpg_render_params {
   rtpg_sdtree sdtree;      /* guide for RTPG_FLAG_USE_TREE, else NULL */
   rtpg_hashgrid hashgrid;  /* spatial hash for RTPG_FLAG_USE_GRID */
   rtpg_mqtree mqtree;      /* direction quadtrees for RTPG_FLAG_USE_GRID */
-  uint32_t max_paths;      /* paths in flight per chunk (0 = default 4M) */
+  uint32_t max_paths;      /* paths in flight per chunk (0 = default 32M; at most 64M) */
   rtpg_mlp mlp;            /* learned guide for RTPG_FLAG_USE_MLP, else NULL */
 } rtpg_render_params;
 int rtpg_render(rtpg_context ctx, const rtpg_render_params* params);

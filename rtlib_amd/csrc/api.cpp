@@ -506,7 +506,9 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     require(guide == 0 || (prm->flags & RTPG_FLAG_FINAL) || prm->max_depth <= 16,
             "recording guide vertices supports max_depth <= 16");
     uint32_t npix = ctx->width * ctx->height;
-    uint32_t max_paths = prm->max_paths ? prm->max_paths : (1u << 22);
+    // paths per chunk: 32M by default (about 30 GB of path state when guiding);
+    // at most 64M so that per-chunk record indices stay within 32 bits
+    uint32_t max_paths = std::min(prm->max_paths ? prm->max_paths : (1u << 25), 1u << 26);
     uint32_t chunk_spp = std::max(1u, max_paths / npix);
     chunk_spp = std::min(chunk_spp, std::max(prm->spp, 1u));
     SceneView sc = ctx->scene_view();

@@ -87,13 +87,13 @@ __device__ __forceinline__ void stack_push(const StackRef& s, uint32_t& sp, uint
   if (sp < kShortStack) {
     s.lds[sp * kBlock] = v;
   } else {
-    s.deep[(sp - kShortStack) * s.deep_stride] = v;
+    s.deep[(size_t)(sp - kShortStack) * s.deep_stride] = v;
   }
   ++sp;
 }
 __device__ __forceinline__ uint2 stack_pop(const StackRef& s, uint32_t& sp) {
   --sp;
-  return sp < kShortStack ? s.lds[sp * kBlock] : s.deep[(sp - kShortStack) * s.deep_stride];
+  return sp < kShortStack ? s.lds[sp * kBlock] : s.deep[(size_t)(sp - kShortStack) * s.deep_stride];
 }
 
 #define RTPG_CS(a, b)                         \

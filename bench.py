@@ -172,6 +172,12 @@ def main(argv=None):
     plain_rate = a.budget * npix / dt_plain / 1e6
     st_plain = ctx.render_stats()
     plain_budget_img = ctx.image()
+    # kernel classes of the same plain render, timed in a separate run
+    ctx.set_profile(time_kernels=True, count_work=False)
+    ctx.clear()
+    ctx.render(spp=a.budget, flags=base, max_depth=depth, seed=77)
+    plain_kernel_ms = ctx.kernel_stats()[0]
+    ctx.set_profile(False, False)
     quality = None
     if a.ref_spp > 0:
         # learned guide on the same budget: a.mlp_passes recording passes of 1
@@ -268,6 +274,7 @@ def main(argv=None):
                 "avg_launch_ms": round(float(per_kernel[0]) / max(n_ext, 1), 4),
                 "dominant_by_time": names[dom],
                 "kernel_ms_per_step": {n: round(float(x), 2) for n, x in zip(names, per_kernel)},
+                "plain_kernel_ms_per_step": {n: round(float(plain_kernel_ms[n]), 2) for n in names},
                 "kernel_launches_per_step": {n: int(x) for n, x in zip(names, launches)}}
 
     # ---- the other configurations, measured on the same context

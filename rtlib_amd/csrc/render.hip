@@ -438,9 +438,15 @@ __device__ __forceinline__ void block_append2(uint32_t* ca, bool pa, uint32_t* c
 
 // kRis instantiates the RIS light selection; without it the kernel needs
 // fewer registers and runs at a higher occupancy.
+// SD-tree shading is held to 80 VGPRs (6 waves per SIMD, as plain shading).
+template <int kGuide, bool kAdj, bool kRis>
+constexpr unsigned shade_min_waves() {
+  return (kGuide == 1 && !kAdj && !kRis) ? 6u : 1u;
+}
+
 template <int kGuide, bool kAdj = false, bool kRis = false>
-__global__ void __launch_bounds__(kShadeBlock) k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd,
-                                                       HashGridView hg, MortonView mq, uint32_t depth) {
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(shade_min_waves<kGuide, kAdj, kRis>())))
+k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView hg, MortonView mq, uint32_t depth) {
   __shared__ LdsAcc3<kAdj ? 1024u : 1u> acc_alb;
   __shared__ LdsAcc3<kAdj ? 256u : 1u> acc_le;
   if (kAdj) {

@@ -77,6 +77,40 @@ __global__ void k_dtree_interleave(DTreeArrays a) {
   a.s_node[2 * i + 1] = make_uint4(c.x, c.y, 0u, 0u);
 }
 
+// Pdf jump table of the sampling trees: work-item (DTree j, cell c) runs the
+// first kPdfJumpBits levels of dtree_pdf's descent for the directions of cell
+// c (child index bits = the cell's coordinate bits, most significant first),
+// multiplying the density factors in the same order, and records where the
+// descent continues (kJumpDone when it ended within those levels).
+__global__ void k_dtree_pdf_jump(DTreeArrays a) {
+#pragma clang fp contract(off)
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n * kPdfJumpCells) return;
+  uint32_t j = t / kPdfJumpCells, c = t % kPdfJumpCells;
+  uint32_t cx = c % (1u << kPdfJumpBits), cy = c / (1u << kPdfJumpBits);
+  uint32_t base = a.s_base[j];
+  float result = 1.0f;
+  uint32_t node = 0, cont = kJumpDone;
+  for (uint32_t l = 0; l < kPdfJumpBits; ++l) {
+    uint32_t b = kPdfJumpBits - 1u - l;
+    int idx = (int)(((cx >> b) & 1u) | (((cy >> b) & 1u) << 1));
+    uint4 r0 = a.s_node[2u * (base + node)], r1 = a.s_node[2u * (base + node) + 1u];
+    float4 s = make_float4(bitsf(r0.x), bitsf(r0.y), bitsf(r0.z), bitsf(r0.w));
+    float total = s.x + s.y + s.z + s.w;
+    float si = f4c(s, idx);
+    if (si <= 0.0f) {
+      result = 0.0f;
+      break;
+    }
+    result *= 4.0f * si / total;
+    uint32_t ch = dnode_child(make_uint2(r1.x, r1.y), idx);
+    if (ch == 0) break;
+    node = ch;
+    if (l + 1u == kPdfJumpBits) cont = node;
+  }
+  a.s_jump[t] = make_uint2(fbits(result), cont);
+}
+
 // RTDTreeT::Reset (OPX7UtilsPathGuiding.h:888-947), one work-item per DTree.
 // The DFS keeps the reference's LIFO order, so nodes get the reference's
 // numbers.  A stack entry either copies a node of the source sampling tree
@@ -299,6 +333,8 @@ hipError_t launch_dtree_publish(DTreeArrays a, hipStream_t s) {
   hipLaunchKernelGGL(k_dtree_publish, dim3(nblk(a.n, 256)), dim3(256), 0, s, a);
   if (a.s_nodes && a.s_node)
     hipLaunchKernelGGL(k_dtree_interleave, dim3(nblk(a.s_nodes, 256)), dim3(256), 0, s, a);
+  if (a.s_node && a.s_jump)
+    hipLaunchKernelGGL(k_dtree_pdf_jump, dim3(nblk(a.n * kPdfJumpCells, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_dtree_reset(DTreeArrays src, DTreeArrays dst, const uint32_t* map, uint32_t max_depth,

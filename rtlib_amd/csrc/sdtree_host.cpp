@@ -73,24 +73,27 @@ DTreeArrays rtpg_sdtree_t::Owned::view() {
   a.s_sums = s_sums.p;
   a.s_child = s_child.p;
   a.s_node = s_node.p;
+  a.s_jump = s_jump.p;
   return a;
 }
 
+// Per-DTree arrays of at least n entries (buffers only grow, so the ping-pong
+// sets of reset() stop allocating once the tree has reached its size).
 static void alloc_per_dtree(rtpg_sdtree_t::Owned& o, uint32_t n) {
   o.n = n;
-  o.b_base.alloc(n);
-  o.b_cnt.alloc(n);
-  o.s_base.alloc(n);
-  o.s_cnt.alloc(n);
-  o.s_depth.alloc(n);
-  o.b_depth.alloc(n);
-  o.b_sw.alloc(n);
-  o.b_sum.alloc(n);
-  o.b_area.alloc(n);
-  o.s_sw.alloc(n);
-  o.s_sum.alloc(n);
-  o.s_area.alloc(n);
-  o.info.alloc(n);
+  o.b_base.reserve(n);
+  o.b_cnt.reserve(n);
+  o.s_base.reserve(n);
+  o.s_cnt.reserve(n);
+  o.s_depth.reserve(n);
+  o.b_depth.reserve(n);
+  o.b_sw.reserve(n);
+  o.b_sum.reserve(n);
+  o.b_area.reserve(n);
+  o.s_sw.reserve(n);
+  o.s_sum.reserve(n);
+  o.s_area.reserve(n);
+  o.info.reserve(n);
 }
 
 static std::vector<uint32_t> encode_stree(const std::vector<rtpg_sdtree_t::SNode>& sn) {
@@ -157,6 +160,7 @@ SdTreeView rtpg_sdtree_t::view() {
   v.samp_sums = dt.s_sums.p;
   v.samp_child = dt.s_child.p;
   v.samp_node = dt.s_node.p;
+  v.samp_jump = dt.s_jump.p;
   v.fraction = fraction;
   v.n_dtrees = dt.n;
   v.filter = filter;
@@ -170,6 +174,7 @@ SdTreeView sdtree_view_of(rtpg_sdtree_t* t) { return t->view(); }
 
 void rtpg_sdtree_t::publish() {
   dt.s_node.reserve(2 * (size_t)std::max(dt.s_nodes, 1u));
+  dt.s_jump.reserve((size_t)kPdfJumpCells * std::max(dt.n, 1u));
   RTPG_HIP(launch_dtree_publish(dt.view(), ctx->stream));
 }
 
@@ -206,12 +211,11 @@ void rtpg_sdtree_t::clear() {
 // sampling = building.
 void rtpg_sdtree_t::build() {
   hipStream_t s = ctx->stream;
-  DBuf<float> area;
-  area.alloc(dt.b_nodes);
-  RTPG_HIP(launch_dtree_build(dt.view(), area.p, s));
+  area_tmp.reserve(dt.b_nodes);
+  RTPG_HIP(launch_dtree_build(dt.view(), area_tmp.p, s));
   uint32_t n = dt.n;
-  dt.s_sums.alloc(dt.b_nodes);
-  dt.s_child.alloc(dt.b_nodes);
+  dt.s_sums.reserve(dt.b_nodes);
+  dt.s_child.reserve(dt.b_nodes);
   dt.s_nodes = dt.b_nodes;
   RTPG_HIP(hipMemcpyAsync(dt.s_sums.p, dt.b_sums.p, sizeof(float4) * dt.b_nodes, hipMemcpyDeviceToDevice, s));
   RTPG_HIP(hipMemcpyAsync(dt.s_child.p, dt.b_child.p, sizeof(uint2) * dt.b_nodes, hipMemcpyDeviceToDevice, s));
@@ -363,12 +367,11 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
     }
   }
   uint32_t n2 = (uint32_t)map.size();
-  // Device: new DTree arrays.
-  Owned nd;
+  // Device: new DTree arrays in the spare set (swapped in at the end).
+  Owned& nd = spare;
   alloc_per_dtree(nd, n2);
-  DBuf<uint32_t> d_map, d_total;
   d_map.upload(map, s);
-  d_total.alloc(1);
+  d_total.reserve(1);
   std::vector<uint32_t> s_cnt2(n2);
   for (uint32_t j = 0; j < n2; ++j) s_cnt2[j] = scnt[map[j]];
   uint32_t s_total = 0;
@@ -376,8 +379,8 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
   nd.s_cnt.upload(s_cnt2, s);
   nd.s_base.upload(s_base2, s);
   nd.s_nodes = s_total;
-  nd.s_sums.alloc(s_total);
-  nd.s_child.alloc(s_total);
+  nd.s_sums.reserve(s_total);
+  nd.s_child.reserve(s_total);
   d_err.reserve(1);
   d_err.zero(1, s);
   DTreeArrays src = dt.view();
@@ -388,8 +391,8 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
   uint32_t b_total = 0;
   d_total.download(&b_total, 1, s);
   nd.b_nodes = b_total;
-  nd.b_sums.alloc(b_total);
-  nd.b_child.alloc(b_total);
+  nd.b_sums.reserve(b_total);
+  nd.b_child.reserve(b_total);
   dstv = nd.view();
   RTPG_HIP(launch_dtree_reset(src, dstv, d_map.p, max_dtree_depth, subdiv_th, 1, d_err.p, s));
   uint32_t err = 0;
@@ -397,7 +400,7 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
   if (err) throw Error(RTPG_ERR_LIMIT, "DTree reset stack bound exceeded");
   snodes.swap(ns);
   upload_stree();
-  dt = std::move(nd);
+  std::swap(dt, spare);
   publish();
   RTPG_HIP(hipStreamSynchronize(s));
 }

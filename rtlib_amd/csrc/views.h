@@ -67,6 +67,10 @@ struct SdTreeView {
   const uint2* samp_child;
   // sampling nodes interleaved for the tracer: per node (sums), (children, 0, 0)
   const uint4* samp_node;
+  // pdf jump table: per DTree and cell of the 2^kPdfJumpBits square grid of
+  // canonical directions, (bits of the density product of the first
+  // kPdfJumpBits levels, node to continue from or kJumpDone)
+  const uint2* samp_jump;
   float fraction;
   uint32_t n_dtrees;
   // recording filters: bit 0 spatial box, bit 1 directional box; per DTree the
@@ -74,6 +78,10 @@ struct SdTreeView {
   uint32_t filter;
   const uint32_t* leaf_depth;
 };
+
+constexpr uint32_t kPdfJumpBits = 4;
+constexpr uint32_t kPdfJumpCells = 1u << (2 * kPdfJumpBits);
+constexpr uint32_t kJumpDone = 0xFFFFFFFFu;
 
 enum RecordFilter : uint32_t {
   kFilterSpatialBox = 1u,      // reference SpatialFilterBox

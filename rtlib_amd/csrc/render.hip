@@ -74,6 +74,7 @@ __device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tm
 // Traversal stack entries: (node or leaf, entry distance of its box).  An
 // inner node is its index; a leaf is kLeafEntry | (count - 1) << 27 | first.
 constexpr uint32_t kLeafEntry = 0x80000000u;
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // never a node (< 2^31) nor a leaf (first < 2^27)
 
 struct StackRef {
   uint2* lds;        // this lane's column of the LDS short stack
@@ -124,11 +125,28 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
   int32_t best = -1;
   float bu = 0.0f, bv = 0.0f;
   uint32_t sp = 0;
+  // cur: the next entry (inner node, leaf, or kNoEntry) and its entry
+  // distance; pleaf: a leaf met in the inner phase and postponed
   uint32_t cur = 0;  // the root
-  bool live = true;
-  while (live) {
-    // ---- inner phase
-    while (!(cur & kLeafEntry)) {
+  float cur_t = 0.0f;
+  uint32_t pleaf = kNoEntry;
+  // next stacked entry whose box can still hold a closer hit
+  auto pop_valid = [&]() {
+    cur = kNoEntry;
+    while (sp > 0) {
+      uint2 e = stack_pop(st, sp);
+      if (bitsf(e.y) <= tbest * 1.00000024f) {
+        cur = e.x;
+        cur_t = bitsf(e.y);
+        break;
+      }
+    }
+  };
+  for (;;) {
+    // ---- inner phase (speculative): a lane that meets a leaf keeps it aside
+    // and goes on visiting inner nodes until every lane of the wave holds a
+    // leaf or has nothing left, so that the leaf phase runs with full waves
+    while (cur != kNoEntry && !(cur & kLeafEntry)) {
       if (kCount) ++wcnt[0];
       const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
       float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
@@ -154,59 +172,55 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
         n_hit += hit ? 1 : 0;
       }
       if (n_hit == 0) {
-        // pop the next entry that can still hold a closer hit
-        bool found = false;
-        while (sp > 0) {
-          uint2 e = stack_pop(st, sp);
-          if (bitsf(e.y) <= tbest * 1.00000024f) {
-            cur = e.x;
-            found = true;
-            break;
-          }
-        }
-        if (!found) {
-          live = false;
-          break;
-        }
-        continue;
-      }
-      RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
-      cur = nk[0];
-      if (sp + (uint32_t)(n_hit - 1) > st.cap) {
-        atomicOr(err, 1u);  // stack bound of the build violated: stop this ray
-        live = false;
-        break;
-      }
-      for (int k = n_hit - 1; k >= 1; --k) stack_push(st, sp, nk[k], tk[k]);
-    }
-    if (!live) break;
-    // ---- leaf phase
-    {
-      uint32_t first = cur & 0x07FFFFFFu, cnt = ((cur >> 27) & 0xFu) + 1u;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        if (kCount) ++wcnt[1];
-        if (isect_tri(sc.tris + first + k, o, d, tmin, tbest, bu, bv)) {
-          best = (int32_t)(first + k);
-          if (kAny) {
-            t_out = tbest;
-            u_out = bu;
-            v_out = bv;
-            tri_out = best;
-            return true;
-          }
+        pop_valid();
+      } else {
+        RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
+        if (sp + (uint32_t)(n_hit - 1) > st.cap) {
+          atomicOr(err, 1u);  // stack bound of the build violated: stop this ray
+          cur = kNoEntry;
+          pleaf = kNoEntry;
+          sp = 0;
+        } else {
+          cur = nk[0];
+          cur_t = tk[0];
+          for (int k = n_hit - 1; k >= 1; --k) stack_push(st, sp, nk[k], tk[k]);
         }
       }
+      if (cur != kNoEntry && (cur & kLeafEntry) && pleaf == kNoEntry) {
+        pleaf = cur;
+        pop_valid();
+      }
+      if (__all(pleaf != kNoEntry || cur == kNoEntry || (cur & kLeafEntry))) break;
     }
-    bool found = false;
-    while (sp > 0) {
-      uint2 e = stack_pop(st, sp);
-      if (bitsf(e.y) <= tbest * 1.00000024f) {
-        cur = e.x;
-        found = true;
-        break;
+    // ---- leaf phase: the postponed leaf, else the current entry if a leaf
+    uint32_t leaf;
+    if (pleaf != kNoEntry) {
+      leaf = pleaf;
+      pleaf = kNoEntry;
+    } else if (cur != kNoEntry && (cur & kLeafEntry)) {
+      leaf = cur;
+      cur = kNoEntry;
+    } else if (cur == kNoEntry) {
+      break;
+    } else {
+      continue;  // an inner node left by the wave's early exit
+    }
+    uint32_t first = leaf & 0x07FFFFFFu, cnt = ((leaf >> 27) & 0xFu) + 1u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      if (kCount) ++wcnt[1];
+      if (isect_tri(sc.tris + first + k, o, d, tmin, tbest, bu, bv)) {
+        best = (int32_t)(first + k);
+        if (kAny) {
+          t_out = tbest;
+          u_out = bu;
+          v_out = bv;
+          tri_out = best;
+          return true;
+        }
       }
     }
-    live = found;
+    // the next entry, unless it is still pending and can hold a closer hit
+    if (cur == kNoEntry || cur_t > tbest * 1.00000024f) pop_valid();
   }
   t_out = tbest;
   u_out = bu;

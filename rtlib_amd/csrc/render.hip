@@ -1155,15 +1155,28 @@ __global__ void __launch_bounds__(kApplyBlock) k_commit_apply(const float4* recs
       // cu < 0: flux dropped by the directional box filter
       if (!(isfinite(value) && value > 0.0f) || cu < 0.0f) continue;
       uint32_t base = fbits(sd.info[cell].w);
-      uint32_t node = 0;
-      int idx = dtree_child_idx(cu, cv);
-      for (;;) {
-        uint32_t ch = dnode_child(sd.build_child[base + node], idx);
-        if (ch == 0) break;
-        node = ch;
-        idx = dtree_child_idx(cu, cv);
+      uint32_t key;
+      // the first levels from the record jump table (cell = top bits of u, v;
+      // the remaining coordinates are exact, as in dtree_pdf)
+      const float sn = (float)(1u << kPdfJumpBits);
+      float gx = cu * sn, gy = cv * sn;
+      float jx = fminf(fmaxf(floorf(gx), 0.0f), sn - 1.0f), jy = fminf(fmaxf(floorf(gy), 0.0f), sn - 1.0f);
+      uint32_t e = sd.build_jump[(size_t)cell * kPdfJumpCells + ((uint32_t)jy << kPdfJumpBits) + (uint32_t)jx];
+      if (e & 0x80000000u) {
+        key = base * 4u + (e & 0x7FFFFFFFu);
+      } else {
+        cu = gx - jx;
+        cv = gy - jy;
+        uint32_t node = e;
+        int idx = dtree_child_idx(cu, cv);
+        for (;;) {
+          uint32_t ch = dnode_child(sd.build_child[base + node], idx);
+          if (ch == 0) break;
+          node = ch;
+          idx = dtree_child_idx(cu, cv);
+        }
+        key = (base + node) * 4u + (uint32_t)idx;
       }
-      uint32_t key = (base + node) * 4u + (uint32_t)idx;
       if (!lds_table_add<float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
     } else {
       uint32_t nn = 1u << mq.level;

@@ -111,6 +111,30 @@ __global__ void k_dtree_pdf_jump(DTreeArrays a) {
   a.s_jump[t] = make_uint2(fbits(result), cont);
 }
 
+// Record jump table of the building trees: work-item (DTree j, cell c) runs
+// the first kPdfJumpBits levels of the record descent (DTreeNodeT::Record
+// :113-125) for the directions of cell c.
+__global__ void k_dtree_record_jump(DTreeArrays a) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n * kPdfJumpCells) return;
+  uint32_t j = t / kPdfJumpCells, c = t % kPdfJumpCells;
+  uint32_t cx = c % (1u << kPdfJumpBits), cy = c / (1u << kPdfJumpBits);
+  uint32_t base = a.b_base[j];
+  uint32_t node = 0, e = 0;
+  for (uint32_t l = 0; l < kPdfJumpBits; ++l) {
+    uint32_t b = kPdfJumpBits - 1u - l;
+    int idx = (int)(((cx >> b) & 1u) | (((cy >> b) & 1u) << 1));
+    uint32_t ch = dnode_child(a.b_child[base + node], idx);
+    if (ch == 0) {
+      e = 0x80000000u | (node << 2) | (uint32_t)idx;
+      break;
+    }
+    node = ch;
+    e = node;
+  }
+  a.b_jump[t] = e;
+}
+
 // RTDTreeT::Reset (OPX7UtilsPathGuiding.h:888-947), one work-item per DTree.
 // The DFS keeps the reference's LIFO order, so nodes get the reference's
 // numbers.  A stack entry either copies a node of the source sampling tree
@@ -335,6 +359,8 @@ hipError_t launch_dtree_publish(DTreeArrays a, hipStream_t s) {
     hipLaunchKernelGGL(k_dtree_interleave, dim3(nblk(a.s_nodes, 256)), dim3(256), 0, s, a);
   if (a.s_node && a.s_jump)
     hipLaunchKernelGGL(k_dtree_pdf_jump, dim3(nblk(a.n * kPdfJumpCells, 256)), dim3(256), 0, s, a);
+  if (a.b_jump)
+    hipLaunchKernelGGL(k_dtree_record_jump, dim3(nblk(a.n * kPdfJumpCells, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_dtree_reset(DTreeArrays src, DTreeArrays dst, const uint32_t* map, uint32_t max_depth,

@@ -74,6 +74,7 @@ DTreeArrays rtpg_sdtree_t::Owned::view() {
   a.s_child = s_child.p;
   a.s_node = s_node.p;
   a.s_jump = s_jump.p;
+  a.b_jump = b_jump.p;
   return a;
 }
 
@@ -161,6 +162,7 @@ SdTreeView rtpg_sdtree_t::view() {
   v.samp_child = dt.s_child.p;
   v.samp_node = dt.s_node.p;
   v.samp_jump = dt.s_jump.p;
+  v.build_jump = dt.b_jump.p;
   v.fraction = fraction;
   v.n_dtrees = dt.n;
   v.filter = filter;
@@ -175,6 +177,7 @@ SdTreeView sdtree_view_of(rtpg_sdtree_t* t) { return t->view(); }
 void rtpg_sdtree_t::publish() {
   dt.s_node.reserve(2 * (size_t)std::max(dt.s_nodes, 1u));
   dt.s_jump.reserve((size_t)kPdfJumpCells * std::max(dt.n, 1u));
+  dt.b_jump.reserve((size_t)kPdfJumpCells * std::max(dt.n, 1u));
   RTPG_HIP(launch_dtree_publish(dt.view(), ctx->stream));
 }
 

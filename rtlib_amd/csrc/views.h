@@ -71,6 +71,10 @@ struct SdTreeView {
   // canonical directions, (bits of the density product of the first
   // kPdfJumpBits levels, node to continue from or kJumpDone)
   const uint2* samp_jump;
+  // record jump table of the building trees, per DTree and cell: bit 31 set:
+  // the descent ends within kPdfJumpBits levels, at node << 2 | slot (low
+  // bits); else the node it continues from
+  const uint32_t* build_jump;
   float fraction;
   uint32_t n_dtrees;
   // recording filters: bit 0 spatial box, bit 1 directional box; per DTree the

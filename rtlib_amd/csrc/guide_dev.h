@@ -149,25 +149,28 @@ __device__ __forceinline__ bool dtree_box_jitter(const SdTreeView& g, uint32_t c
 // density on the sphere of the sampling tree of DTree `cell`.
 __device__ __forceinline__ float dtree_pdf(const SdTreeView& g, uint32_t cell, float px, float py) {
 #pragma clang fp contract(off)
-  float4 inf = g.info[cell];
-  if (dtree_mean(inf.y, inf.z) <= 0.0f) return 1.0f * kInv2Pi / 2.0f;
-  uint32_t base = fbits(inf.x);
   float result = 1.0f;
-  uint32_t node = 0;
+  uint32_t node = 0, base;
   if (g.samp_jump) {
-    // the first kPdfJumpBits levels from the jump table: their child indices
-    // are the top bits of (u, v), and kPdfJumpBits doublings of u minus those
-    // bits leave exactly u 2^kPdfJumpBits - cell (as in stree_lookup)
+    // the first kPdfJumpBits levels (and the mean-0 case) from the jump
+    // table: their child indices are the top bits of (u, v), and
+    // kPdfJumpBits doublings of u minus those bits leave exactly
+    // u 2^kPdfJumpBits - cell (as in stree_lookup)
     const float sn = (float)(1u << kPdfJumpBits);
     float gx = px * sn, gy = py * sn;
     float cx = fminf(fmaxf(floorf(gx), 0.0f), sn - 1.0f);
     float cy = fminf(fmaxf(floorf(gy), 0.0f), sn - 1.0f);
     px = gx - cx;
     py = gy - cy;
-    uint2 e = g.samp_jump[(size_t)cell * kPdfJumpCells + ((uint32_t)cy << kPdfJumpBits) + (uint32_t)cx];
+    uint4 e = g.samp_jump[(size_t)cell * kPdfJumpCells + ((uint32_t)cy << kPdfJumpBits) + (uint32_t)cx];
     result = bitsf(e.x);
     if (e.y == kJumpDone) return result * kInv2Pi / 2.0f;
     node = e.y;
+    base = e.z;
+  } else {
+    float4 inf = g.info[cell];
+    if (dtree_mean(inf.y, inf.z) <= 0.0f) return 1.0f * kInv2Pi / 2.0f;
+    base = fbits(inf.x);
   }
   int idx = dtree_child_idx(px, py);
   for (;;) {

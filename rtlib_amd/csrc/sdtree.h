@@ -42,7 +42,7 @@ struct DTreeArrays {
   float4* s_sums = nullptr;
   uint2* s_child = nullptr;
   uint4* s_node = nullptr;      // 2 per sampling node: sums bits, (children, 0, 0)
-  uint2* s_jump = nullptr;      // pdf jump table: kPdfJumpCells per DTree
+  uint4* s_jump = nullptr;      // pdf jump table: kPdfJumpCells per DTree
   uint32_t* b_jump = nullptr;   // record jump table of the building trees: kPdfJumpCells per DTree
 };
 

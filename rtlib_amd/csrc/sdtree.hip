@@ -91,7 +91,10 @@ __global__ void k_dtree_pdf_jump(DTreeArrays a) {
   uint32_t base = a.s_base[j];
   float result = 1.0f;
   uint32_t node = 0, cont = kJumpDone;
-  for (uint32_t l = 0; l < kPdfJumpBits; ++l) {
+  float4 inf = a.info[j];
+  // a DTree of mean 0 samples uniformly: density 1 / (4 pi) (DTreeT::Pdf)
+  const uint32_t levels = dtree_mean(inf.y, inf.z) <= 0.0f ? 0u : kPdfJumpBits;
+  for (uint32_t l = 0; l < levels; ++l) {
     uint32_t b = kPdfJumpBits - 1u - l;
     int idx = (int)(((cx >> b) & 1u) | (((cy >> b) & 1u) << 1));
     uint4 r0 = a.s_node[2u * (base + node)], r1 = a.s_node[2u * (base + node) + 1u];
@@ -108,7 +111,7 @@ __global__ void k_dtree_pdf_jump(DTreeArrays a) {
     node = ch;
     if (l + 1u == kPdfJumpBits) cont = node;
   }
-  a.s_jump[t] = make_uint2(fbits(result), cont);
+  a.s_jump[t] = make_uint4(fbits(result), cont, base, 0u);
 }
 
 // Record jump table of the building trees: work-item (DTree j, cell c) runs

@@ -69,8 +69,9 @@ struct SdTreeView {
   const uint4* samp_node;
   // pdf jump table: per DTree and cell of the 2^kPdfJumpBits square grid of
   // canonical directions, (bits of the density product of the first
-  // kPdfJumpBits levels, node to continue from or kJumpDone)
-  const uint2* samp_jump;
+  // kPdfJumpBits levels, node to continue from or kJumpDone, the DTree's
+  // node base, 0); a DTree of mean 0 has product 1 and kJumpDone everywhere
+  const uint4* samp_jump;
   // record jump table of the building trees, per DTree and cell: bit 31 set:
   // the descent ends within kPdfJumpBits levels, at node << 2 | slot (low
   // bits); else the node it continues from

@@ -96,7 +96,7 @@ struct rtpg_sdtree_t {
     rtpg::DBuf<uint2> b_child, s_child;
     rtpg::DBuf<uint4> s_node;  // interleaved sampling nodes (written by publish)
     rtpg::DBuf<uint4> s_jump;  // pdf jump table (written by publish)
-    rtpg::DBuf<uint32_t> b_jump;  // record jump table (written by publish)
+    rtpg::DBuf<uint2> b_jump;  // record jump table (written by publish)
     rtpg::DTreeArrays view();
   } dt, spare;  // spare: the other set of reset()'s ping-pong
   rtpg::DBuf<float> area_tmp;

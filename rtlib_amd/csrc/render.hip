@@ -1115,6 +1115,65 @@ __device__ __forceinline__ bool lds_table_add(uint32_t* keys, T* vals, uint32_t 
   return false;
 }
 
+// One record slot for k_commit_apply: cell == kInvalidCell when the slot
+// holds no record; flux when its value is recorded: SD-tree records with a
+// positive finite value not dropped by the directional box filter (cu < 0),
+// as DTreeT::RecordIrradiance; every Morton record, as MortonQuadTreeT::Record
+// (OPX7UtilsMorton.h:262-285) tests no value.
+struct ApplyRec {
+  uint32_t cell, key;
+  float cu, cv, value;
+  bool flux;
+};
+
+template <int kKind>
+__device__ __forceinline__ ApplyRec apply_load(const float4* recs, const uint32_t* nhit, uint32_t npaths, uint32_t r,
+                                               uint32_t end) {
+  ApplyRec q{kInvalidCell, 0u, 0.0f, 0.0f, 0.0f, false};
+  if (r >= end) return q;
+  uint32_t d = r / npaths, pid = r - d * npaths;
+  if (d + 1u >= nhit[pid]) return q;  // no vertex d on this path
+  float4 rec = recs[r];
+  q.cell = fbits(rec.x);
+  q.cu = rec.y;
+  q.cv = rec.z;
+  q.value = rec.w;
+  q.flux = q.cell != kInvalidCell && (kKind == 2 || (isfinite(q.value) && q.value > 0.0f && q.cu >= 0.0f));
+  return q;
+}
+
+// Flux slot of a record (DTreeNodeT::Record :113-125): the first levels of the
+// descent from the record jump table (cell = top bits of u, v; the remaining
+// coordinates are exact, as in dtree_pdf), the rest node by node.
+__device__ __forceinline__ uint32_t record_key(const SdTreeView& sd, uint32_t cell, float cu, float cv) {
+  const float sn = (float)(1u << kPdfJumpBits);
+  float gx = cu * sn, gy = cv * sn;
+  float jx = fminf(fmaxf(floorf(gx), 0.0f), sn - 1.0f), jy = fminf(fmaxf(floorf(gy), 0.0f), sn - 1.0f);
+  uint2 e = sd.build_jump[(size_t)cell * kPdfJumpCells + ((uint32_t)jy << kPdfJumpBits) + (uint32_t)jx];
+  const uint32_t base = e.y;
+  if (e.x & 0x80000000u) return base * 4u + (e.x & 0x7FFFFFFFu);
+  cu = gx - jx;
+  cv = gy - jy;
+  uint32_t node = e.x;
+  int idx = dtree_child_idx(cu, cv);
+  for (;;) {
+    uint32_t ch = dnode_child(sd.build_child[base + node], idx);
+    if (ch == 0) break;
+    node = ch;
+    idx = dtree_child_idx(cu, cv);
+  }
+  return (base + node) * 4u + (uint32_t)idx;
+}
+
+// Leaf bin of a record in the Morton quadtree of hash cell `cell`.
+__device__ __forceinline__ uint32_t morton_key(const MortonView& mq, uint32_t cell, float cu, float cv) {
+  uint32_t nn = 1u << mq.level;
+  uint32_t x = (uint32_t)(cu * (float)nn), y = (uint32_t)(cv * (float)nn);
+  x = x < nn ? x : nn - 1u;
+  y = y < nn ? y : nn - 1u;
+  return mq.per_cell * cell + mq_level_offset(mq.level) + morton2(x, y);
+}
+
 // kind 1: SD-tree (DTreeT::RecordIrradiance :419-427 -> DTreeNodeT::Record
 // :113-125: weight += 1, flux into the first leaf slot on the descent);
 // kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285).
@@ -1141,50 +1200,22 @@ __global__ void __launch_bounds__(kApplyBlock) k_commit_apply(const float4* recs
   __syncthreads();
   uint32_t end = min(n, begin + kApplyBlock * kApplyPerThread);
   float* flat = kKind == 1 ? reinterpret_cast<float*>(sd.build_sums) : mq.build;
-  for (uint32_t r = begin + threadIdx.x; r < end; r += kApplyBlock) {
-    uint32_t d = r / npaths, pid = r - d * npaths;
-    uint32_t nh = nhit[pid];
-    if (d + 1u >= nh) continue;  // no vertex d on this path
-    float4 rec = recs[r];
-    uint32_t cell = fbits(rec.x);
-    if (cell == kInvalidCell) continue;
-    float value = rec.w;
-    float cu = rec.y, cv = rec.z;
-    if (kKind == 1) {
-      if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) unsafeAtomicAdd(sd.build_sw + cell, 1.0f);
-      // cu < 0: flux dropped by the directional box filter
-      if (!(isfinite(value) && value > 0.0f) || cu < 0.0f) continue;
-      uint32_t base = fbits(sd.info[cell].w);
-      uint32_t key;
-      // the first levels from the record jump table (cell = top bits of u, v;
-      // the remaining coordinates are exact, as in dtree_pdf)
-      const float sn = (float)(1u << kPdfJumpBits);
-      float gx = cu * sn, gy = cv * sn;
-      float jx = fminf(fmaxf(floorf(gx), 0.0f), sn - 1.0f), jy = fminf(fmaxf(floorf(gy), 0.0f), sn - 1.0f);
-      uint32_t e = sd.build_jump[(size_t)cell * kPdfJumpCells + ((uint32_t)jy << kPdfJumpBits) + (uint32_t)jx];
-      if (e & 0x80000000u) {
-        key = base * 4u + (e & 0x7FFFFFFFu);
-      } else {
-        cu = gx - jx;
-        cv = gy - jy;
-        uint32_t node = e;
-        int idx = dtree_child_idx(cu, cv);
-        for (;;) {
-          uint32_t ch = dnode_child(sd.build_child[base + node], idx);
-          if (ch == 0) break;
-          node = ch;
-          idx = dtree_child_idx(cu, cv);
-        }
-        key = (base + node) * 4u + (uint32_t)idx;
+  // two records per iteration: their loads and descents are independent, so
+  // their latencies overlap before the LDS updates
+  for (uint32_t r = begin + threadIdx.x; r < end; r += 2u * kApplyBlock) {
+    ApplyRec a = apply_load<kKind>(recs, nhit, npaths, r, end);
+    ApplyRec b = apply_load<kKind>(recs, nhit, npaths, r + kApplyBlock, end);
+    if (a.flux) a.key = kKind == 1 ? record_key(sd, a.cell, a.cu, a.cv) : morton_key(mq, a.cell, a.cu, a.cv);
+    if (b.flux) b.key = kKind == 1 ? record_key(sd, b.cell, b.cu, b.cv) : morton_key(mq, b.cell, b.cu, b.cv);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const ApplyRec& q = k == 0 ? a : b;
+      if (kKind == 1 && q.cell != kInvalidCell) {
+        if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, q.cell, 1u)) unsafeAtomicAdd(sd.build_sw + q.cell, 1.0f);
       }
-      if (!lds_table_add<float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
-    } else {
-      uint32_t nn = 1u << mq.level;
-      uint32_t x = (uint32_t)(cu * (float)nn), y = (uint32_t)(cv * (float)nn);
-      x = x < nn ? x : nn - 1u;
-      y = y < nn ? y : nn - 1u;
-      uint32_t key = mq.per_cell * cell + mq_level_offset(mq.level) + morton2(x, y);
-      if (!lds_table_add<float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
+      if (q.flux && !lds_table_add<float>(lkey, lval, kLeafTab - 1u, q.key, q.value)) {
+        unsafeAtomicAdd(flat + q.key, q.value);
+      }
     }
   }
   __syncthreads();

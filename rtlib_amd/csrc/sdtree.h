@@ -43,7 +43,7 @@ struct DTreeArrays {
   uint2* s_child = nullptr;
   uint4* s_node = nullptr;      // 2 per sampling node: sums bits, (children, 0, 0)
   uint4* s_jump = nullptr;      // pdf jump table: kPdfJumpCells per DTree
-  uint32_t* b_jump = nullptr;   // record jump table of the building trees: kPdfJumpCells per DTree
+  uint2* b_jump = nullptr;      // record jump table of the building trees: kPdfJumpCells per DTree
 };
 
 // Kernel launchers (sdtree.hip).

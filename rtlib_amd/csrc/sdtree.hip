@@ -135,7 +135,7 @@ __global__ void k_dtree_record_jump(DTreeArrays a) {
     node = ch;
     e = node;
   }
-  a.b_jump[t] = e;
+  a.b_jump[t] = make_uint2(e, base);
 }
 
 // RTDTreeT::Reset (OPX7UtilsPathGuiding.h:888-947), one work-item per DTree.

@@ -72,10 +72,11 @@ struct SdTreeView {
   // kPdfJumpBits levels, node to continue from or kJumpDone, the DTree's
   // node base, 0); a DTree of mean 0 has product 1 and kJumpDone everywhere
   const uint4* samp_jump;
-  // record jump table of the building trees, per DTree and cell: bit 31 set:
-  // the descent ends within kPdfJumpBits levels, at node << 2 | slot (low
-  // bits); else the node it continues from
-  const uint32_t* build_jump;
+  // record jump table of the building trees, per DTree and cell: (x, base of
+  // the DTree's nodes); x with bit 31 set: the descent ends within
+  // kPdfJumpBits levels, at node << 2 | slot (low bits); else the node it
+  // continues from
+  const uint2* build_jump;
   float fraction;
   uint32_t n_dtrees;
   // recording filters: bit 0 spatial box, bit 1 directional box; per DTree the

@@ -1035,7 +1035,7 @@ hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, ucha
 // leaves near the quadtree roots do not serialise on HBM atomics.
 constexpr int kMaxCommitVtx = 16;
 constexpr uint32_t kApplyBlock = 256;
-constexpr uint32_t kApplyPerThread = 128;
+constexpr uint32_t kApplyPerThread = 256;
 constexpr uint32_t kLeafTab = 4096;
 constexpr uint32_t kCellTab = 1024;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;

@@ -323,7 +323,8 @@ def other_configs(ctx, timed):
     benchmark configurations: Cornell box plain path tracing (256^2, 64 spp,
     depth 8, RR after 3); many-light RIS (2000 icospheres, 2.56M tris, 4096
     emissive triangles, 1920x1080, 16 spp, depth 6, M = 32); path-replay
-    adjoint (Cornell 512^2, 8 orbit views 45 degrees apart, 16 spp, depth 6);
+    adjoint (8 orbit views 45 degrees apart at 512^2, 16 spp, depth 6, on the
+    Cornell box and on the 1.05M-triangle guided room);
     learned guide on the Cornell box against SD-tree guiding and plain path
     tracing at equal time (64-spp budgets, 16k-spp reference)."""
     from rtlib_amd import scenes
@@ -423,6 +424,22 @@ def other_configs(ctx, timed):
                       "msamples_per_s": round(8 * 512 * 512 * 16 / dt / 1e6, 1),
                       "note": "primal + replay per view, gradients downloaded and permuted on the host"}
     log(f"adjoint: {out['adjoint']}")
+
+    # the same adjoint batch on the 1.05M-triangle guided room (about 1M x 3
+    # albedo parameters): 8 views 45 degrees apart orbiting the left part of
+    # the room from inside
+    sc = scenes.guided_room(512, 512)
+    ctx.upload_scene(sc)
+    views = []
+    for k in range(8):
+        ang = np.radians(45.0 * k)
+        views.append(((2.5 + 1.5 * np.sin(ang), 1.5, 2.0 + 1.5 * np.cos(ang)), (2.5, 1.0, 2.0)))
+    adjoint_batch()
+    dt, _ = timed(adjoint_batch)
+    out["adjoint_room"] = {"ms_per_batch": round(dt * 1e3, 1),
+                           "msamples_per_s": round(8 * 512 * 512 * 16 / dt / 1e6, 1),
+                           "parameters": int(sc["idx"].shape[0]) * 3 + 3}
+    log(f"adjoint (room): {out['adjoint_room']}")
     return out
 
 

@@ -414,15 +414,19 @@ extern "C" int rtpg_adjoint(rtpg_context ctx, const rtpg_render_params* prm, con
     hipStream_t s = ctx->stream;
     uint32_t npix = ctx->width * ctx->height;
     uint32_t nl = std::max(ctx->n_lights, 1u);
-    DBuf<float> d_dldi, d_galb, d_gle, scratch;
-    DBuf<float4> d_ltot;
+    // buffers kept by the context between calls (they only grow)
+    DBuf<float>& d_dldi = ctx->adj_dldi;
+    DBuf<float>& d_galb = ctx->adj_galb;
+    DBuf<float>& d_gle = ctx->adj_gle;
+    DBuf<float>& scratch = ctx->adj_scratch;
+    DBuf<float4>& d_ltot = ctx->adj_ltot;
     d_dldi.upload(dl_dimage, 3 * (size_t)npix, s);
-    d_galb.alloc(3 * (size_t)ctx->n_tris);
+    d_galb.reserve(3 * (size_t)ctx->n_tris);
     d_galb.zero(3 * (size_t)ctx->n_tris, s);
-    d_gle.alloc(3 * (size_t)nl);
+    d_gle.reserve(3 * (size_t)nl);
     d_gle.zero(3 * (size_t)nl, s);
-    scratch.alloc(3 * (size_t)npix);
-    uint32_t max_paths = prm->max_paths ? prm->max_paths : (1u << 22);
+    scratch.reserve(3 * (size_t)npix);
+    uint32_t max_paths = std::min(prm->max_paths ? prm->max_paths : (1u << 25), 1u << 26);
     uint32_t chunk_spp = std::min(std::max(1u, max_paths / npix), prm->spp);
     SceneView sc = ctx->scene_view();
     SdTreeView sd{};
@@ -449,7 +453,9 @@ extern "C" int rtpg_adjoint(rtpg_context ctx, const rtpg_render_params* prm, con
       RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, MlpView{}, 0, scratch.p, s));
       done += cs;
     }
-    std::vector<float> galb = d_galb.to_vector(3 * (size_t)ctx->n_tris, s);
+    std::vector<float>& galb = ctx->adj_galb_host;
+    galb.resize(3 * (size_t)ctx->n_tris);
+    d_galb.download(galb.data(), galb.size(), s);
     std::vector<float> gle = d_gle.to_vector(3 * (size_t)nl, s);
     uint32_t err = 0;
     ctx->p_err.download(&err, 1, s);

@@ -28,6 +28,10 @@ struct rtpg_context_t {
   std::vector<float> shade_host;     // leaf order, 4 per triangle
   std::vector<float4> lights_host;   // 4 per light, as on the device
   std::vector<uint32_t> light_tri;   // light -> original triangle
+  // path-replay adjoint buffers, kept between calls
+  rtpg::DBuf<float> adj_dldi, adj_galb, adj_gle, adj_scratch;
+  rtpg::DBuf<float4> adj_ltot;
+  std::vector<float> adj_galb_host;
   uint32_t n_tris = 0, n_lights = 0, n_nodes = 0;
   uint32_t bvh_depth = 0, max_stack = 0, max_leaf = 0;
   float world[6] = {0, 0, 0, 0, 0, 0};

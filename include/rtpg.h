@@ -332,11 +332,20 @@ typedef struct rtpg_mlp_config {
   uint32_t record_paths; /* paths per render chunk that record samples (2^16) */
   uint64_t seed;         /* initialisation */
   uint32_t guide_depth;  /* vertices guided and recorded: depth < guide_depth (0 = all) */
+  uint32_t cache_res;    /* 0: the network is evaluated at every guided vertex; n > 0: the
+                            renderer reads a spatial cache of n^3 cells x 6 normal directions
+                            over the box, refilled from the network after training */
 } rtpg_mlp_config;
 /* cfg = NULL takes the defaults above.  The position box is made cubic. */
 int rtpg_mlp_create(rtpg_context ctx, const float aabb_min[3], const float aabb_max[3], const rtpg_mlp_config* cfg,
                     rtpg_mlp* out);
 int rtpg_mlp_destroy(rtpg_mlp m);
+/* The spatial cache (cfg.cache_res > 0), refilled first if the network changed:
+ * entries = cache_res^3 * 6 (cell (x + res (y + res z)) * 6 + face, face 0 +x,
+ * 1 -x, 2 +y, 3 -y, 4 +z, 5 -z); per entry mix[48] = 8 x (mu xyz, kappa),
+ * pi[8], pi_j kappa_j / (2 pi (1 - exp(-2 kappa_j)))[8] and c[1].  mix and c
+ * may be NULL (only *entries is written). */
+int rtpg_mlp_cache_download(rtpg_mlp m, float* mix, float* c, uint32_t* entries);
 /* Re-initialise weights and table from seed; clear the Adam state and the recorded samples. */
 int rtpg_mlp_reset(rtpg_mlp m, uint64_t seed);
 /* Network queries at positions pos[3n] with outgoing directions wo[3n], normals

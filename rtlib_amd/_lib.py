@@ -51,6 +51,7 @@ class MlpConfig(C.Structure):
         ("record_paths", C.c_uint32),
         ("seed", C.c_uint64),
         ("guide_depth", C.c_uint32),
+        ("cache_res", C.c_uint32),
     ]
 
 
@@ -80,6 +81,7 @@ SIGNATURES = {
     "rtpg_render_kernel_stats": [c_vp, C.POINTER(C.c_double), c_u64p, c_u64p],
     "rtpg_mlp_create": [c_vp, c_fp, c_fp, C.POINTER(MlpConfig), C.POINTER(c_vp)],
     "rtpg_mlp_destroy": [c_vp],
+    "rtpg_mlp_cache_download": [c_vp, c_fp, c_fp, C.POINTER(C.c_uint32)],
     "rtpg_mlp_reset": [c_vp, C.c_uint64],
     "rtpg_mlp_forward": [c_vp, c_fp, c_fp, c_fp, c_fp, C.c_uint32, c_fp, c_fp],
     "rtpg_mlp_sample": [c_vp, c_fp, C.c_uint32, C.c_uint64, c_fp],

@@ -312,7 +312,8 @@ SdTreeView sdtree_view_of(rtpg_sdtree_t* t);
 void guide_views(const rtpg_render_params* rp, rtpg_context_t* ctx, HashGridView& hg, MortonView& mq);
 }  // namespace rtpg
 
-static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool guide, uint32_t mlp_rec) {
+static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool guide, uint32_t mlp_rec,
+                         uint32_t mix_slots) {
   c->p_ray_o.reserve(n);
   c->p_ray_d.reserve(n);
   c->p_T.reserve(n);
@@ -324,7 +325,7 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
   c->p_q1.reserve(n);
   c->p_shadow.reserve(3 * (size_t)n);
   c->p_count.reserve(2 * (size_t)(max_depth + 2));
-  // spill stack: (entry, distance) pairs beyond the 16 LDS entries
+  // spill stack: (entry, distance) pairs beyond the kShortStack LDS entries
   if (c->max_stack > kShortStack) c->p_deep.reserve(2 * (size_t)(c->max_stack - kShortStack) * n);
   if (guide) {
     c->p_vA.reserve((size_t)max_depth * n);
@@ -333,17 +334,19 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
     c->p_recs.reserve((size_t)(max_depth > 1 ? max_depth - 1 : 1) * n);
     c->p_rec_count.reserve(1);
   }
-  if (mlp_rec) {
-    c->p_mix.reserve((size_t)kMlpMixF4 * n);
-    c->p_mixc.reserve(n);
-    c->p_mv.reserve((size_t)5 * max_depth * mlp_rec);
+  if (mix_slots) {
+    c->p_mix.reserve((size_t)kMlpMixF4 * mix_slots);
+    c->p_mixc.reserve(mix_slots);
   }
+  if (mlp_rec) c->p_mv.reserve((size_t)5 * max_depth * mlp_rec);
 }
 
 static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, bool guide, bool counting,
                         rtpg_mlp_t* mlp = nullptr) {
   uint32_t mlp_rec = mlp ? std::min(n, mlp->cfg.record_paths) : 0u;
-  ensure_arena(ctx, n, max_depth, guide, mlp_rec);
+  // per-slot mixtures of the inference pass (none with the spatial cache)
+  uint32_t mix_slots = (mlp && mlp->cfg.cache_res == 0) ? n : 0u;
+  ensure_arena(ctx, n, max_depth, guide, mlp_rec, mix_slots);
   PathSoA ps{};
   ps.n = n;
   ps.ray_o = ctx->p_ray_o.p;
@@ -377,6 +380,13 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
     ps.mlp_out = mlp->samples.p;
     ps.mlp_count = mlp->count.p;
     ps.mlp_cap = mlp->cfg.capacity;
+    if (mlp->cfg.cache_res > 0) {
+      ps.mcache = mlp->cache_mix.p;
+      ps.mcache_c = mlp->cache_c.p;
+      ps.mcache_res = mlp->cfg.cache_res;
+      for (int k = 0; k < 3; ++k) ps.mcache_lo[k] = mlp->lo[k];
+      ps.mcache_inv = 1.0f / mlp->size;
+    }
   }
   return ps;
 }
@@ -524,7 +534,10 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     MlpView ml{};
     if (guide == 1) sd = sdtree_view_of(prm->sdtree);
     if (guide == 2) guide_views(prm, ctx, hg, mq);
-    if (guide == 3) ml = mlp_view_of(prm->mlp, false);
+    if (guide == 3) {
+      if (prm->flags & RTPG_FLAG_GUIDE_SAMPLE) mlp_cache_refresh(prm->mlp);
+      ml = mlp_view_of(prm->mlp, false);
+    }
     hipStream_t s = ctx->stream;
     ctx->p_stats.zero(4, s);
     const bool timing = (ctx->prof_mode & 1u) != 0;
@@ -577,7 +590,8 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       ctx->kernel_launches[3] += 1;
       if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL) && !(guide == 2 && (prm->flags & RTPG_FLAG_LOCATE)))
         ctx->kernel_launches[4] += guide == 3 ? 1 : 2;
-      if (guide == 3 && (prm->flags & RTPG_FLAG_GUIDE_SAMPLE)) ctx->kernel_launches[1] += prm->max_depth;
+      if (guide == 3 && (prm->flags & RTPG_FLAG_GUIDE_SAMPLE) && prm->mlp->cfg.cache_res == 0)
+        ctx->kernel_launches[1] += prm->max_depth;
       done += cs;
     }
     if (timing) drain_events();

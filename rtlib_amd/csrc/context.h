@@ -133,7 +133,13 @@ struct rtpg_mlp_t {
   rtpg::DBuf<float4> staging;
   rtpg::DBuf<float> out;
   uint64_t steps = 0;
+  rtpg::DBuf<float4> cache_mix;  // spatial cache (cfg.cache_res > 0)
+  rtpg::DBuf<float> cache_c;
+  bool cache_stale = true;       // the network changed since the last fill
   size_t table_floats() const { return (size_t)rtpg::kMlpLevels * entries * rtpg::kMlpFeat; }
+  uint32_t cache_entries() const { return cfg.cache_res * cfg.cache_res * cfg.cache_res * rtpg::kMlpCacheFaces; }
 };
 
 rtpg::MlpView mlp_view_of(const rtpg_mlp_t* m, bool training);
+// Refills the spatial cache if the network changed since the last fill.
+void mlp_cache_refresh(rtpg_mlp_t* m);

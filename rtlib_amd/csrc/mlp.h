@@ -28,5 +28,7 @@ hipError_t launch_mlp_forward(const MlpView& m, const float4* q, uint32_t n, flo
 // Draws one direction per sample from the mixture of launch_mlp_forward's
 // mix output (41 floats per sample); out: (w xyz, mixture pdf) per sample.
 hipError_t launch_mlp_sample(const float* mix, uint32_t n, uint64_t seed, float4* out, hipStream_t s);
+// Fills the n = cache_res^3 * 6 entries of the spatial cache (mlp_cache_entry).
+hipError_t launch_mlp_cache_fill(const MlpView& m, uint32_t n, float4* mix, float* c, hipStream_t s);
 
 }  // namespace rtpg

@@ -334,6 +334,27 @@ __global__ void __launch_bounds__(256) k_mlp_forward_q(MlpView m, const float4* 
   }
 }
 
+// Spatial cache fill: every entry's mixture (kMlpMixF4 float4, the layout
+// k_shade reads) and selection probability, 64 entries per work-group.
+__global__ void __launch_bounds__(256) k_mlp_cache_fill(MlpView m, uint32_t n, float4* mix, float* cout) {
+  __shared__ uint16_t lw[kLdsWeights];
+  __shared__ float hs[4][16 * kHeadPitch];
+  mlp_load_weights(lw, m.w);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15;
+  uint32_t e = blockIdx.x * 64u + 16u * (uint32_t)wave + (uint32_t)r;
+  bool valid = e < n;
+  MlpInput in = mlp_cache_input(m, valid ? e : 0u);
+  MlpActs act;
+  mlp_inputs(m, in, lane, act.x0);
+  f32x4_t o[3];
+  mlp_forward(lw, act, o, lane);
+  mlp_spill(hs[wave], o, lane);
+  __syncthreads();
+  MlpHead h = mlp_head(hs[wave] + r * kHeadPitch, lane);
+  if (valid) mlp_write_mix(h, reinterpret_cast<float*>(mix + (size_t)e * kMlpMixF4), cout + e, lane);
+}
+
 __global__ void k_mlp_sample_q(const float* mix, uint32_t n, uint64_t seed, float4* out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -388,6 +409,12 @@ hipError_t launch_mlp_adam(float* p, float* m1, float* m2, float* g, float gscal
 hipError_t launch_mlp_forward(const MlpView& m, const float4* q, uint32_t n, float* raw, float* mix, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_mlp_forward_q, dim3(nblk(n, 64)), dim3(256), 0, s, m, q, n, raw, mix);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_cache_fill(const MlpView& m, uint32_t n, float4* mix, float* c, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_cache_fill, dim3(nblk(n, 64)), dim3(256), 0, s, m, n, mix, c);
   return hipGetLastError();
 }
 

@@ -43,7 +43,36 @@ struct MlpView {
   uint32_t dense;         // bit l: level l indexed densely
   float res[kMlpLevels];  // grid resolution of level l (integral)
   uint32_t ready;         // trained: the renderer may sample from it
+  uint32_t cache_res;     // spatial cache: cells per axis of the box (0: no cache)
 };
+
+// Spatial cache of the guide: one mixture per (cell of a cache_res^3 grid
+// over the network's cubic box, normal direction), the network evaluated at
+// the cell centre with that normal, wo = normal and roughness 0.  Entry =
+// cell * 6 + face, face 0 +x, 1 -x, 2 +y, 3 -y, 4 +z, 5 -z (largest normal
+// component).  The renderer samples and evaluates the cached mixture of the
+// vertex's entry, so proposal and density stay consistent.
+constexpr uint32_t kMlpCacheFaces = 6;
+
+RTPG_HD uint32_t mlp_cache_face(V3 n) {
+  float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+  if (ax >= ay && ax >= az) return n.x >= 0.0f ? 0u : 1u;
+  if (ay >= az) return n.y >= 0.0f ? 2u : 3u;
+  return n.z >= 0.0f ? 4u : 5u;
+}
+
+RTPG_HD uint32_t mlp_cache_cell1(float p, float lo, float inv_size, uint32_t res) {
+  float q = (p - lo) * inv_size * (float)res;
+  q = fminf(fmaxf(floorf(q), 0.0f), (float)(res - 1u));
+  return (uint32_t)q;
+}
+
+RTPG_HD uint32_t mlp_cache_entry(const float* lo, float inv_size, uint32_t res, V3 p, V3 n) {
+  uint32_t x = mlp_cache_cell1(p.x, lo[0], inv_size, res);
+  uint32_t y = mlp_cache_cell1(p.y, lo[1], inv_size, res);
+  uint32_t z = mlp_cache_cell1(p.z, lo[2], inv_size, res);
+  return ((z * res + y) * res + x) * kMlpCacheFaces + mlp_cache_face(n);
+}
 
 // ------------------------------------------------------------- bf16 helpers
 RTPG_HD uint16_t f2bf(float f) {

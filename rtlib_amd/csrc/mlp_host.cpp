@@ -49,6 +49,7 @@ rtpg_mlp_config default_config() {
   c.record_paths = 1u << 16;
   c.seed = 0;
   c.guide_depth = 0;
+  c.cache_res = 0;
   return c;
 }
 
@@ -79,6 +80,7 @@ void grad_pass(rtpg_mlp_t* m, const float4* smp, uint32_t n) {
 void adam_step(rtpg_mlp_t* m, uint32_t n) {
   hipStream_t s = m->ctx->stream;
   m->steps += 1;
+  m->cache_stale = true;
   const rtpg_mlp_config& c = m->cfg;
   double b1t = std::pow((double)c.beta1, (double)m->steps), b2t = std::pow((double)c.beta2, (double)m->steps);
   float lr_t = (float)(c.lr * std::sqrt(1.0 - b2t) / (1.0 - b1t));
@@ -97,6 +99,7 @@ float download_loss(rtpg_mlp_t* m, uint32_t n) {
 
 void reset_state(rtpg_mlp_t* m, uint64_t seed) {
   hipStream_t s = m->ctx->stream;
+  m->cache_stale = true;
   size_t tf = m->table_floats();
   RTPG_HIP(launch_mlp_init(m->w.p, m->w_bf16.p, m->t.p, m->t_bf16.p, tf, seed, s));
   m->w_m1.zero(kMlpWeights, s);
@@ -128,7 +131,17 @@ MlpView mlp_view_of(const rtpg_mlp_t* m, bool training) {
   v.dense = m->dense;
   for (int l = 0; l < kMlpLevels; ++l) v.res[l] = m->res[l];
   v.ready = 1u;
+  v.cache_res = m->cfg.cache_res;
   return v;
+}
+
+void mlp_cache_refresh(rtpg_mlp_t* m) {
+  if (m->cfg.cache_res == 0 || !m->cache_stale) return;
+  uint32_t n = m->cache_entries();
+  m->cache_mix.reserve((size_t)kMlpMixF4 * n);
+  m->cache_c.reserve(n);
+  RTPG_HIP(launch_mlp_cache_fill(mlp_view_of(m, false), n, m->cache_mix.p, m->cache_c.p, m->ctx->stream));
+  m->cache_stale = false;
 }
 
 extern "C" {
@@ -143,6 +156,7 @@ int rtpg_mlp_create(rtpg_context ctx, const float aabb_min[3], const float aabb_
     require(c.lr > 0.0f && c.beta1 >= 0.0f && c.beta1 < 1.0f && c.beta2 >= 0.0f && c.beta2 < 1.0f && c.eps >= 0.0f,
             "bad Adam parameters");
     require(c.batch >= 1 && c.capacity >= 1 && c.record_paths >= 1, "batch, capacity and record_paths must be >= 1");
+    require(c.cache_res <= 256, "cache_res must be in [0, 256]");
     RTPG_HIP(hipSetDevice(ctx->device));
     auto m = new rtpg_mlp_t();
     try {
@@ -329,9 +343,24 @@ int rtpg_mlp_export(rtpg_mlp m, float* weights, float* table) {
   });
 }
 
+int rtpg_mlp_cache_download(rtpg_mlp m, float* mix, float* c, uint32_t* entries) {
+  return guarded([&] {
+    require(m != nullptr && entries != nullptr, "NULL argument");
+    require(m->cfg.cache_res > 0, "the mlp has no spatial cache (cache_res = 0)");
+    RTPG_HIP(hipSetDevice(m->ctx->device));
+    *entries = m->cache_entries();
+    if (!mix && !c) return;
+    mlp_cache_refresh(m);
+    hipStream_t s = m->ctx->stream;
+    if (mix) m->cache_mix.download(reinterpret_cast<float4*>(mix), (size_t)kMlpMixF4 * *entries, s);
+    if (c) m->cache_c.download(c, *entries, s);
+  });
+}
+
 int rtpg_mlp_import(rtpg_mlp m, const float* weights, const float* table) {
   return guarded([&] {
     require(m != nullptr, "NULL mlp");
+    m->cache_stale = true;
     RTPG_HIP(hipSetDevice(m->ctx->device));
     hipStream_t s = m->ctx->stream;
     if (weights) {

@@ -115,6 +115,17 @@ struct MlpInput {
   float rough;
 };
 
+// Network input of spatial-cache entry e (see mlp_cache_entry).
+__device__ __forceinline__ MlpInput mlp_cache_input(const MlpView& m, uint32_t e) {
+  const uint32_t res = m.cache_res, face = e % kMlpCacheFaces, cell = e / kMlpCacheFaces;
+  const uint32_t x = cell % res, y = (cell / res) % res, z = cell / (res * res);
+  const float size = 1.0f / m.inv_size, step = size / (float)res;
+  V3 p{m.lo[0] + ((float)x + 0.5f) * step, m.lo[1] + ((float)y + 0.5f) * step, m.lo[2] + ((float)z + 0.5f) * step};
+  float sgn = (face & 1u) ? -1.0f : 1.0f;
+  V3 n{face / 2u == 0u ? sgn : 0.0f, face / 2u == 1u ? sgn : 0.0f, face / 2u == 2u ? sgn : 0.0f};
+  return MlpInput{p, n, n, 0.0f};
+}
+
 struct MlpActs {
   bf16x4_t x0[3], h1[4], h2[4], h3[4];
 };

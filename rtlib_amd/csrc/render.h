@@ -66,6 +66,13 @@ struct PathSoA {
   float4* mv;
   uint32_t mlp_rec;
   uint32_t mlp_guide_depth;  // vertices at depth < this are guided and recorded
+  // spatial cache of the learned guide (null: the per-slot mixtures of the
+  // inference pass): entries of mlp_cache_entry over the network's box
+  const float4* mcache;
+  const float* mcache_c;
+  uint32_t mcache_res;
+  float mcache_lo[3];
+  float mcache_inv;
   float4* mlp_out;
   uint32_t* mlp_count;
   uint32_t mlp_cap;

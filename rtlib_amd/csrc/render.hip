@@ -611,8 +611,14 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
           const float4* mx = nullptr;
           float csel = 0.0f;
           if (kGuide == 3 && (rp.flags & kFlagGuideSample) && depth < ps.mlp_guide_depth) {
-            mx = ps.mix + (size_t)i * kMlpMixF4;
-            csel = ps.mixc[i];
+            if (ps.mcache) {
+              uint32_t e = mlp_cache_entry(ps.mcache_lo, ps.mcache_inv, ps.mcache_res, p, n);
+              mx = ps.mcache + (size_t)e * kMlpMixF4;
+              csel = ps.mcache_c[e];
+            } else {
+              mx = ps.mix + (size_t)i * kMlpMixF4;
+              csel = ps.mixc[i];
+            }
           }
           const bool guided = kGuide == 3 ? mx != nullptr
                                           : (kGuide != 0) && (rp.flags & kFlagGuideSample) && vcell != kInvalidCell;
@@ -1416,7 +1422,7 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
       RTPG_SHADE(2, false)
     } else if (guide == 3) {
       // the learned guide's inference pass is timed with the shade class
-      if ((rp.flags & kFlagGuideSample) && d < ps.mlp_guide_depth) {
+      if ((rp.flags & kFlagGuideSample) && d < ps.mlp_guide_depth && !ps.mcache) {
         uint32_t ni = std::min(blocks_for(ps.n, 64u), kInferGrid);
         hipLaunchKernelGGL(k_mlp_infer, dim3(ni), dim3(kInferBlock), 0, s, sc, ps, ml, d);
       }

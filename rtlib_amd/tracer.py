@@ -572,9 +572,9 @@ class MLPGuide:
 
     def __init__(self, ctx: Context, aabb_min, aabb_max, log2_table=19, base_res=16.0, max_res=2048.0, lr=1e-2,
                  beta1=0.9, beta2=0.99, eps=1e-15, batch=1 << 18, capacity=1 << 20, record_paths=1 << 16, seed=0,
-                 guide_depth=0):
+                 guide_depth=0, cache_res=0):
         cfg = _lib.MlpConfig(log2_table, base_res, max_res, lr, beta1, beta2, eps, batch, capacity, record_paths,
-                             seed, guide_depth)
+                             seed, guide_depth, cache_res)
         h = C.c_void_p()
         call("rtpg_mlp_create", ctx.h, ptr(_f32(aabb_min)), ptr(_f32(aabb_max)), C.byref(cfg), C.byref(h))
         self.h = h.value
@@ -585,6 +585,15 @@ class MLPGuide:
         if getattr(self, "h", None):
             call("rtpg_mlp_destroy", self.h)
             self.h = None
+
+    def cache(self):
+        """Spatial cache (cache_res > 0): (mix entries x 48, c entries)."""
+        n = C.c_uint32()
+        call("rtpg_mlp_cache_download", self.h, None, None, C.byref(n))
+        mix = np.zeros((n.value, 48), np.float32)
+        c = np.zeros(n.value, np.float32)
+        call("rtpg_mlp_cache_download", self.h, ptr(mix), ptr(c), C.byref(n))
+        return mix, c
 
     def __del__(self):
         try:

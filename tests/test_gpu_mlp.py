@@ -144,3 +144,34 @@ def test_training_fits_target(ctx):
     # the guide is preferred over the (uniform) BSDF
     assert np.median(mix[:, 40]) > 0.8
     assert g.info()["steps"] == 300
+
+
+def test_spatial_cache_is_the_network_at_cell_centres(ctx):
+    # cache entry (cell, face) = the network at the cell centre with normal =
+    # face direction, wo = normal, roughness 0
+    rng = np.random.default_rng(9)
+    g, enc = _guide(ctx, seed=2, cache_res=4)
+    _randomise(g, enc, rng)
+    mix, c = g.cache()
+    res = 4
+    assert mix.shape == (res ** 3 * 6, 48) and c.shape == (res ** 3 * 6,)
+    lo32, hi32 = LO.astype(np.float32), HI.astype(np.float32)
+    ext = np.float32(np.max(hi32 - lo32))
+    lo = np.float32(0.5) * (lo32 + hi32) - np.float32(0.5) * ext
+    step = (np.float32(1) / (np.float32(1) / ext)) / np.float32(res)
+    e = np.arange(res ** 3 * 6)
+    cell, face = e // 6, e % 6
+    xyz = np.stack([cell % res, (cell // res) % res, cell // (res * res)], -1).astype(np.float32)
+    pos = lo + (xyz + np.float32(0.5)) * step
+    nrm = np.zeros((e.size, 3), np.float32)
+    nrm[np.arange(e.size), face // 2] = np.where(face % 2 == 0, 1.0, -1.0)
+    _, fwd = g.forward(pos, nrm, nrm, np.zeros(e.size, np.float32))
+    pi, mu, kappa = fwd[:, 0:40:5], np.stack([fwd[:, 1:40:5], fwd[:, 2:40:5], fwd[:, 3:40:5]], -1), fwd[:, 4:40:5]
+    assert np.allclose(mix[:, 32:40], pi, rtol=1e-3, atol=1e-5)
+    assert np.allclose(mix[:, :32].reshape(-1, 8, 4)[..., :3], mu, atol=1e-3)
+    assert np.allclose(mix[:, :32].reshape(-1, 8, 4)[..., 3], kappa, rtol=1e-3)
+    assert np.allclose(c, fwd[:, 40], rtol=1e-3)
+    # A_j = pi_j kappa_j / (2 pi (1 - exp(-2 kappa_j)))
+    a = pi * kappa / (2 * np.pi * -np.expm1(-2 * kappa))
+    assert np.allclose(mix[:, 40:48], a, rtol=2e-3)
+    g.close()

@@ -61,6 +61,10 @@ def main(argv=None):
                     "learned-guide configurations")
     ap.add_argument("--mlp-passes", type=int, default=8)
     ap.add_argument("--mlp-steps", type=int, default=4)
+    ap.add_argument("--mlp-cache", type=int, default=48, help="spatial-cache cells per axis of the learned guide "
+                    "(0: network inference at every guided vertex)")
+    ap.add_argument("--large-budget", type=int, default=1024, help="second budget for the equal-time comparison "
+                    "of the SD-tree and the learned guide (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--small", action="store_true", help="reduced size for a quick functional check")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -181,19 +185,27 @@ def main(argv=None):
     quality = None
     if a.ref_spp > 0:
         # learned guide on the same budget: a.mlp_passes recording passes of 1
-        # spp, each followed by a.mlp_steps Adam steps, then one final guided pass
-        mlp = MLPGuide(ctx, lo, hi, seed=1)
+        # spp, each followed by a.mlp_steps Adam steps, then one final guided
+        # pass that reads the spatial cache of the network (a.mlp_cache cells
+        # per axis); and the same with inference at every guided vertex
+        mlp = MLPGuide(ctx, lo, hi, seed=1, cache_res=a.mlp_cache)
+        mlp_pv = MLPGuide(ctx, lo, hi, seed=1)
 
-        def mlp_step():
-            mlp.reset(1)
+        def mlp_step(g=mlp, budget=a.budget):
+            g.reset(1)
             ctx.clear()
-            return render_mlp_guided(ctx, mlp, a.budget, base, max_depth=depth, seed=11,
+            return render_mlp_guided(ctx, g, budget, base, max_depth=depth, seed=11,
                                      train_passes=a.mlp_passes, steps_per_pass=a.mlp_steps)
 
         timed(mlp_step)
         t_mlp, _ = timed(mlp_step)
         mlp_img = ctx.image()
-        log(f"learned guide step: {t_mlp * 1e3:.1f} ms ({a.budget} spp)")
+        log(f"learned guide step: {t_mlp * 1e3:.1f} ms ({a.budget} spp, cache {a.mlp_cache})")
+        timed(lambda: mlp_step(mlp_pv))
+        t_pv, _ = timed(lambda: mlp_step(mlp_pv))
+        pv_img = ctx.image()
+        log(f"learned guide step, per-vertex inference: {t_pv * 1e3:.1f} ms ({a.budget} spp)")
+        mlp_pv.close()
 
         def plain_equal_time(t, seed):
             spp = max(1, int(round(t * plain_rate * 1e6 / npix)))
@@ -231,7 +243,25 @@ def main(argv=None):
         spp_tree, plain_tree = plain_equal_time(t_step, 78)
         spp_box, plain_box = plain_equal_time(t_box, 81)
         spp_mlp, plain_mlp = plain_equal_time(t_mlp, 79)
+        spp_pv, plain_pv = plain_equal_time(t_pv, 82)
         spp_grid, plain_grid = plain_equal_time(t_grid, 80)
+        # the SD-tree and the learned guide on a larger budget, where their
+        # training passes are a smaller share of the time
+        if a.large_budget > 0:
+            def tree_large():
+                ctrl = RTSTreeController(tree, a.large_budget, 0, 0.5, 1)
+                ctx.clear()
+                return render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=92)
+
+            timed(tree_large)
+            t_tl, _ = timed(tree_large)
+            tl_img = ctx.image()
+            timed(lambda: mlp_step(mlp, a.large_budget))
+            t_ml, _ = timed(lambda: mlp_step(mlp, a.large_budget))
+            ml_img = ctx.image()
+            log(f"budget {a.large_budget}: SD-tree {t_tl * 1e3:.1f} ms, learned guide {t_ml * 1e3:.1f} ms")
+            spp_tl, plain_tl = plain_equal_time(t_tl, 83)
+            spp_ml, plain_ml = plain_equal_time(t_ml, 84)
         # reference image (plain, many samples, independent seed)
         t0 = time.time()
         ctx.clear()
@@ -254,8 +284,17 @@ def main(argv=None):
                          "plain_spp": spp_grid, "plain_relmse": round(rel_mse(plain_grid, ref), 5)},
             "mlp": {"ms": round(t_mlp * 1e3, 1), "relmse": round(rel_mse(mlp_img, ref), 5),
                     "plain_spp": spp_mlp, "plain_relmse": round(rel_mse(plain_mlp, ref), 5),
-                    "train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps},
+                    "train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps, "cache_res": a.mlp_cache},
+            "mlp_per_vertex_inference": {"ms": round(t_pv * 1e3, 1), "relmse": round(rel_mse(pv_img, ref), 5),
+                                         "plain_spp": spp_pv, "plain_relmse": round(rel_mse(plain_pv, ref), 5)},
         }
+        if a.large_budget > 0:
+            quality[f"budget_{a.large_budget}"] = {
+                "sdtree": {"ms": round(t_tl * 1e3, 1), "relmse": round(rel_mse(tl_img, ref), 5),
+                           "plain_spp": spp_tl, "plain_relmse": round(rel_mse(plain_tl, ref), 5)},
+                "mlp": {"ms": round(t_ml * 1e3, 1), "relmse": round(rel_mse(ml_img, ref), 5),
+                        "plain_spp": spp_ml, "plain_relmse": round(rel_mse(plain_ml, ref), 5)},
+            }
         mlp.close()
 
     # ---- roofline of the dominant kernel (k_extend): algorithmic bytes of

@@ -53,6 +53,11 @@ extern "C" {
 #define RTPG_MAT_PHONG 2u /* reference "PHONG": param ignored, specular + shininess from spec[] */
 #define RTPG_MAT_GLASS 3u /* param = index of refraction */
 
+/* rtpg_scene_attributes: "no texture" index; texture address modes */
+#define RTPG_TEX_NONE 0xFFFFFFFFu
+#define RTPG_TEX_CLAMP 0u
+#define RTPG_TEX_WRAP 1u
+
 typedef struct rtpg_context_t* rtpg_context;
 typedef struct rtpg_sdtree_t* rtpg_sdtree;
 typedef struct rtpg_sdtree_ctrl_t* rtpg_sdtree_ctrl;
@@ -90,6 +95,27 @@ int rtpg_scene_upload(rtpg_context ctx, const float* verts, uint32_t nv, const u
                       uint32_t light_mode, const uint32_t* light_mesh);
 /* Replace per-triangle albedo (rgb, 3*nt) in original triangle order. */
 int rtpg_scene_set_albedo(rtpg_context ctx, const float* albedo_rgb);
+/* Texture maps and per-vertex attributes of the uploaded triangles, in the
+ * triangle order of rtpg_scene_upload.  Replaces TextureData::LoadFromPath and
+ * the hitgroup/mesh-light texture objects (RTLibExtOPX7Test.h:386-511,
+ * 1602-1638; RTLibExtOPX7TestApplication.cpp:262-280), read where the
+ * reference calls GetTexCrd, SampleDiffuse/SampleSpecular/SampleEmission,
+ * MeshLight::GetEmission and GetTriangleVNormal (SimpleKernel.h:70-104,
+ * 238-272).
+ *   texcoord: 6*nt floats (u, v at the triangle's three vertices)
+ *   tex_id:   4*nt texture indices: diffuse, specular, emission on hit,
+ *             emission when light-sampled; RTPG_TEX_NONE = the reference's
+ *             constant white texture (factor 1).  NULL: no texturing.
+ *   vnormal:  9*nt floats (vertex normals; glass shades with their
+ *             barycentric interpolation, normalised), or NULL (face normal)
+ *   texels:   n_texels RGBA8 words (R in the low byte), rows bottom-up
+ *   tex:      4*n_tex words per texture: first texel, width, height,
+ *             address mode (RTPG_TEX_CLAMP / RTPG_TEX_WRAP); at most 65535
+ * Sampling is bilinear on normalised coordinates with 8-bit fractional
+ * weights and unorm texels, the filtering of the reference's CUDA texture
+ * objects.  A new rtpg_scene_upload drops the attributes. */
+int rtpg_scene_attributes(rtpg_context ctx, const float* texcoord, const uint32_t* tex_id, const float* vnormal,
+                          const uint32_t* texels, uint64_t n_texels, const uint32_t* tex, uint32_t n_tex);
 /* info: [0] BVH nodes, [1] BVH4 depth, [2] stack bound, [3] max leaf size,
  *       [4] lights, [5] triangles.  aabb: world box (lo xyz, hi xyz). */
 int rtpg_scene_info(rtpg_context ctx, uint32_t info[6], float aabb[6]);

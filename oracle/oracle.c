@@ -990,6 +990,13 @@ struct orc_scene {
   bnode* nodes;
   uint32_t nn;
   uint32_t* prim;  /* leaf order -> triangle */
+  uint32_t* light_tri; /* light -> triangle */
+  /* optional attributes (original triangle order), NULL when absent */
+  float* tc;          /* 6 per triangle: u, v at vertices 0, 1, 2 */
+  uint32_t* tid;      /* 4 per triangle: diffuse, specular, emission on hit, emission light-sampled */
+  float* vn;          /* 9 per triangle: vertex normals */
+  uint32_t* texels;   /* RGBA8 */
+  uint32_t* tex;      /* 4 per texture: first texel, width, height, wrap */
 };
 
 typedef struct {
@@ -1164,10 +1171,12 @@ orc_scene* orc_scene_create(const float* verts, uint32_t nv, const uint32_t* idx
   s->nl = nl;
   s->light = (float*)malloc(sizeof(float) * 15 * (nl ? nl : 1));
   s->cdf = (float*)malloc(sizeof(float) * (nl ? nl : 1));
+  s->light_tri = (uint32_t*)malloc(sizeof(uint32_t) * (nl ? nl : 1));
   double acc = 0.0;
   for (uint32_t t = 0; t < nt; ++t) {
     int32_t k = s->light_of[t];
     if (k < 0) continue;
+    s->light_tri[k] = t;
     const float* q = s->tri + 9 * t;
     float* L = s->light + 15 * k;
     memcpy(L, q, 9 * sizeof(float));
@@ -1209,7 +1218,101 @@ void orc_scene_destroy(orc_scene* s) {
   free(s->cdf);
   free(s->nodes);
   free(s->prim);
+  free(s->light_tri);
+  free(s->tc);
+  free(s->tid);
+  free(s->vn);
+  free(s->texels);
+  free(s->tex);
   free(s);
+}
+
+static void* dup_or_null(const void* src, size_t bytes) {
+  if (!src) return NULL;
+  void* d = malloc(bytes ? bytes : 1);
+  memcpy(d, src, bytes);
+  return d;
+}
+
+/* Texture coordinates, texture indices (0xFFFFFFFF = the constant white
+ * texture), vertex normals and RGBA8 textures of the scene's triangles, as
+ * the reference's hitgroup and mesh-light records carry them
+ * (RTLibExtOPX7Test.h:1602-1638, RTLibExtOPX7TestApplication.cpp:262-280).
+ * tex_id NULL: no texturing; vnormal NULL: glass shades with the face normal. */
+void orc_scene_set_attributes(orc_scene* s, const float* texcoord, const uint32_t* tex_id, const float* vnormal,
+                              const uint32_t* texels, uint64_t n_texels, const uint32_t* tex, uint32_t n_tex) {
+  free(s->tc);
+  free(s->tid);
+  free(s->vn);
+  free(s->texels);
+  free(s->tex);
+  s->tc = (float*)dup_or_null(tex_id ? texcoord : NULL, sizeof(float) * 6 * (size_t)s->nt);
+  s->tid = (uint32_t*)dup_or_null(tex_id, sizeof(uint32_t) * 4 * (size_t)s->nt);
+  s->vn = (float*)dup_or_null(vnormal, sizeof(float) * 9 * (size_t)s->nt);
+  s->texels = (uint32_t*)dup_or_null(tex_id ? texels : NULL, sizeof(uint32_t) * (size_t)n_texels);
+  s->tex = (uint32_t*)dup_or_null(tex_id ? tex : NULL, sizeof(uint32_t) * 4 * (size_t)n_tex);
+}
+
+/* Filtered fetch of a CUDA texture object with cudaFilterModeLinear,
+ * normalised coordinates and cudaReadModeNormalizedFloat over RGBA8 texels,
+ * the textures of TextureData (RTLibExtOPX7Test.h:396-437, 455-499), as the
+ * CUDA C Programming Guide ("Texture Fetching", "Linear Filtering") states
+ * it: wrap mode replaces u by frac(u), clamp mode clamps to [0, 1];
+ * x = u W - 0.5, i = floor(x), alpha = frac(x) in 9-bit fixed point with 8
+ * fractional bits (rounded to nearest even here); T[i], T[i + 1] with the
+ * index wrapped or clamped; texel channels c / 255;
+ * (1-a)(1-b) T00 + a(1-b) T10 + (1-a) b T01 + a b T11.  Rows are bottom-up
+ * (LoadDefImage flips them, Core/Src/BinaryReader.cpp:768-782). */
+void orc_tex_sample(const uint32_t* texels, const uint32_t* t, float u, float v, float* out3) {
+  if (!isfinite(u)) u = 0.0f;
+  if (!isfinite(v)) v = 0.0f;
+  if (t[3]) {
+    u = u - floorf(u);
+    v = v - floorf(v);
+  } else {
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    v = fminf(fmaxf(v, 0.0f), 1.0f);
+  }
+  const int w = (int)t[1], h = (int)t[2];
+  float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float a = rintf((x - fx) * 256.0f) * (1.0f / 256.0f);
+  float b = rintf((y - fy) * 256.0f) * (1.0f / 256.0f);
+  int i[2] = {(int)fx, (int)fx + 1}, j[2] = {(int)fy, (int)fy + 1};
+  for (int k = 0; k < 2; ++k) {
+    if (t[3]) {
+      i[k] = (i[k] % w + w) % w;
+      j[k] = (j[k] % h + h) % h;
+    } else {
+      i[k] = i[k] < 0 ? 0 : (i[k] >= w ? w - 1 : i[k]);
+      j[k] = j[k] < 0 ? 0 : (j[k] >= h ? h - 1 : j[k]);
+    }
+  }
+  const uint32_t* base = texels + t[0];
+  uint32_t c00 = base[(size_t)j[0] * w + i[0]], c10 = base[(size_t)j[0] * w + i[1]];
+  uint32_t c01 = base[(size_t)j[1] * w + i[0]], c11 = base[(size_t)j[1] * w + i[1]];
+  float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+  for (int k = 0; k < 3; ++k) {
+    int sh = 8 * k;
+    float t00 = (float)((c00 >> sh) & 0xFFu) / 255.0f, t10 = (float)((c10 >> sh) & 0xFFu) / 255.0f;
+    float t01 = (float)((c01 >> sh) & 0xFFu) / 255.0f, t11 = (float)((c11 >> sh) & 0xFFu) / 255.0f;
+    out3[k] = w00 * t00 + w10 * t10 + w01 * t01 + w11 * t11;
+  }
+}
+
+/* Texture coordinate at barycentrics (b0, b1, b2) (GetTexCrd,
+ * SimpleKernel.h:70-77, 251-257) and the factor of texture slot `slot` of
+ * triangle `tri` there (1 without a texture). */
+static v3 tex_factor(const orc_scene* s, int64_t tri, int slot, float b0, float b1, float b2) {
+  if (!s->tid) return V(1, 1, 1);
+  uint32_t id = s->tid[4 * tri + slot];
+  if (id == 0xFFFFFFFFu) return V(1, 1, 1);
+  const float* c = s->tc + 6 * tri;
+  float u = b0 * c[0] + b1 * c[2] + b2 * c[4];
+  float v = b0 * c[1] + b1 * c[3] + b2 * c[5];
+  float o[3];
+  orc_tex_sample(s->texels, s->tex + 4 * (size_t)id, u, v, o);
+  return V(o[0], o[1], o[2]);
 }
 
 static int box_hit(const bnode* n, v3 o, v3 inv, float tmin, float tmax) {
@@ -1229,7 +1332,8 @@ static int box_hit(const bnode* n, v3 o, v3 inv, float tmin, float tmax) {
 }
 
 /* closest (any = 0) or any hit (any = 1); returns triangle or -1 */
-static int64_t trace(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int any, float* t_out, uint64_t* rays) {
+static int64_t trace_uv(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int any, float* t_out, uint64_t* rays,
+                        float* u_out, float* v_out) {
   (*rays)++;
   const float eps = 1e-20f;
   v3 dd = V(fabsf(d.x) > eps ? d.x : copysignf(eps, d.x), fabsf(d.y) > eps ? d.y : copysignf(eps, d.y),
@@ -1250,6 +1354,10 @@ static int64_t trace(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int
         const float* q = s->tri + 9 * t;
         if (orc_isect_tri(ov, dv, q, q + 3, q + 6, tmin, &tb, &u, &v)) {
           best = t;
+          if (u_out) {
+            *u_out = u;
+            *v_out = v;
+          }
           if (any) {
             *t_out = tb;
             return best;
@@ -1263,6 +1371,10 @@ static int64_t trace(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int
   }
   *t_out = tb;
   return best;
+}
+
+static int64_t trace(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int any, float* t_out, uint64_t* rays) {
+  return trace_uv(s, o, d, tmin, tmax, any, t_out, rays, NULL, NULL);
 }
 
 /* ======================================================================
@@ -1419,6 +1531,9 @@ static lsample sample_light(const orc_scene* s, pcg32* r) {
   ls.p = add(add(v0, scl(e1, 1.0f - hi)), scl(e2, hi - lo));
   ls.n = nrm(cross(e1, e2));
   ls.le = ld3(L + 11);
+  /* MeshLight::GetEmission at the sampled point (SimpleKernel.h:81-104):
+   * barycentrics (lo, 1 - hi, hi - lo) */
+  if (s->tid) ls.le = mul(ls.le, tex_factor(s, s->light_tri[li], 3, lo, 1.0f - hi, hi - lo));
   ls.area = L[9];
   ls.pdf = L[10];
   return ls;
@@ -1456,8 +1571,8 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
   vtx vb[64];
   uint32_t nh = 0;
   for (uint32_t depth = 0; depth < p->max_depth; ++depth) {
-    float t;
-    int64_t tri = trace(s, o, d, 1e-4f, 1e20f, 0, &t, &rays);
+    float t, bu = 0.0f, bv = 0.0f;
+    int64_t tri = trace_uv(s, o, d, 1e-4f, 1e20f, 0, &t, &rays, &bu, &bv);
     v3 c = V(0, 0, 0);
     vtx vx;
     memset(&vx, 0, sizeof vx);
@@ -1494,7 +1609,9 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
               w = 0.0f;
             }
           }
-          c = scl(mul(T, ld3(Lr + 11)), w);
+          /* SampleEmission at the hit's texture coordinate */
+          v3 le = mul(ld3(Lr + 11), tex_factor(s, tri, 2, 1.0f - bu - bv, bu, bv));
+          c = scl(mul(T, le), w);
           if (ar) {
             ar->e[nh] = c;
             ar->el[nh] = s->light_of[tri];
@@ -1513,7 +1630,18 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
           spec = ld3(s->spec + 4 * tri);
           param = s->spec[4 * tri + 3];
         }
+        /* SampleDiffuse / SampleSpecular (SimpleKernel.h:251-272); glass keeps
+         * its specular colour in albedo */
+        albedo = mul(albedo, tex_factor(s, tri, type == M_GLASS ? 1 : 0, 1.0f - bu - bv, bu, bv));
+        if (type == M_PHONG) spec = mul(spec, tex_factor(s, tri, 1, 1.0f - bu - bv, bu, bv));
         if (type == M_GLASS) {
+          /* the interpolated vertex normal when the scene has them
+           * (GetTriangleVNormal, SimpleKernel.h:238-250) */
+          if (s->vn) {
+            const float* q9 = s->vn + 9 * tri;
+            float b0 = 1.0f - bu - bv;
+            ng = nrm(add(add(scl(ld3(q9), b0), scl(ld3(q9 + 3), bu)), scl(ld3(q9 + 6), bv)));
+          }
           float ior = param, ci = dot(ng, d), eta;
           v3 rn;
           if (ci < 0.0f) {
@@ -1816,9 +1944,9 @@ void orc_debug_frame(const orc_scene* s, const orc_params* p, uint32_t type, flo
     float dy = 2.0f * ((float)y + jy) / (float)p->height - 1.0f;
     v3 o = ld3(p->eye);
     v3 d = nrm(add(add(scl(ld3(p->u), dx), scl(ld3(p->v), dy)), ld3(p->w)));
-    float t = 0.0f;
+    float t = 0.0f, bu = 0.0f, bv = 0.0f;
     uint64_t rays = 0;
-    int64_t tri = trace(s, o, d, 1e-4f, 1e20f, 0, &t, &rays);
+    int64_t tri = trace_uv(s, o, d, 1e-4f, 1e20f, 0, &t, &rays, &bu, &bv);
     v3 q = V(0, 0, 0);
     if (tri < 0) {
       if (type == 0) q = V(1, 0, 0);
@@ -1831,10 +1959,11 @@ void orc_debug_frame(const orc_scene* s, const orc_params* p, uint32_t type, flo
       int light = s->light_of[tri] >= 0;
       if (type == 0) q = nrm(cross(ld3(tq + 3), ld3(tq + 6)));
       if (type == 1) q = V(t, t, t);
-      if (type == 2 && mt != M_GLASS && !light) q = ld3(al);
-      if (type == 3 && mt == M_GLASS) q = ld3(al);
-      if (type == 3 && mt == M_PHONG && sp) q = ld3(sp);
-      if (type == 4 && light) q = ld3(s->light + 15 * s->light_of[tri] + 11);
+      float b0 = 1.0f - bu - bv;
+      if (type == 2 && mt != M_GLASS && !light) q = mul(ld3(al), tex_factor(s, tri, 0, b0, bu, bv));
+      if (type == 3 && mt == M_GLASS) q = mul(ld3(al), tex_factor(s, tri, 1, b0, bu, bv));
+      if (type == 3 && mt == M_PHONG && sp) q = mul(ld3(sp), tex_factor(s, tri, 1, b0, bu, bv));
+      if (type == 4 && light) q = mul(ld3(s->light + 15 * s->light_of[tri] + 11), tex_factor(s, tri, 2, b0, bu, bv));
       if (type == 5 && mt == M_PHONG && sp) q = V(sp[3], sp[3], sp[3]);
       if (type == 6 && mt == M_GLASS) q = V(al[3], al[3], al[3]);
       if (type == 8) q = add(o, scl(d, t));

@@ -102,6 +102,9 @@ void orc_scene_set_light_pdf(orc_scene* s, const float* pdf);
 orc_scene* orc_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* albedo,
                             const uint32_t* mat, const float* emission, const float* spec);
 void orc_scene_destroy(orc_scene* s);
+void orc_scene_set_attributes(orc_scene* s, const float* texcoord, const uint32_t* tex_id, const float* vnormal,
+                              const uint32_t* texels, uint64_t n_texels, const uint32_t* tex, uint32_t n_tex);
+void orc_tex_sample(const uint32_t* texels, const uint32_t* tex, float u, float v, float* out3);
 typedef struct orc_params {
   uint32_t width, height, flags, max_depth, rr_depth, num_candidates, spp, sample_offset;
   uint64_t seed;

@@ -74,6 +74,8 @@ def lib():
         L.orc_scene_create.restype = C.c_void_p
         L.orc_scene_create.argtypes = [f32p, C.c_uint32, u32p, C.c_uint32, f32p, u32p, f32p, f32p]
         L.orc_scene_set_light_pdf.argtypes = [C.c_void_p, f32p]
+        L.orc_scene_set_attributes.argtypes = [C.c_void_p, f32p, u32p, f32p, u32p, C.c_uint64, u32p, C.c_uint32]
+        L.orc_tex_sample.argtypes = [u32p, u32p, C.c_float, C.c_float, f32p]
         L.orc_scene_destroy.argtypes = [C.c_void_p]
         L.orc_debug_frame.argtypes = [C.c_void_p, C.POINTER(Params), C.c_uint32, f32p, C.POINTER(C.c_int32)]
         L.orc_render.restype = C.c_uint64
@@ -368,12 +370,31 @@ class Scene:
             cnt = dict(zip(uniq.tolist(), counts.tolist()))
             pdf = np.array([1.0 / (len(uniq) * cnt[m]) for m in meshes.tolist()], np.float32)
             L.orc_scene_set_light_pdf(C.c_void_p(self.h), _p(pdf))
+        if sc.get("tex_id") is not None or sc.get("vnormal") is not None:
+            tid = None if sc.get("tex_id") is None else _u(np.asarray(sc["tex_id"]).reshape(-1))
+            tc = None if tid is None else _f(np.asarray(sc["texcoord"]).reshape(-1))
+            vn = None if sc.get("vnormal") is None else _f(np.asarray(sc["vnormal"]).reshape(-1))
+            tx = None if tid is None else _u(np.asarray(sc["texels"]).reshape(-1))
+            td = None if tid is None else _u(np.asarray(sc["tex"]).reshape(-1))
+            self._keep += [tc, tid, vn, tx, td]
+            L.orc_scene_set_attributes(C.c_void_p(self.h), _p(tc), _p(tid, u32p), _p(vn), _p(tx, u32p),
+                                       0 if tx is None else tx.size, _p(td, u32p), 0 if td is None else td.size // 4)
 
     def __del__(self):
         try:
             lib().orc_scene_destroy(self.h)
         except Exception:
             pass
+
+
+def tex_sample(texels, tex, u, v):
+    """Filtered RGB of one RGBA8 texture (first texel, width, height, wrap) at
+    normalised (u, v)."""
+    tx = _u(np.asarray(texels).reshape(-1))
+    td = _u(np.asarray(tex).reshape(4))
+    out = np.zeros(3, np.float32)
+    lib().orc_tex_sample(_p(tx, u32p), _p(td, u32p), float(u), float(v), _p(out))
+    return out
 
 
 def render(scene: Scene, camera_uvw, width, height, spp, flags, max_depth=8, rr_depth=3, num_candidates=32, seed=0,

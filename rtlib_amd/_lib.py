@@ -65,6 +65,7 @@ SIGNATURES = {
     "rtpg_synchronize": [c_vp],
     "rtpg_scene_upload": [c_vp, c_fp, C.c_uint32, c_u32p, C.c_uint32, c_fp, c_u32p, c_fp, c_fp, C.c_uint32, c_u32p],
     "rtpg_scene_set_albedo": [c_vp, c_fp],
+    "rtpg_scene_attributes": [c_vp, c_fp, c_u32p, c_fp, c_u32p, C.c_uint64, c_u32p, C.c_uint32],
     "rtpg_scene_info": [c_vp, c_u32p, c_fp],
     "rtpg_camera_set": [c_vp, c_fp, c_fp, c_fp, c_fp],
     "rtpg_frame_resize": [c_vp, C.c_uint32, C.c_uint32],

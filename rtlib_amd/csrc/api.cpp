@@ -58,6 +58,10 @@ SceneView rtpg_context_t::scene_view() const {
   v.n_tris = n_tris;
   v.n_nodes = n_nodes;
   v.max_stack = max_stack;
+  v.tattr = has_tattr ? tattr.p : nullptr;
+  v.vnorm = has_vnorm ? vnorm.p : nullptr;
+  v.tex = has_tattr ? tex.p : nullptr;
+  v.texels = has_tattr ? texels.p : nullptr;
   return v;
 }
 
@@ -198,6 +202,11 @@ int rtpg_scene_upload(rtpg_context ctx, const float* verts, uint32_t nv, const u
     ctx->light_cdf.upload(cdf, s);
     RTPG_HIP(hipStreamSynchronize(s));
     ctx->has_spec = spec != nullptr;
+    ctx->has_tattr = ctx->has_vnorm = false;
+    ctx->tattr.release();
+    ctx->vnorm.release();
+    ctx->tex.release();
+    ctx->texels.release();
     ctx->lights_host = lights;
     ctx->light_tri = light_tris;
     ctx->order = b.order;
@@ -212,6 +221,69 @@ int rtpg_scene_upload(rtpg_context ctx, const float* verts, uint32_t nv, const u
       ctx->world[k] = b.world_lo[k];
       ctx->world[3 + k] = b.world_hi[k];
     }
+  });
+}
+
+int rtpg_scene_attributes(rtpg_context ctx, const float* texcoord, const uint32_t* tex_id, const float* vnormal,
+                          const uint32_t* texels, uint64_t n_texels, const uint32_t* tex, uint32_t n_tex) {
+  return guarded([&] {
+    require(ctx != nullptr, "NULL argument");
+    require(ctx->n_tris > 0, "no scene");
+    require(tex_id == nullptr || texcoord != nullptr, "tex_id needs texcoord");
+    require(n_tex < kTexNone, "at most 65535 textures");
+    require(n_tex == 0 || (tex != nullptr && texels != nullptr), "textures need tex and texels");
+    for (uint32_t k = 0; k < n_tex; ++k) {
+      const uint32_t* t = tex + 4 * (size_t)k;
+      require(t[1] > 0 && t[2] > 0 && t[1] <= 32768u && t[2] <= 32768u, "texture size must be 1..32768");
+      require(t[3] <= 1u, "texture address mode must be RTPG_TEX_CLAMP or RTPG_TEX_WRAP");
+      require((uint64_t)t[0] + (uint64_t)t[1] * t[2] <= n_texels, "texture texels out of range");
+    }
+    const uint32_t nt = ctx->n_tris;
+    RTPG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    bool any_tex = false;
+    if (tex_id) {
+      for (size_t k = 0; k < 4 * (size_t)nt; ++k) {
+        require(tex_id[k] == RTPG_TEX_NONE || tex_id[k] < n_tex, "texture index out of range");
+        any_tex |= tex_id[k] != RTPG_TEX_NONE;
+      }
+    }
+    // leaf order; 16-bit texture indices, kTexNone = factor 1
+    if (any_tex) {
+      std::vector<float4> ta(2 * (size_t)nt);
+      auto id16 = [&](size_t k) { return tex_id[k] == RTPG_TEX_NONE ? kTexNone : tex_id[k]; };
+      for (uint32_t sl = 0; sl < nt; ++sl) {
+        size_t t = ctx->order[sl];
+        const float* c = texcoord + 6 * t;
+        ta[2 * (size_t)sl] = make_float4(c[0], c[1], c[2], c[3]);
+        ta[2 * (size_t)sl + 1] = make_float4(c[4], c[5], bitsf(id16(4 * t) | id16(4 * t + 1) << 16),
+                                             bitsf(id16(4 * t + 2) | id16(4 * t + 3) << 16));
+      }
+      std::vector<uint4> td(n_tex);
+      for (uint32_t k = 0; k < n_tex; ++k) td[k] = make_uint4(tex[4 * k], tex[4 * k + 1], tex[4 * k + 2], tex[4 * k + 3]);
+      ctx->tattr.upload(ta, s);
+      ctx->tex.upload(td, s);
+      ctx->texels.upload(texels, (size_t)n_texels, s);
+    } else {
+      ctx->tattr.release();
+      ctx->tex.release();
+      ctx->texels.release();
+    }
+    if (vnormal) {
+      std::vector<float4> vn(3 * (size_t)nt);
+      for (uint32_t sl = 0; sl < nt; ++sl) {
+        const float* q = vnormal + 9 * (size_t)ctx->order[sl];
+        vn[3 * (size_t)sl] = make_float4(q[0], q[1], q[2], q[3]);
+        vn[3 * (size_t)sl + 1] = make_float4(q[4], q[5], q[6], q[7]);
+        vn[3 * (size_t)sl + 2] = make_float4(q[8], 0.0f, 0.0f, 0.0f);
+      }
+      ctx->vnorm.upload(vn, s);
+    } else {
+      ctx->vnorm.release();
+    }
+    RTPG_HIP(hipStreamSynchronize(s));
+    ctx->has_tattr = any_tex;
+    ctx->has_vnorm = vnormal != nullptr;
   });
 }
 

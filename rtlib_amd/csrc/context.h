@@ -23,6 +23,11 @@ struct rtpg_context_t {
   rtpg::DBuf<float4> shade, spec, lights;
   rtpg::DBuf<uint32_t> info;
   rtpg::DBuf<float> light_cdf;
+  // optional texture coordinates + texture indices, vertex normals, textures
+  rtpg::DBuf<float4> tattr, vnorm;
+  rtpg::DBuf<uint4> tex;
+  rtpg::DBuf<uint32_t> texels;
+  bool has_tattr = false, has_vnorm = false;
   std::vector<uint32_t> order;       // leaf slot -> original triangle
   std::vector<uint32_t> slot_of;     // original triangle -> leaf slot
   std::vector<float> shade_host;     // leaf order, 4 per triangle

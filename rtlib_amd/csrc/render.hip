@@ -331,6 +331,7 @@ struct LightSample {
 };
 
 // Uniform point on a light triangle, barycentrics of Random.h:290-296.
+template <bool kTex>
 __device__ __forceinline__ LightSample sample_light(const SceneView& sc, Pcg32& rng) {
   float ul = rnd(rng);
   float u1 = rnd(rng), u2 = rnd(rng);
@@ -343,6 +344,18 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& sc, Pcg32& 
   s.p = v0 + e1 * (1.0f - hi) + e2 * (hi - lo);
   s.n = normalize(cross(e1, e2));
   s.Le = V3{e.x, e.y, e.z};
+  if (kTex && sc.tattr) {
+    // MeshLight::GetEmission at the sampled point's texture coordinate
+    // (SimpleKernel.h:81-104): barycentrics (lo, 1 - hi, hi - lo)
+    uint32_t tri = fbits(c.w);
+    float4 ta = sc.tattr[2u * tri], tb = sc.tattr[2u * tri + 1u];
+    uint32_t id = fbits(tb.w) >> 16;
+    if (id != kTexNone) {
+      float tu, tv;
+      tri_texcoord(ta, tb, lo, 1.0f - hi, hi - lo, tu, tv);
+      s.Le = s.Le * tex_sample(sc.tex[id], sc.texels, tu, tv);
+    }
+  }
   s.area = a.w;
   s.pdf_sel = b.w;
   s.idx = li;
@@ -453,13 +466,16 @@ __device__ __forceinline__ void block_append2(uint32_t* ca, bool pa, uint32_t* c
 // kRis instantiates the RIS light selection; without it the kernel needs
 // fewer registers and runs at a higher occupancy.
 // SD-tree shading is held to 80 VGPRs (6 waves per SIMD, as plain shading).
-template <int kGuide, bool kAdj, bool kRis>
+// kTex instantiates texture and vertex-normal reads (scenes with attributes
+// only: the code costs registers everywhere else).
+template <int kGuide, bool kAdj, bool kRis, bool kTex>
 constexpr unsigned shade_min_waves() {
-  return (kGuide == 1 && !kAdj && !kRis) ? 6u : 1u;
+  return (kGuide == 1 && !kAdj && !kRis && !kTex) ? 6u : 1u;
 }
 
-template <int kGuide, bool kAdj = false, bool kRis = false>
-__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(shade_min_waves<kGuide, kAdj, kRis>())))
+template <int kGuide, bool kAdj = false, bool kRis = false, bool kTex = false>
+__global__ void __launch_bounds__(kShadeBlock)
+__attribute__((amdgpu_waves_per_eu(shade_min_waves<kGuide, kAdj, kRis, kTex>())))
 k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView hg, MortonView mq, uint32_t depth) {
   __shared__ LdsAcc3<kAdj ? 1024u : 1u> acc_alb;
   __shared__ LdsAcc3<kAdj ? 256u : 1u> acc_le;
@@ -529,7 +545,15 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
               w = 0.0f;
             }
           }
-          c = T * V3{e.x, e.y, e.z} * w;
+          V3 le{e.x, e.y, e.z};
+          if (kTex && sc.tattr) {
+            // SampleEmission at the hit's texture coordinate
+            float4 ta = sc.tattr[2u * tri], tb = sc.tattr[2u * tri + 1u];
+            float tu, tv;
+            tri_texcoord(ta, tb, 1.0f - hit.y - hit.z, hit.y, hit.z, tu, tv);
+            le = le * tex_factor(sc, fbits(tb.w) & 0xFFFFu, tu, tv);
+          }
+          c = T * le * w;
           if (adj) adj_add_le(acc_le, ps, li, adj_w, c, V3{e.x, e.y, e.z});
         }
       } else if (depth + 1u < rp.max_depth) {
@@ -552,8 +576,27 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
           spec = V3{s4.x, s4.y, s4.z};
           param = s4.w;
         }
+        float bu = hit.y, bv = hit.z;
+        if (kTex && sc.tattr) {
+          // SampleDiffuse / SampleSpecular at the hit's texture coordinate
+          // (SimpleKernel.h:251-272); glass keeps its specular colour in albedo
+          float4 ta = sc.tattr[2u * tri], tb = sc.tattr[2u * tri + 1u];
+          float tu, tv;
+          tri_texcoord(ta, tb, 1.0f - bu - bv, bu, bv, tu, tv);
+          uint32_t ids = fbits(tb.z);
+          albedo = albedo * tex_factor(sc, type == kMatGlass ? ids >> 16 : ids & 0xFFFFu, tu, tv);
+          if (type == kMatPhong) spec = spec * tex_factor(sc, ids >> 16, tu, tv);
+        }
         if (type == kMatGlass) {
-          // Dielectric of SimpleTrace.cu:347-407 (Schlick Fresnel), a delta event.
+          // Dielectric of SimpleTrace.cu:347-407 (Schlick Fresnel), a delta
+          // event; it shades with the interpolated vertex normal when the
+          // scene has one (GetTriangleVNormal, SimpleKernel.h:238-250)
+          if (kTex && sc.vnorm) {
+            const float4* vq = sc.vnorm + 3u * tri;
+            float4 q0 = vq[0], q1 = vq[1], q2 = vq[2];
+            float b0 = 1.0f - bu - bv;
+            ng = normalize(V3{q0.x, q0.y, q0.z} * b0 + V3{q0.w, q1.x, q1.y} * bu + V3{q1.z, q1.w, q2.x} * bv);
+          }
           float ior = param;
           float cos_i = dot(ng, d);
           V3 rn;
@@ -630,7 +673,7 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
               uint32_t m = 0;
               V3 f_y{0, 0, 0}, dir_y{0, 0, 0};
               for (uint32_t k = 0; k < rp.num_candidates; ++k) {
-                LightSample ls = sample_light(sc, rng);
+                LightSample ls = sample_light<kTex>(sc, rng);
                 V3 dl = ls.p - p;
                 float dist = length(dl);
                 V3 dir = dl / dist;
@@ -660,7 +703,7 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
                 shadow = max3(scon) > 0.0f;
               }
             } else {
-              LightSample ls = sample_light(sc, rng);
+              LightSample ls = sample_light<kTex>(sc, rng);
               V3 dl = ls.p - p;
               float dist2 = dot(dl, dl);
               float dist = sqrtf(dist2);
@@ -957,6 +1000,15 @@ __global__ void __launch_bounds__(kBlock) k_debug(RenderParams rp, Camera cam, S
     float4 sh = sc.shade[tri];
     uint32_t info = sc.info[tri];
     uint32_t mt = tri_type(info);
+    // texture factors of the diffuse, specular and emission slots
+    float tu = 0.0f, tv = 0.0f;
+    uint32_t ids = 0xFFFFFFFFu, ide = 0xFFFFFFFFu;
+    if (sc.tattr && type >= 2u && type <= 4u) {
+      float4 ta = sc.tattr[2u * tri], tb = sc.tattr[2u * tri + 1u];
+      tri_texcoord(ta, tb, 1.0f - u - v, u, v, tu, tv);
+      ids = fbits(tb.z);
+      ide = fbits(tb.w);
+    }
     switch (type) {
       case 0:
         q = normalize(cross(load3(tq[1]), load3(tq[2])));
@@ -965,20 +1017,20 @@ __global__ void __launch_bounds__(kBlock) k_debug(RenderParams rp, Camera cam, S
         q = V3{t, t, t};
         break;
       case 2:
-        if (mt != kMatGlass && !tri_is_light(info)) q = V3{sh.x, sh.y, sh.z};
+        if (mt != kMatGlass && !tri_is_light(info)) q = V3{sh.x, sh.y, sh.z} * tex_factor(sc, ids & 0xFFFFu, tu, tv);
         break;
       case 3:
         if (mt == kMatGlass) {
-          q = V3{sh.x, sh.y, sh.z};
+          q = V3{sh.x, sh.y, sh.z} * tex_factor(sc, ids >> 16, tu, tv);
         } else if (mt == kMatPhong && sc.spec) {
           float4 s4 = sc.spec[tri];
-          q = V3{s4.x, s4.y, s4.z};
+          q = V3{s4.x, s4.y, s4.z} * tex_factor(sc, ids >> 16, tu, tv);
         }
         break;
       case 4:
         if (tri_is_light(info)) {
           float4 e = sc.lights[4u * tri_light(info) + 3u];
-          q = V3{e.x, e.y, e.z};
+          q = V3{e.x, e.y, e.z} * tex_factor(sc, ide & 0xFFFFu, tu, tv);
         }
         break;
       case 5:
@@ -1406,16 +1458,23 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     const bool adjoint = ps.grad_alb != nullptr;
     const bool ris = (rp.flags & kFlagRis) != 0;
     const uint32_t nbs = blocks_for(ps.n, kShadeBlock);
-#define RTPG_SHADE(G, A)                                                                                      \
-  if (ris) {                                                                                                  \
-    hipLaunchKernelGGL((k_shade<G, A, true>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);  \
-  } else {                                                                                                    \
-    hipLaunchKernelGGL((k_shade<G, A, false>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d); \
+    const bool tex = sc.tattr != nullptr || sc.vnorm != nullptr;
+#define RTPG_SHADE_T(G, A, R)                                                                                       \
+  if (tex) {                                                                                                        \
+    hipLaunchKernelGGL((k_shade<G, A, R, true>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);  \
+  } else {                                                                                                          \
+    hipLaunchKernelGGL((k_shade<G, A, R, false>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d); \
+  }
+#define RTPG_SHADE(G, A)    \
+  if (ris) {                \
+    RTPG_SHADE_T(G, A, true)  \
+  } else {                  \
+    RTPG_SHADE_T(G, A, false) \
   }
     if (adjoint) {
       // both passes of the adjoint run this one instantiation, so the replay
       // reproduces the primal paths operation for operation (RIS is refused)
-      hipLaunchKernelGGL((k_shade<0, true, false>), dim3(nbs), dim3(kShadeBlock), 0, s, rp, sc, ps, sd, hg, mq, d);
+      RTPG_SHADE_T(0, true, false)
     } else if (guide == 1) {
       RTPG_SHADE(1, false)
     } else if (guide == 2) {
@@ -1431,6 +1490,7 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
       RTPG_SHADE(0, false)
     }
 #undef RTPG_SHADE
+#undef RTPG_SHADE_T
     mark(1, d, 1);
     mark(2, d, 0);
     if (adjoint) {

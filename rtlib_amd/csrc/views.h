@@ -31,7 +31,79 @@ struct SceneView {
   uint32_t n_tris;
   uint32_t n_nodes;
   uint32_t max_stack;
+  // Optional per-triangle attributes, leaf order (null when absent).
+  // tattr: 2 x float4 per triangle, (u0, v0, u1, v1) and (u2, v2, texture
+  // words): word z = diffuse | specular << 16, word w = emission on hit |
+  // emission when light-sampled << 16, 16-bit indices, kTexNone = factor 1.
+  const float4* tattr;
+  const float4* vnorm;     // 3 x float4 per triangle: the 9 vertex-normal floats in order
+  const uint4* tex;        // per texture: first texel, width, height, wrap (1) / clamp (0)
+  const uint32_t* texels;  // RGBA8, R in the low byte, rows bottom-up
 };
+
+constexpr uint32_t kTexNone = 0xFFFFu;
+
+// Texture coordinate at barycentrics (b0, b1, b2) of vertices 0, 1, 2, in
+// the reference's order of terms (GetTexCrd, SimpleKernel.h:70-77, 251-257).
+RTPG_HD void tri_texcoord(float4 a, float4 b, float b0, float b1, float b2, float& u, float& v) {
+#pragma clang fp contract(off)
+  u = b0 * a.x + b1 * a.z + b2 * b.x;
+  v = b0 * a.y + b1 * a.w + b2 * b.y;
+}
+
+// Bilinear fetch of an RGBA8 texture at normalised (u, v): the filtering of
+// a CUDA texture object with linear filter, normalised coordinates and
+// normalised-float reads (CUDA C Programming Guide, "Texture Fetching"):
+// x = u W - 1/2, i = floor(x), alpha = frac(x) held with 8 fractional bits
+// (rounded to nearest), texel indices wrapped (wrap mode, after u -> frac(u))
+// or clamped to the edge, texel channels c / 255.  Returns rgb.
+RTPG_HD V3 tex_sample(uint4 t, const uint32_t* texels, float u, float v) {
+#pragma clang fp contract(off)
+  if (!isfinite(u)) u = 0.0f;
+  if (!isfinite(v)) v = 0.0f;
+  if (t.w) {
+    u = u - floorf(u);
+    v = v - floorf(v);
+  } else {
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    v = fminf(fmaxf(v, 0.0f), 1.0f);
+  }
+  const int w = (int)t.y, h = (int)t.z;
+  float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float a = rintf((x - fx) * 256.0f) * (1.0f / 256.0f);
+  float b = rintf((y - fy) * 256.0f) * (1.0f / 256.0f);
+  int i0 = (int)fx, j0 = (int)fy, i1 = i0 + 1, j1 = j0 + 1;
+  if (t.w) {
+    i0 = (i0 % w + w) % w;
+    i1 = (i1 % w + w) % w;
+    j0 = (j0 % h + h) % h;
+    j1 = (j1 % h + h) % h;
+  } else {
+    i0 = i0 < 0 ? 0 : (i0 >= w ? w - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 >= w ? w - 1 : i1);
+    j0 = j0 < 0 ? 0 : (j0 >= h ? h - 1 : j0);
+    j1 = j1 < 0 ? 0 : (j1 >= h ? h - 1 : j1);
+  }
+  const uint32_t* base = texels + t.x;
+  uint32_t c00 = base[(size_t)j0 * w + i0], c10 = base[(size_t)j0 * w + i1];
+  uint32_t c01 = base[(size_t)j1 * w + i0], c11 = base[(size_t)j1 * w + i1];
+  float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+  float r[3];
+  for (int k = 0; k < 3; ++k) {
+    int sh = 8 * k;
+    float t00 = (float)((c00 >> sh) & 0xFFu) / 255.0f, t10 = (float)((c10 >> sh) & 0xFFu) / 255.0f;
+    float t01 = (float)((c01 >> sh) & 0xFFu) / 255.0f, t11 = (float)((c11 >> sh) & 0xFFu) / 255.0f;
+    r[k] = w00 * t00 + w10 * t10 + w01 * t01 + w11 * t11;
+  }
+  return V3{r[0], r[1], r[2]};
+}
+
+// Factor of texture slot `id` (kTexNone: 1).
+RTPG_HD V3 tex_factor(const SceneView& sc, uint32_t id, float u, float v) {
+  if (id == kTexNone) return V3{1.0f, 1.0f, 1.0f};
+  return tex_sample(sc.tex[id], sc.texels, u, v);
+}
 
 // Flags of a render launch.  The low seven bits keep the meaning of the
 // reference's ParamFlag (SimpleKernel.h:149-159).

@@ -11,6 +11,11 @@
   transforms, the camera controller and the trace config.
 - Material types as SpecifyMaterialType (Test/RTLibExtOPX7Test.h:1572-1601):
   emissive -> light, illum 7 -> glass, otherwise Phong.
+- Texture coordinates, vertex normals and texture maps (map_Kd / map_Ks /
+  map_Ke) as the hitgroup and mesh-light records carry them
+  (RTLibExtOPX7Test.h:386-511, 1602-1638; RTLibExtOPX7TestApplication.cpp:
+  262-280); a missing texture coordinate is (0.5, 0.5) and a missing normal
+  (0, 1, 0), RTLib's defaults (Core/Src/BinaryReader.cpp:323-349).
 
 Meshes are flattened into one triangle list in world space (the tracer has a
 single BVH over all instances); every emissive mesh instance becomes one light
@@ -91,12 +96,19 @@ def specify_material_type(mat):
 @dataclass
 class Mesh:
     """A MeshUniqueResource: triangles (indices into the model's vertices),
-    per-triangle index into `materials` (global material indices)."""
+    per-triangle index into `materials` (global material indices), and per
+    corner the index of its texture coordinate and normal (-1: none)."""
     name: str
     tris: np.ndarray
     mat_local: np.ndarray
     materials: list
     has_light: bool = False
+    tc: np.ndarray = None
+    nrm: np.ndarray = None
+
+    def select(self, sel, name, mat_local, materials, has_light):
+        return Mesh(name, self.tris[sel], mat_local, materials, has_light,
+                    None if self.tc is None else self.tc[sel], None if self.nrm is None else self.nrm[sel])
 
 
 @dataclass
@@ -104,6 +116,8 @@ class ObjModel:
     verts: np.ndarray
     meshes: dict = field(default_factory=dict)    # name -> Mesh, in file order
     materials: list = field(default_factory=list)  # RTLib material dicts
+    texcoords: np.ndarray = None                   # vt, N x 2
+    normals: np.ndarray = None                     # vn, N x 3
 
     def mesh_materials(self, mesh):
         return [self.materials[i] for i in mesh.materials]
@@ -114,14 +128,24 @@ def _face_index(tok, nv):
     return i - 1 if i > 0 else nv + i
 
 
+def _corner_index(tok, k, n):
+    """Index k (1: texture coordinate, 2: normal) of a face corner `v/vt/vn`,
+    -1 when absent."""
+    parts = tok.split("/")
+    if len(parts) <= k or parts[k] == "":
+        return -1
+    i = int(parts[k])
+    return i - 1 if i > 0 else n + i
+
+
 def load_obj(path, split_light=True):
     """OBJ (+ its MTL libraries) -> ObjModel.  Polygons are triangulated as a
     fan from their first vertex.  Faces without `usemtl` take tinyobjloader's
     default material.  Groups of the same name are merged."""
     base = os.path.dirname(os.path.abspath(path))
-    verts = []
+    verts, vts, vns = [], [], []
     materials, mat_index = [], {}
-    groups = {}   # name -> [tris list, mats list]
+    groups = {}   # name -> [tris, mats, texcoord corners, normal corners]
     order = []
     cur_name = ""
     cur_mat = -1
@@ -136,17 +160,25 @@ def load_obj(path, split_light=True):
             if key == "v":
                 verts.append((float(tok[1]), float(tok[2]), float(tok[3])))
                 nv += 1
+            elif key == "vt":
+                vts.append((float(tok[1]), float(tok[2]) if len(tok) > 2 else 0.0))
+            elif key == "vn":
+                vns.append((float(tok[1]), float(tok[2]), float(tok[3])))
             elif key == "f":
                 idx = [_face_index(t, nv) for t in tok[1:]]
                 if len(idx) < 3:
                     continue
+                ti = [_corner_index(t, 1, len(vts)) for t in tok[1:]]
+                ni = [_corner_index(t, 2, len(vns)) for t in tok[1:]]
                 if cur_name not in groups:
-                    groups[cur_name] = ([], [])
+                    groups[cur_name] = ([], [], [], [])
                     order.append(cur_name)
                 g = groups[cur_name]
                 for k in range(1, len(idx) - 1):
                     g[0].append((idx[0], idx[k], idx[k + 1]))
                     g[1].append(cur_mat)
+                    g[2].append((ti[0], ti[k], ti[k + 1]))
+                    g[3].append((ni[0], ni[k], ni[k + 1]))
             elif key in ("o", "g"):
                 cur_name = " ".join(tok[1:]) if len(tok) > 1 else ""
             elif key == "usemtl":
@@ -164,11 +196,17 @@ def load_obj(path, split_light=True):
         mat_index.setdefault("default", len(materials))
         materials.append(default_material())
         dflt = len(materials) - 1
-    model = ObjModel(v, {}, materials)
+    model = ObjModel(v, {}, materials, np.asarray(vts, np.float32).reshape(-1, 2),
+                     np.asarray(vns, np.float32).reshape(-1, 3))
     for name in order:
         tris = np.asarray(groups[name][0], np.int64).reshape(-1, 3)
         if tris.size and (tris.min() < 0 or tris.max() >= v.shape[0]):
             raise ValueError(f"{path}: face index out of range in group '{name}'")
+        tc = np.asarray(groups[name][2], np.int64).reshape(-1, 3)
+        nrm = np.asarray(groups[name][3], np.int64).reshape(-1, 3)
+        # corners naming a missing vt / vn take RTLib's defaults
+        tc[(tc < 0) | (tc >= len(vts))] = -1
+        nrm[(nrm < 0) | (nrm >= len(vns))] = -1
         gm = [dflt if m == -1 else m for m in groups[name][1]]
         # per-mesh material list in order of first use (BinaryReader.cpp:356-371)
         local, mats = {}, []
@@ -178,7 +216,7 @@ def load_obj(path, split_light=True):
                 local[m] = len(mats)
                 mats.append(m)
             ml[j] = local[m]
-        model.meshes[name] = Mesh(name, tris.astype(np.uint32), ml, mats)
+        model.meshes[name] = Mesh(name, tris.astype(np.uint32), ml, mats, False, tc, nrm)
     if split_light:
         split_lights(model)
     return model
@@ -201,7 +239,7 @@ def split_lights(model):
                         local[int(i)] = len(mats)
                         mats.append(m.materials[int(i)])
                     ml[j] = local[int(i)]
-                out[name + suffix] = Mesh(name + suffix, m.tris[sel], ml, mats, light)
+                out[name + suffix] = m.select(sel, name + suffix, ml, mats, light)
         else:
             m.has_light = bool(all(emit)) and len(emit) > 0
             out[name] = m
@@ -210,8 +248,7 @@ def split_lights(model):
         if m.has_light and len(m.materials) > 1:
             for i, g in enumerate(m.materials):
                 sel = m.mat_local == i
-                final[f"{name}.{i}"] = Mesh(f"{name}.{i}", m.tris[sel], np.zeros(int(sel.sum()), np.uint32), [g],
-                                            True)
+                final[f"{name}.{i}"] = m.select(sel, f"{name}.{i}", np.zeros(int(sel.sum()), np.uint32), [g], True)
         else:
             final[name] = m
     model.meshes = final
@@ -297,11 +334,48 @@ def _material_arrays(mats):
     return rows
 
 
-def load_scene(path_or_json, base_dir=None, nee_threshold=None):
+TEX_NONE = 0xFFFFFFFF
+
+
+def load_texture(path):
+    """Image file -> RGBA8 texels (H x W uint32, R in the low byte), rows
+    bottom-up as LoadDefImage stores them (Core/Src/BinaryReader.cpp:768-782,
+    stbi_load with 4 channels: grey -> (g, g, g, 255), 16-bit -> high byte)."""
+    from PIL import Image
+
+    with Image.open(path) as im:
+        if im.mode in ("I;16", "I;16B", "I;16L", "I"):
+            g = (np.asarray(im, np.uint32) >> 8).astype(np.uint8)
+            a = np.stack([g, g, g, np.full_like(g, 255)], axis=-1)
+        else:
+            a = np.asarray(im.convert("RGBA"), np.uint8)
+    a = a[::-1].astype(np.uint32)
+    return np.ascontiguousarray(a[..., 0] | (a[..., 1] << 8) | (a[..., 2] << 16) | (a[..., 3] << 24))
+
+
+# texture of each slot (diffuse, specular, emission on hit, emission when
+# light-sampled): "reference" as the reference binds them -- its hitgroup
+# record takes the specular slot from emitTex and the emission slot from
+# specTex (RTLibExtOPX7Test.h:1634-1636), its mesh lights emitTex
+# (RTLibExtOPX7TestApplication.cpp:272-277); "named" by the MTL keys.
+_TEX_SLOTS = {"reference": ("diffTex", "emitTex", "specTex", "emitTex"),
+              "named": ("diffTex", "specTex", "emitTex", "emitTex")}
+
+
+def load_scene(path_or_json, base_dir=None, nee_threshold=None, texture_slots="reference", missing_normal="reference"):
     """Scene JSON (file path or dict) -> (scene dict, light_mesh, config dict).
 
-    scene: the arrays of rtpg_scene_upload plus `camera`; light_mesh: light
-    mesh id per triangle (for light_mode 1); config: the JSON's Config."""
+    scene: the arrays of rtpg_scene_upload plus `camera`, and when the scene
+    has texture maps `texcoord`, `tex_id`, `texels`, `tex`, when it has glass
+    `vnormal` (rtpg_scene_attributes); light_mesh: light mesh id per triangle
+    (for light_mode 1); config: the JSON's Config.  missing_normal: a corner
+    without `vn` takes (0, 1, 0) ("reference", RTLib's default) or the face
+    normal ("face")."""
+    if texture_slots not in _TEX_SLOTS:
+        raise ValueError(f"texture_slots must be one of {sorted(_TEX_SLOTS)}")
+    if missing_normal not in ("reference", "face"):
+        raise ValueError("missing_normal must be 'reference' or 'face'")
+    slot_keys = _TEX_SLOTS[texture_slots]
     if isinstance(path_or_json, (str, os.PathLike)):
         base_dir = base_dir or os.path.dirname(os.path.abspath(path_or_json))
         with open(path_or_json, "r", encoding="utf-8") as f:
@@ -334,9 +408,25 @@ def load_scene(path_or_json, base_dir=None, nee_threshold=None):
             out.append((model, mesh, mats))
         return out
 
-    V, I, ALB, MAT, EM, SP, LM = [], [], [], [], [], [], []
+    V, I, ALB, MAT, EM, SP, LM, TC, NR, TID = [], [], [], [], [], [], [], [], [], []
     nv = 0
     n_light = 0
+    tex_of, tex_rows, texel_chunks = {}, [], []
+    n_texels = 0
+
+    def tex_index(path):
+        nonlocal n_texels
+        if not path:
+            return TEX_NONE
+        if path not in tex_of:
+            if not os.path.exists(path):
+                raise ValueError(f"texture '{path}' not found")
+            img = load_texture(path)
+            tex_of[path] = len(tex_rows)
+            tex_rows.append((n_texels, img.shape[1], img.shape[0], 1))  # LoadFromPath: wrap addressing
+            texel_chunks.append(img.ravel())
+            n_texels += img.size
+        return tex_of[path]
 
     def emit(gas_name, xf):
         nonlocal nv, n_light
@@ -352,8 +442,30 @@ def load_scene(path_or_json, base_dir=None, nee_threshold=None):
                 V.append(p.astype(np.float32))
                 I.append((remap[mesh.tris] + nv).astype(np.uint32))
                 nv += used.size
+                # corner attributes: texture coordinates and normals (world space:
+                # the inverse transpose of the instance transform)
+                nt_m = mesh.tris.shape[0]
+                tci = mesh.tc if mesh.tc is not None else np.full((nt_m, 3), -1, np.int64)
+                tcs = np.full((nt_m, 3, 2), 0.5, np.float32)
+                if model.texcoords is not None and model.texcoords.shape[0]:
+                    ok = tci >= 0
+                    tcs[ok] = model.texcoords[tci[ok]]
+                TC.append(tcs)
+                nri = mesh.nrm if mesh.nrm is not None else np.full((nt_m, 3), -1, np.int64)
+                nrs = np.zeros((nt_m, 3, 3), np.float64)
+                nrs[...] = (0.0, 1.0, 0.0)
+                if missing_normal == "face":
+                    q = model.verts[mesh.tris.astype(np.int64)].astype(np.float64)
+                    nrs[...] = np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0])[:, None, :]
+                if model.normals is not None and model.normals.shape[0]:
+                    ok = nri >= 0
+                    nrs[ok] = model.normals[nri[ok]]
+                nx = np.linalg.inv(xf[:3, :3]).T
+                NR.append((nrs @ nx.T).astype(np.float32))
+                ids = np.array([[tex_index(m.get(k, "")) for k in slot_keys] for m in mats], np.uint32).reshape(-1, 4)
                 rows = _material_arrays(mats)
                 ml = mesh.mat_local
+                TID.append(ids[ml])
                 MAT.append(np.array([rows[i][0] for i in ml], np.uint32))
                 ALB.append(np.array([rows[i][1] for i in ml], np.float32).reshape(-1, 4))
                 SP.append(np.array([rows[i][2] for i in ml], np.float32).reshape(-1, 4))
@@ -389,6 +501,13 @@ def load_scene(path_or_json, base_dir=None, nee_threshold=None):
         "spec": np.ascontiguousarray(np.concatenate(SP), np.float32),
         "camera": cc.camera(w, h),
     }
+    if tex_rows:
+        scene["texcoord"] = np.ascontiguousarray(np.concatenate(TC), np.float32)
+        scene["tex_id"] = np.ascontiguousarray(np.concatenate(TID), np.uint32)
+        scene["texels"] = np.ascontiguousarray(np.concatenate(texel_chunks), np.uint32)
+        scene["tex"] = np.asarray(tex_rows, np.uint32).reshape(-1, 4)
+    if np.any(scene["mat"] == MAT_GLASS):
+        scene["vnormal"] = np.ascontiguousarray(np.concatenate(NR), np.float32)
     return scene, np.ascontiguousarray(np.concatenate(LM), np.uint32), cfg
 
 
@@ -431,12 +550,20 @@ def save_obj(scene, light_mesh, obj_path, name="scene"):
                 raise ValueError(f"material type {t} has no MTL form")
     v = np.asarray(scene["verts"], np.float32)
     idx = np.asarray(scene["idx"], np.int64)
+    # one normal per triangle (its face normal) so that glass, which shades
+    # with vertex normals, reads back flat
+    q = v[idx].astype(np.float64)
+    fn = np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0])
+    fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-30)
     with open(obj_path, "w") as f:
         f.write(f"mtllib {os.path.basename(mtl_path)}\n")
         for p in v:
             f.write(f"v {c3(*p)}\n")
+        for n in fn:
+            f.write(f"vn {c3(*n)}\n")
         for k in range(uniq.shape[0]):
             f.write(f"g {name}_{k}\nusemtl m{k}\n")
-            for t in idx[inv == k] + 1:
-                f.write(f"f {t[0]} {t[1]} {t[2]}\n")
+            for t in np.nonzero(inv == k)[0]:
+                a, b, c = idx[t] + 1
+                f.write(f"f {a}//{t + 1} {b}//{t + 1} {c}//{t + 1}\n")
     return mtl_path

@@ -144,6 +144,67 @@ def cornell_materials(width=64, height=64, fovy=40.0):
     return s, light_mesh
 
 
+def _rgba(r, g, b, a=255):
+    return np.uint32(r) | (np.uint32(g) << 8) | (np.uint32(b) << 16) | (np.uint32(a) << 24)
+
+
+def textured_cornell(width=64, height=64, fovy=40.0):
+    """cornell_materials with texture maps and vertex normals: planar texture
+    coordinates per face (outside [0, 1] on purpose), an 8 x 8 checker
+    (wrap) on the floor, ceiling and back wall and as the Phong box's
+    specular map, a 16 x 4 colour ramp (clamp) on the side walls and as the
+    Phong box's diffuse map, a 4 x 4 emission map (clamp) on both light
+    meshes (on hit and when light-sampled), and on the glass box vertex
+    normals pointing away from its centre.  Returns (scene, light_mesh); the
+    scene holds the arrays of rtpg_scene_attributes."""
+    s, lm = cornell_materials(width, height, fovy)
+    nt = s["idx"].shape[0]
+    q = s["verts"][s["idx"].astype(np.int64)].astype(np.float64)  # nt x 3 x 3
+    fn = np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    ax = np.argmax(np.abs(fn), axis=1)
+    uv = np.empty((nt, 3, 2), np.float64)
+    for t in range(nt):
+        a, b = [(1, 2), (0, 2), (0, 1)][ax[t]]
+        uv[t] = q[t][:, [a, b]] * 0.8 + 0.1
+    checker = np.empty((8, 8), np.uint32)
+    for j in range(8):
+        for i in range(8):
+            checker[j, i] = _rgba(230, 225, 215) if (i + j) % 2 == 0 else _rgba(60, 90, 200)
+    ramp = np.empty((4, 16), np.uint32)
+    for j in range(4):
+        for i in range(16):
+            ramp[j, i] = _rgba(17 * i, 255 - 17 * i, 60 * j + 30)
+    glow = np.empty((4, 4), np.uint32)
+    for j in range(4):
+        for i in range(4):
+            glow[j, i] = _rgba(255 - 40 * i, 200 + 15 * j, 128 + 30 * ((i + j) % 3))
+    texels = np.concatenate([checker.ravel(), ramp.ravel(), glow.ravel()])
+    tex = np.array([[0, 8, 8, 1], [64, 16, 4, 0], [128, 4, 4, 0]], np.uint32)
+    none = 0xFFFFFFFF
+    tid = np.full((nt, 4), none, np.uint32)
+    mat, em = s["mat"], s["emission"]
+    n_walls = 10
+    side = (ax[:n_walls] == 0)
+    tid[:n_walls, 0] = np.where(side, 1, 0)
+    tid[mat == MAT_PHONG, 0] = 1
+    tid[mat == MAT_PHONG, 1] = 0
+    lights = em.sum(axis=1) > 0
+    tid[lights, 2] = 2
+    tid[lights, 3] = 2
+    vn = np.repeat(fn[:, None, :], 3, axis=1)
+    glass = mat == MAT_GLASS
+    centre = q[glass].reshape(-1, 3).mean(axis=0)
+    d = q[glass] - centre
+    vn[glass] = d / np.linalg.norm(d, axis=2, keepdims=True)
+    s["texcoord"] = uv.astype(np.float32)
+    s["tex_id"] = tid
+    s["vnormal"] = vn.astype(np.float32)
+    s["texels"] = texels
+    s["tex"] = tex
+    return s, lm
+
+
 def _value_noise(nx, nz, octaves, seed):
     """fBm of bilinearly interpolated value noise on an nx x nz grid in [0,1]^2."""
     rng = np.random.default_rng(seed)

@@ -28,6 +28,9 @@ FLAG_LOCATE = 64
 FLAG_MIS = 128
 FLAG_RR = 256
 FLAG_USE_MLP = 512
+TEX_NONE = 0xFFFFFFFF
+TEX_CLAMP = 0
+TEX_WRAP = 1
 
 u32 = C.c_uint32
 u16 = C.c_uint16
@@ -78,9 +81,26 @@ class Context:
         call("rtpg_scene_upload", self.h, ptr(v), v.shape[0], ptr(i, u32), i.shape[0], ptr(alb), ptr(mat, u32),
              ptr(em), ptr(sp), light_mode, ptr(lm, u32))
         self.n_tris = i.shape[0]
+        if scene.get("tex_id") is not None or scene.get("vnormal") is not None:
+            self.set_attributes(scene.get("texcoord"), scene.get("tex_id"), scene.get("vnormal"),
+                                scene.get("texels"), scene.get("tex"))
         cam = scene.get("camera")
         if cam is not None:
             self.set_camera_look(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], cam["width"], cam["height"])
+
+    def set_attributes(self, texcoord=None, tex_id=None, vnormal=None, texels=None, tex=None):
+        """Texture coordinates (nt x 3 x 2), texture indices per triangle
+        (nt x 4: diffuse, specular, emission on hit, emission when
+        light-sampled; TEX_NONE = white), vertex normals (nt x 3 x 3), the
+        RGBA8 texels (uint32, R in the low byte, rows bottom-up) and the
+        textures (n x 4: first texel, width, height, TEX_CLAMP / TEX_WRAP)."""
+        tc = None if texcoord is None else _f32(texcoord).reshape(-1)
+        ti = None if tex_id is None else _u32(tex_id).reshape(-1)
+        vn = None if vnormal is None else _f32(vnormal).reshape(-1)
+        tx = None if texels is None else _u32(texels).reshape(-1)
+        td = None if tex is None else _u32(tex).reshape(-1, 4)
+        call("rtpg_scene_attributes", self.h, ptr(tc), ptr(ti, u32), ptr(vn), ptr(tx, u32),
+             0 if tx is None else tx.size, ptr(td, u32), 0 if td is None else td.shape[0])
 
     def set_albedo(self, albedo_rgb):
         a = _f32(albedo_rgb)

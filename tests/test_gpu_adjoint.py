@@ -83,3 +83,33 @@ def test_torch_autograd_wrapper(ctx):
     # d loss / d Le . Le = 2 * loss for a loss quadratic in a linear function of Le
     assert abs(float((le.grad * le).sum()) - 2 * float(loss)) <= 2e-3 * float(loss)
     assert alb.grad.shape == alb.shape and torch.isfinite(alb.grad).all() and alb.grad.abs().sum() > 0
+
+
+def test_adjoint_with_textures_matches_oracle(ctx, orc):
+    # diffuse maps on the Lambertian walls and an emission map on the light:
+    # the gradient is still with respect to the per-triangle albedo and the
+    # light's radiance (the texture is a fixed factor)
+    sc = scenes.cornell(W, H)
+    nt = sc["idx"].shape[0]
+    q = sc["verts"][sc["idx"].astype(np.int64)].astype(np.float64)
+    fn = np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0])
+    ax = np.argmax(np.abs(fn), axis=1)
+    uv = np.stack([q[t][:, [(1, 2), (0, 2), (0, 1)][ax[t]]] for t in range(nt)]) * 0.7
+    rng = np.random.default_rng(3)
+    sc["texcoord"] = uv.astype(np.float32)
+    sc["tex_id"] = np.full((nt, 4), 0xFFFFFFFF, np.uint32)
+    light = sc["emission"].sum(axis=1) > 0
+    sc["tex_id"][~light, 0] = 0
+    sc["tex_id"][light, 2] = 1
+    sc["tex_id"][light, 3] = 1
+    sc["texels"] = rng.integers(0, 2**32, 64 + 4, dtype=np.uint64).astype(np.uint32)
+    sc["tex"] = np.array([[0, 8, 8, 1], [64, 2, 2, 0]], np.uint32)
+    ctx.upload_scene(sc)
+    dldi = rng.uniform(-1, 1, (H, W, 3)).astype(np.float32)
+    ga, gl = ctx.adjoint(dldi, spp=8, flags=FLAGS, max_depth=6, seed=4)
+    cam = sc["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 1.0)
+    oa, ol = orc.adjoint(orc.Scene(sc), uvw, W, H, 8, FLAGS, dldi, 2, max_depth=6, seed=4)
+    scale = np.abs(oa).max()
+    assert np.max(np.abs(ga - oa)) <= 2e-3 * scale
+    assert np.allclose(gl, ol, rtol=2e-3, atol=2e-3 * np.abs(ol).max())

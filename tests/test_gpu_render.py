@@ -324,3 +324,87 @@ def test_debug_grid_value_is_the_root_weight_of_the_hit_cell(ctx):
     # without the grid the value is 0
     v0, _, _ = ctx.render_debug(ctx.DEBUG_GRID_VALUE, seed=1)
     assert np.all(v0 == 0)
+
+
+def test_textured_scene_matches_cpu(ctx, orc):
+    # texture maps on the diffuse, specular and both emission slots and
+    # smooth vertex normals on the glass box, GPU vs oracle on the same streams
+    sc, lm = scenes.textured_cornell(48, 48)
+    cam = sc["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 1.0)
+    kw = dict(max_depth=8, rr_depth=3, num_candidates=8, seed=14)
+    # (1) the glass box made Lambertian: the same streams give the same paths
+    lamb = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in sc.items()}
+    lamb["albedo"][lamb["mat"] == scenes.MAT_GLASS, 3] = 0.0
+    lamb["mat"][lamb["mat"] == scenes.MAT_GLASS] = scenes.MAT_LAMBERT
+    ctx.upload_scene(lamb, light_mode=1, light_mesh=lm)
+    plain = dict(sc)
+    for k in ("texcoord", "tex_id", "texels", "tex", "vnormal"):
+        plain.pop(k)
+    for flags in (FLAG_NEE | FLAG_MIS | FLAG_RR, FLAG_NEE | FLAG_RIS):
+        ctx.clear()
+        ctx.render(spp=16, flags=flags, **kw)
+        g = ctx.image()
+        acc, _ = orc.render(orc.Scene(lamb, light_mesh=lm), uvw, 48, 48, 16, flags, **kw)
+        assert _rel_mse(g, acc / 16) < 2e-3, flags
+    # (2) with the glass box and its vertex normals: closer than an
+    # independent render, same mean
+    ctx.upload_scene(sc, light_mode=1, light_mesh=lm)
+    flags = FLAG_NEE | FLAG_MIS | FLAG_RR
+    ctx.clear()
+    ctx.render(spp=64, flags=flags, **kw)
+    g = ctx.image()
+    ctx.clear()
+    ctx.render(spp=64, flags=flags, **dict(kw, seed=98))
+    other = ctx.image()
+    acc, _ = orc.render(orc.Scene(sc, light_mesh=lm), uvw, 48, 48, 64, flags, **kw)
+    c = acc / 64
+    assert _rel_mse(g, c) < 0.5 * _rel_mse(other, c)
+    assert abs(g.mean() - c.mean()) / c.mean() < 0.01
+    # the textures change the image (the untextured scene differs in mean)
+    ctx.upload_scene(plain, light_mode=1, light_mesh=lm)
+    ctx.clear()
+    ctx.render(spp=64, flags=flags, **kw)
+    assert abs(ctx.image().mean() - g.mean()) / g.mean() > 0.05
+
+
+def test_white_texture_render_equals_untextured(ctx):
+    # the texture-reading kernels with every slot on a white 1 x 1 texture
+    # against the untextured kernels: same paths, same image up to rounding
+    sc, lm = scenes.cornell_materials(48, 48)
+    nt = sc["idx"].shape[0]
+    tx = dict(sc)
+    tx["texcoord"] = np.random.default_rng(2).random((nt, 3, 2)).astype(np.float32) * 4 - 2
+    tx["tex_id"] = np.zeros((nt, 4), np.uint32)
+    tx["texels"] = np.array([0xFFFFFFFF], np.uint32)
+    tx["tex"] = np.array([[0, 1, 1, 1]], np.uint32)
+    kw = dict(max_depth=8, rr_depth=3, num_candidates=8, seed=15)
+    for flags in (FLAG_NEE | FLAG_MIS | FLAG_RR, FLAG_NEE | FLAG_RIS):
+        ctx.upload_scene(sc, light_mode=1, light_mesh=lm)
+        ctx.clear()
+        ctx.render(spp=16, flags=flags, **kw)
+        a = ctx.image()
+        ctx.upload_scene(tx, light_mode=1, light_mesh=lm)
+        ctx.clear()
+        ctx.render(spp=16, flags=flags, **kw)
+        b = ctx.image()
+        assert _rel_mse(b, a) < 2e-3 and abs(b.mean() - a.mean()) < 2e-3 * a.mean(), flags
+
+
+def test_textured_debug_frames_match_cpu(ctx, orc):
+    sc, lm = scenes.textured_cornell(40, 32)
+    cam = sc["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 40 / 32)
+    ctx.upload_scene(sc, light_mode=1, light_mesh=lm)
+    ctx.resize(40, 32)
+    ctx.set_camera_look(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 40, 32)
+    S = orc.Scene(sc, light_mesh=lm)
+    for ft in (2, 3, 4):
+        val, prim, _ = ctx.render_debug(ft, seed=5, sample_offset=1)
+        ref, tri = orc.debug_frame(S, uvw, 40, 32, ft, seed=5, sample_offset=1)
+        same = prim.astype(np.int64) == np.where(tri < 0, 0xFFFFFFFF, tri.astype(np.int64))
+        assert same.mean() > 0.995, ft
+        # textured values to float rounding of the texture coordinate
+        assert np.allclose(val[same], ref[same], rtol=1e-4, atol=1e-5), ft
+    d, _, _ = ctx.render_debug(2, seed=5)
+    assert len(np.unique(d[..., 2].round(4))) > 8  # the maps vary across the walls

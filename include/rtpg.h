@@ -344,8 +344,11 @@ int rtpg_mqtree_ctrl_query(rtpg_mqtree_ctrl c, uint32_t q[6]);
  * w_i = avg_rgb(f Li |cos|) / p_sample of a radiance sample recorded by the
  * tracer (the KL divergence from the radiance-weighted target, up to a
  * constant).  With RTPG_FLAG_USE_MLP a render records training samples from
- * the first record_paths paths of every launch chunk (unless RTPG_FLAG_FINAL)
- * and, with RTPG_FLAG_GUIDE_SAMPLE, samples directions from the guide. */
+ * at most record_paths paths of every launch chunk (unless RTPG_FLAG_FINAL):
+ * every s-th path, s = ceil(paths / record_paths), starting at path
+ * sample_offset mod s, so that they cover the whole image and successive
+ * passes record different pixels; with RTPG_FLAG_GUIDE_SAMPLE it samples
+ * directions from the guide. */
 typedef struct rtpg_mlp_config {
   uint32_t log2_table;   /* hash entries per level (19) */
   float base_res;        /* coarsest grid resolution (16) */
@@ -355,7 +358,7 @@ typedef struct rtpg_mlp_config {
   float eps;             /* Adam epsilon (1e-15) */
   uint32_t batch;        /* samples per training step (2^18) */
   uint32_t capacity;     /* recorded samples kept between trainings (2^20) */
-  uint32_t record_paths; /* paths per render chunk that record samples (2^16) */
+  uint32_t record_paths; /* at most this many paths per render chunk record samples (2^16) */
   uint64_t seed;         /* initialisation */
   uint32_t guide_depth;  /* vertices guided and recorded: depth < guide_depth (0 = all) */
   uint32_t cache_res;    /* 0: the network is evaluated at every guided vertex; n > 0: the

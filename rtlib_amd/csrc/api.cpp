@@ -415,7 +415,9 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
 
 static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, bool guide, bool counting,
                         rtpg_mlp_t* mlp = nullptr) {
-  uint32_t mlp_rec = mlp ? std::min(n, mlp->cfg.record_paths) : 0u;
+  // learned guide: every stride-th path records (at most record_paths slots)
+  uint32_t mlp_stride = mlp ? std::max(1u, (uint32_t)(((uint64_t)n + mlp->cfg.record_paths - 1u) / mlp->cfg.record_paths)) : 0u;
+  uint32_t mlp_rec = mlp ? (n + mlp_stride - 1u) / mlp_stride : 0u;
   // per-slot mixtures of the inference pass (none with the spatial cache)
   uint32_t mix_slots = (mlp && mlp->cfg.cache_res == 0) ? n : 0u;
   ensure_arena(ctx, n, max_depth, guide, mlp_rec, mix_slots);
@@ -442,12 +444,14 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
   ps.recs = guide ? ctx->p_recs.p : nullptr;
   ps.rec_count = guide ? ctx->p_rec_count.p : nullptr;
   ps.npix = ctx->width * ctx->height;
-  ps.rec_limit = mlp ? mlp_rec : n;
+  ps.rec_limit = n;
   if (mlp) {
     ps.mix = ctx->p_mix.p;
     ps.mixc = ctx->p_mixc.p;
     ps.mv = ctx->p_mv.p;
     ps.mlp_rec = mlp_rec;
+    ps.mlp_stride = mlp_stride;
+    ps.mlp_phase = 0;
     ps.mlp_guide_depth = mlp->cfg.guide_depth ? mlp->cfg.guide_depth : 0xFFFFFFFFu;
     ps.mlp_out = mlp->samples.p;
     ps.mlp_count = mlp->count.p;
@@ -654,6 +658,7 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       uint32_t n = cs * npix;
       PathSoA ps = make_soa(ctx, n, prm->max_depth, guide != 0, counting, guide == 3 ? prm->mlp : nullptr);
       RenderParams rp = make_rp(ctx, prm, cs, done);
+      if (guide == 3) ps.mlp_phase = rp.sample_offset % ps.mlp_stride;
       if (timing && ev_pending == kEvChunks) drain_events();
       RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, ml, guide, ctx->accum.p, s,
                                    timing ? ctx->events.data() + (ev_pending++) * ev_stride : nullptr));

@@ -174,15 +174,26 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
     __syncthreads();
     // gradient of the 32 encoded features: lane holds levels {2g, 2g+1} and {8+2g, 9+2g}
     dense_bwd<2, 4>(lw + kLdsW0, kLdsP0, dz, dh, lane);
-    if (valid) {
+    {
       float px = fminf(fmaxf((in.p.x - m.lo[0]) * m.inv_size, 0.0f), 1.0f);
       float py = fminf(fmaxf((in.p.y - m.lo[1]) * m.inv_size, 0.0f), 1.0f);
       float pz = fminf(fmaxf((in.p.z - m.lo[2]) * m.inv_size, 0.0f), 1.0f);
+      // coarse levels 0-7: neighbouring samples share cells, so each row of
+      // 16 samples sums its runs before the atomics; fine levels 8-15 direct
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int q = 0; q < 2; ++q) {
+        int l = 2 * g + q;
+        if (m.res[l] < 1024.0f) {
+          mlp_encode_level_grad_runs(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1], valid);
+        } else if (valid) {
+          mlp_encode_level_grad(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1]);
+        }
+      }
+      if (valid) {
 #pragma unroll
         for (int q = 0; q < 2; ++q)
-          mlp_encode_level_grad(m, 8 * b + 2 * g + q, px, py, pz, dh[b][2 * q], dh[b][2 * q + 1]);
+          mlp_encode_level_grad(m, 8 + 2 * g + q, px, py, pz, dh[1][2 * q], dh[1][2 * q + 1]);
+      }
     }
   }
   float* dst = partial + (size_t)blockIdx.x * kMlpWeights;
@@ -196,19 +207,37 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
   if (threadIdx.x == 0) loss_partial[blockIdx.x] = lred[0] + lred[1] + lred[2] + lred[3];
 }
 
-// Sums the per-work-group partials in work-group order (deterministic).
-__global__ void k_mlp_reduce(const float* partial, const float* loss_partial, uint32_t grid, float* grad_w,
-                             float* loss) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// Sums the per-work-group partials in a fixed order (deterministic): a
+// work-group of 4 waves owns 64 weights (one per lane); wave w adds the
+// partial rows w, w + 4, w + 8, ... and the four wave sums are added in wave
+// order.  Work-group 0 also sums the loss partials the same way (256 lanes,
+// then a fixed tree).
+constexpr uint32_t kReduceCols = 64;
+
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float* partial, const float* loss_partial, uint32_t grid,
+                                                    float* grad_w, float* loss) {
+  __shared__ float red[4][kReduceCols];
+  __shared__ float lred[256];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t k = blockIdx.x * kReduceCols + lane;
+  float s = 0.0f;
   if (k < (uint32_t)kMlpWeights) {
-    float s = 0.0f;
-    for (uint32_t b = 0; b < grid; ++b) s += partial[(size_t)b * kMlpWeights + k];
-    grad_w[k] = s;
+    for (uint32_t b = wave; b < grid; b += 4u) s += partial[(size_t)b * kMlpWeights + k];
   }
-  if (k == 0) {
-    float s = 0.0f;
-    for (uint32_t b = 0; b < grid; ++b) s += loss_partial[b];
-    *loss = s;
+  red[wave][lane] = s;
+  if (blockIdx.x == 0) {
+    float t = 0.0f;
+    for (uint32_t b = threadIdx.x; b < grid; b += 256u) t += loss_partial[b];
+    lred[threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (wave == 0 && k < (uint32_t)kMlpWeights) grad_w[k] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  if (blockIdx.x == 0) {
+    for (uint32_t h = 128; h >= 1; h >>= 1) {
+      if (threadIdx.x < h) lred[threadIdx.x] += lred[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = lred[0];
   }
 }
 
@@ -393,8 +422,8 @@ hipError_t launch_mlp_train(const MlpView& m, const float4* samples, uint32_t n,
 
 hipError_t launch_mlp_reduce(const float* partial, const float* loss_partial, uint32_t grid, float* grad_w,
                              float* loss, hipStream_t s) {
-  hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(kMlpWeights, 256)), dim3(256), 0, s, partial, loss_partial, grid, grad_w,
-                     loss);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(kMlpWeights, kReduceCols)), dim3(256), 0, s, partial, loss_partial, grid,
+                     grad_w, loss);
   return hipGetLastError();
 }
 

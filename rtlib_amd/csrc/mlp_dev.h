@@ -146,6 +146,55 @@ __device__ __forceinline__ void mlp_encode_level_grad(const MlpView& m, int l, f
   }
 }
 
+// The same scatter for coarse levels, where neighbouring samples often share a
+// grid cell.  Each row of 16 lanes holds 16 samples at one level (lane r of
+// the row: sample r; every lane of the row takes part, valid = false lanes
+// contribute nothing).  Runs of consecutive lanes in the same cell sum their
+// 8 corner x 2 feature contributions with a segmented suffix scan and the
+// first lane of a run issues one atomic per corner and feature for the whole
+// run.  Needs res[l] < 1024 (the cell key packs 10 bits per axis).
+__device__ __forceinline__ void mlp_encode_level_grad_runs(const MlpView& m, int l, float px, float py, float pz,
+                                                           float g0, float g1, bool valid) {
+  MlpCell c = mlp_cell(m, l, px, py, pz);
+  const uint32_t lane = __lane_id(), r = lane & 15u;
+  const uint32_t key = valid ? (c.x | (c.y << 10) | (c.z << 20)) : 0xFFFFFFFFu;
+  const uint32_t kn = (uint32_t)__shfl_down((int)key, 1, 16);
+  const uint32_t kp = (uint32_t)__shfl_up((int)key, 1, 16);
+  const bool same_next = r != 15u && kn == key;
+  float v[16];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    float w = (dx ? c.fx : 1.0f - c.fx) * (dy ? c.fy : 1.0f - c.fy) * (dz ? c.fz : 1.0f - c.fz);
+    v[2 * k] = valid ? w * g0 : 0.0f;
+    v[2 * k + 1] = valid ? w * g1 : 0.0f;
+  }
+  if (__ballot(same_next) != 0ull) {
+    // last lane of this lane's run: the first run end at or after it (lane 15
+    // of every row is one)
+    const uint64_t ends = __ballot(!same_next);
+    const uint32_t e = lane + (uint32_t)__ffsll((long long)(ends >> lane)) - 1u;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        float t = __shfl_down(v[j], off, 16);
+        if (lane + (uint32_t)off <= e) v[j] += t;
+      }
+    }
+  }
+  if (valid && (r == 0u || kp != key)) {
+    float* gt = m.grad + (size_t)l * (m.tmask + 1u) * 2u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+      uint32_t i = mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz);
+      if (v[2 * k] != 0.0f) unsafeAtomicAdd(gt + 2u * i, v[2 * k]);
+      if (v[2 * k + 1] != 0.0f) unsafeAtomicAdd(gt + 2u * i + 1u, v[2 * k + 1]);
+    }
+  }
+}
+
 // ------------------------------------------------------------- mixture
 // Unit-sphere vMF: v(w) = kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)).
 RTPG_HD float vmf_norm(float kappa) { return kappa / (2.0f * kPi * -expm1f(-2.0f * kappa)); }

@@ -70,11 +70,13 @@ uint32_t train_grid(uint32_t n) { return std::max(1u, std::min(kMlpTrainGrid, (n
 
 // Forward + backward over n device samples; leaves the weight gradient sum in
 // w_grad, the table gradient sum in t_grad and the loss sum in loss[0].
-void grad_pass(rtpg_mlp_t* m, const float4* smp, uint32_t n) {
+// The loss sum goes to loss[slot] (loss holds one slot per step of a call).
+void grad_pass(rtpg_mlp_t* m, const float4* smp, uint32_t n, uint32_t slot = 0) {
   hipStream_t s = m->ctx->stream;
   uint32_t grid = train_grid(n);
+  m->loss.reserve(slot + 1u);
   RTPG_HIP(launch_mlp_train(mlp_view_of(m, true), smp, n, m->partial.p, m->loss_partial.p, grid, s));
-  RTPG_HIP(launch_mlp_reduce(m->partial.p, m->loss_partial.p, grid, m->w_grad.p, m->loss.p, s));
+  RTPG_HIP(launch_mlp_reduce(m->partial.p, m->loss_partial.p, grid, m->w_grad.p, m->loss.p + slot, s));
 }
 
 void adam_step(rtpg_mlp_t* m, uint32_t n) {
@@ -323,13 +325,18 @@ int rtpg_mlp_train_recorded(rtpg_mlp m, uint32_t steps, float* loss, uint32_t* u
     }
     uint32_t b = std::min(n, m->cfg.batch);
     uint32_t windows = n / b;  // whole batches in recording order
+    // every step's loss sum stays on the device until the last step
+    m->loss.reserve(std::max(steps, 1u));
     for (uint32_t k = 0; k < steps; ++k) {
       const float4* smp = m->samples.p + 4 * (size_t)((k % windows) * b);
-      grad_pass(m, smp, b);
-      if (loss) loss[k] = download_loss(m, b);
+      grad_pass(m, smp, b, k);
       adam_step(m, b);
     }
     m->count.zero(1, s);
+    if (loss && steps > 0) {
+      m->loss.download(loss, steps, s);
+      for (uint32_t k = 0; k < steps; ++k) loss[k] /= (float)b;
+    }
     RTPG_HIP(hipStreamSynchronize(s));
   });
 }

@@ -56,15 +56,19 @@ struct PathSoA {
   const float4* adjL;    // primal radiance per path
   float* grad_alb;       // 3 per leaf-order triangle
   float* grad_le;        // 3 per light
-  // guide vertices (vA, vB, shadow contributions) are recorded for pid < rec_limit
+  // guide vertices (vA, vB, shadow contributions) are recorded for pid <
+  // rec_limit, or with the learned guide (mlp_stride > 0) for the paths
+  // pid = slot * mlp_stride + mlp_phase
   uint32_t rec_limit;
   // learned guide: mixture per queue slot (kMlpMixF4 float4 + c), training
-  // vertices of the first mlp_rec paths ([5][depth][pid] float4) and the
-  // sample buffer they are turned into
+  // vertices of the recorded paths ([5][depth][slot] float4, mlp_rec slots)
+  // and the sample buffer they are turned into
   float4* mix;
   float* mixc;
   float4* mv;
   uint32_t mlp_rec;
+  uint32_t mlp_stride;
+  uint32_t mlp_phase;
   uint32_t mlp_guide_depth;  // vertices at depth < this are guided and recorded
   // spatial cache of the learned guide (null: the per-slot mixtures of the
   // inference pass): entries of mlp_cache_entry over the network's box

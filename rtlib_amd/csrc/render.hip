@@ -40,6 +40,13 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
 
 __device__ __forceinline__ V3 load3(float4 v) { return V3{v.x, v.y, v.z}; }
 
+// Paths whose guide vertices are recorded: pid < rec_limit, or with the
+// learned guide (mlp_stride > 0) every mlp_stride-th path from mlp_phase
+// (training slot pid / mlp_stride).
+__device__ __forceinline__ bool records_path(const PathSoA& ps, uint32_t pid) {
+  return ps.mlp_stride ? pid % ps.mlp_stride == ps.mlp_phase : pid < ps.rec_limit;
+}
+
 // Moller-Trumbore without FMA contraction so hits are bit-identical to the
 // CPU brute-force oracle.  Two-sided.
 __device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tmin, float& tbest,
@@ -793,10 +800,10 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
           }
           bool bsdf_ok = finite3(be.f) && be.f.x >= 0.0f && be.f.y >= 0.0f && be.f.z >= 0.0f;
           vflags = bsdf_ok ? 0u : kVtxBadBsdf;
-          if (kGuide == 3 && record && pid < ps.mlp_rec) {
+          if (kGuide == 3 && record && records_path(ps, pid)) {
             // training vertex: network inputs, sampled direction, BSDF density
             // and the BSDF weight f |cos| / p_sample before Russian roulette
-            size_t mi = (size_t)depth * ps.mlp_rec + pid, st = (size_t)rp.max_depth * ps.mlp_rec;
+            size_t mi = (size_t)depth * ps.mlp_rec + pid / ps.mlp_stride, st = (size_t)rp.max_depth * ps.mlp_rec;
             float rough = type == kMatGGX ? param : 0.0f;
             V3 bw = (wo_pdf > 0.0f) ? be.f * (fabsf(cosd) / wo_pdf) : V3{0.0f, 0.0f, 0.0f};
             uint32_t ok = (wo_pdf > 0.0f && bsdf_ok) ? kVtxValid : 0u;
@@ -816,7 +823,7 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
     float4 L4 = ps.L[pid];
     ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
     ps.nhit[pid] = depth + 1u;
-    if (record && pid < ps.rec_limit) {
+    if (record && records_path(ps, pid)) {
       size_t vi = (size_t)depth * ps.n + pid;
       ps.vB[vi] = make_float4(c.x, c.y, c.z, cos_out);
       ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
@@ -837,10 +844,10 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
         // integer store: the all-ones sentinel is a NaN pattern as a float
         reinterpret_cast<uint4*>(ps.vC)[vi] = make_uint4(fbits(vcu), fbits(vcv), has_vertex ? vcell : kInvalidCell,
                                                          vflags | (has_vertex ? kVtxValid : 0u));
-      } else if (pid < ps.mlp_rec && (!cont || (vflags & kVtxDelta))) {
+      } else if (!cont || (vflags & kVtxDelta)) {
         // no training vertex here (light, miss, delta or absorbed path)
         size_t st = (size_t)rp.max_depth * ps.mlp_rec;
-        ps.mv[2 * st + (size_t)depth * ps.mlp_rec + pid] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
+        ps.mv[2 * st + (size_t)depth * ps.mlp_rec + pid / ps.mlp_stride] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
       }
     }
     if (cont) {
@@ -900,7 +907,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uin
       float4 le = sc.lights[4u * li + 3u];
       adj_add_le(acc_le, ps, li, adj_weight(ps, pid), V3{c.x, c.y, c.z}, V3{le.x, le.y, le.z});
     }
-    if (record && pid < ps.rec_limit) {
+    if (record && records_path(ps, pid)) {
       size_t vi = (size_t)depth * ps.n + pid;
       float4 vb = ps.vB[vi];
       ps.vB[vi] = make_float4(vb.x + c.x, vb.y + c.y, vb.z + c.z, vb.w);
@@ -1336,15 +1343,16 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_infer(SceneView sc, PathSoA
   }
 }
 
-// Training samples of the first mlp_rec paths: the radiance that reached
+// Training samples of the recorded paths (one thread per slot): the radiance that reached
 // vertex d from its sampled direction is the sum of the contributions of the
 // later hits divided by the throughput after the bounce (per channel, guarded
 // as in the SD-tree record); the sample weight is avg_rgb(f |cos| / p_sample *
 // Li).  Appended with one atomic per wave; samples beyond the capacity are
 // dropped.
 __global__ void __launch_bounds__(kBlock) k_mlp_emit(PathSoA ps, uint32_t max_depth, uint32_t max_vtx) {
-  uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
-  bool in = pid < ps.n && pid < ps.mlp_rec;
+  uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t pid = slot * ps.mlp_stride + ps.mlp_phase;
+  bool in = slot < ps.mlp_rec && pid < ps.n;
   uint32_t nh = in ? ps.nhit[pid] : 0u;
   uint32_t nv = nh > 0 ? nh - 1u : 0u;
   nv = nv < max_vtx ? nv : max_vtx;
@@ -1361,7 +1369,7 @@ __global__ void __launch_bounds__(kBlock) k_mlp_emit(PathSoA ps, uint32_t max_de
     }
     bool ok = false;
     float w = 0.0f;
-    size_t mi = (size_t)d * ps.mlp_rec + pid;
+    size_t mi = (size_t)d * ps.mlp_rec + slot;
     float4 mc{0, 0, 0, 0};
     if ((uint32_t)d < nv && (uint32_t)d < ps.mlp_guide_depth) {
       mc = ps.mv[2 * st + mi];
@@ -1521,7 +1529,7 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     }
   }
   if (record && guide == 3) {
-    uint32_t ne = blocks_for(std::min(ps.n, ps.mlp_rec), kBlock);
+    uint32_t ne = blocks_for(ps.mlp_rec, kBlock);
     hipLaunchKernelGGL(k_mlp_emit, dim3(ne), dim3(kBlock), 0, s, ps, rp.max_depth, rp.max_depth - 1u);
   }
   mark(4, 0, 1);

@@ -73,6 +73,10 @@ const char* rtpg_version(void);
  * Replaces OPX7Context/CUDAContext creation (OPX7/Src/OPX7Context.cpp:274-292,
  * Ext/CUDA/Src/CUDAContext.cpp) and CUDAStream::Synchronize. */
 int rtpg_context_create(int device, rtpg_context* out);
+/* Fails (RTPG_ERR_INVALID, the context untouched) while SD-trees, hash grids,
+ * Morton quadtrees or learned guides created on it are alive: they use its
+ * device and stream, so destroy them first (the reference requires the same
+ * order of its Destroy() calls). */
 int rtpg_context_destroy(rtpg_context ctx);
 int rtpg_context_set_stream(rtpg_context ctx, void* hip_stream);
 int rtpg_synchronize(rtpg_context ctx);

@@ -95,6 +95,8 @@ int rtpg_context_create(int device, rtpg_context* out) {
 int rtpg_context_destroy(rtpg_context ctx) {
   return guarded([&] {
     if (!ctx) return;
+    require(ctx->children == 0, "objects created on this context (SD-trees, hash grids, Morton quadtrees, "
+                                "learned guides) are still alive: destroy them before the context");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     hipStream_t own = ctx->own_stream;

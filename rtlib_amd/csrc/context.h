@@ -15,6 +15,9 @@
 
 struct rtpg_context_t {
   int device = 0;
+  // SD-trees, hash grids, Morton quadtrees and learned guides created on this
+  // context and not yet destroyed (they use its device and stream)
+  int children = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   // scene

@@ -115,6 +115,7 @@ int rtpg_hashgrid_create(rtpg_context ctx, const float aabb_min[3], const float 
       delete g;
       throw;
     }
+    g->ctx->children++;
     *out = g;
   });
 }
@@ -123,6 +124,7 @@ int rtpg_hashgrid_destroy(rtpg_hashgrid g) {
   return guarded([&] {
     if (!g) return;
     (void)hipStreamSynchronize(g->ctx->stream);
+    g->ctx->children--;
     delete g;
   });
 }
@@ -187,6 +189,7 @@ int rtpg_mqtree_create(rtpg_context ctx, uint32_t slots, uint32_t level, float f
       delete m;
       throw;
     }
+    m->ctx->children++;
     *out = m;
   });
 }
@@ -195,6 +198,7 @@ int rtpg_mqtree_destroy(rtpg_mqtree m) {
   return guarded([&] {
     if (!m) return;
     (void)hipStreamSynchronize(m->ctx->stream);
+    m->ctx->children--;
     delete m;
   });
 }

@@ -196,10 +196,13 @@ __device__ __forceinline__ void mlp_encode_level_grad_runs(const MlpView& m, int
 }
 
 // ------------------------------------------------------------- mixture
+constexpr float kLog2E = 1.44269504088896340736f;
 // Unit-sphere vMF: v(w) = kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)).
 RTPG_HD float vmf_norm(float kappa) { return kappa / (2.0f * kPi * -expm1f(-2.0f * kappa)); }
 
 // Mixture density at unit direction d; mx: kMlpMixF4 float4 of one vertex.
+// The exponentials use the hardware base-2 exponential (v_exp_f32, 1 ulp):
+// exp(x) = 2^(x log2 e); the arguments are <= 0, so no overflow path is needed.
 __device__ __forceinline__ float mix_pdf(const float4* mx, V3 d) {
   float4 a0 = mx[10], a1 = mx[11];
   float A[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
@@ -207,7 +210,8 @@ __device__ __forceinline__ float mix_pdf(const float4* mx, V3 d) {
 #pragma unroll
   for (int j = 0; j < kMlpLobes; ++j) {
     float4 l = mx[j];
-    q += A[j] * expf(l.w * (l.x * d.x + l.y * d.y + l.z * d.z - 1.0f));
+    float x = l.w * (l.x * d.x + l.y * d.y + l.z * d.z - 1.0f);
+    q += A[j] * __builtin_amdgcn_exp2f(x * kLog2E);
   }
   return q;
 }
@@ -242,7 +246,14 @@ __device__ __forceinline__ V3 mix_sample(const float4* mx, Pcg32& rng) {
   for (int k = 1; k < kMlpLobes; ++k)
     if (k == j) l = mx[k];
   float u1 = rnd(rng), u2 = rnd(rng);
-  return vmf_sample(V3{l.x, l.y, l.z}, l.w, u1, u2);
+  // vmf_sample with the azimuth's sine and cosine from the hardware
+  // (v_sin_f32 / v_cos_f32 take the angle in revolutions: sin(2 pi u2))
+  float kappa = l.w;
+  float a = fmaxf(u1 + (1.0f - u1) * expf(-2.0f * kappa), 1e-38f);
+  float w = fminf(fmaxf(1.0f + logf(a) / kappa, -1.0f), 1.0f);
+  float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  Onb o = make_onb(V3{l.x, l.y, l.z});
+  return normalize(o.t * (s * __builtin_amdgcn_cosf(u2)) + o.b * (s * __builtin_amdgcn_sinf(u2)) + o.n * w);
 }
 
 }  // namespace rtpg

@@ -202,6 +202,7 @@ int rtpg_mlp_create(rtpg_context ctx, const float aabb_min[3], const float aabb_
       delete m;
       throw;
     }
+    m->ctx->children++;
     *out = m;
   });
 }
@@ -211,6 +212,7 @@ int rtpg_mlp_destroy(rtpg_mlp m) {
     if (m) {
       (void)hipSetDevice(m->ctx->device);
       (void)hipStreamSynchronize(m->ctx->stream);
+      m->ctx->children--;
     }
     delete m;
   });

@@ -435,6 +435,7 @@ int rtpg_sdtree_create(rtpg_context ctx, const float aabb_min[3], const float aa
       delete t;
       throw;
     }
+    t->ctx->children++;
     *out = t;
   });
 }
@@ -443,6 +444,7 @@ int rtpg_sdtree_destroy(rtpg_sdtree tree) {
   return guarded([&] {
     if (!tree) return;
     (void)hipStreamSynchronize(tree->ctx->stream);
+    tree->ctx->children--;
     delete tree;
   });
 }

@@ -11,6 +11,7 @@ All work runs through librtpg.so on the GPU; there is no host fallback.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -54,9 +55,16 @@ class Context:
         call("rtpg_context_create", device, C.byref(h))
         self.h = h
         self.width = self.height = 0
+        # guides created on this context: the C ABI requires them destroyed first
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if self.h:
+            for c in list(self._children):
+                c._release()
             call("rtpg_context_destroy", self.h)
             self.h = None
 
@@ -268,11 +276,14 @@ class RTSTreeWrapper:
         self.h = h
         self.ctx = ctx
         self.fraction = fraction
+        ctx._adopt(self)
 
     def Destroy(self):
         if self.h:
             call("rtpg_sdtree_destroy", self.h)
             self.h = None
+
+    _release = Destroy
 
     def __del__(self):
         try:
@@ -433,6 +444,13 @@ class DoubleBufferedHashGrid3Buffer:
         call("rtpg_hashgrid_create", ctx.h, ptr(lo), ptr(hi), ptr(b, u32), slots, key_mode, C.byref(h))
         self.h = h
         self.slots = slots
+        self.ctx = ctx
+        ctx._adopt(self)
+
+    def _release(self):
+        if self.h:
+            call("rtpg_hashgrid_destroy", self.h)
+            self.h = None
 
     def Clear(self):
         call("rtpg_hashgrid_clear", self.h)
@@ -453,9 +471,7 @@ class DoubleBufferedHashGrid3Buffer:
 
     def __del__(self):
         try:
-            if self.h:
-                call("rtpg_hashgrid_destroy", self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -469,6 +485,13 @@ class RTMortonQuadTreeWrapper:
         self.h = h
         self.slots, self.level, self.fraction = slots, level, fraction
         self.per_cell = ((1 << (2 * (level + 1))) - 1) // 3
+        self.ctx = ctx
+        ctx._adopt(self)
+
+    def _release(self):
+        if self.h:
+            call("rtpg_mqtree_destroy", self.h)
+            self.h = None
 
     def Allocate(self):
         call("rtpg_mqtree_allocate", self.h)
@@ -506,9 +529,7 @@ class RTMortonQuadTreeWrapper:
 
     def __del__(self):
         try:
-            if self.h:
-                call("rtpg_mqtree_destroy", self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -600,11 +621,14 @@ class MLPGuide:
         self.h = h.value
         self.ctx = ctx
         self.log2_table = log2_table
+        ctx._adopt(self)
 
     def close(self):
         if getattr(self, "h", None):
             call("rtpg_mlp_destroy", self.h)
             self.h = None
+
+    _release = close
 
     def cache(self):
         """Spatial cache (cache_res > 0): (mix entries x 48, c entries)."""

@@ -189,3 +189,27 @@ def test_lookup_deep_tree_and_box_edges(ctx, orc):
     want = orc.stree_lookup(words, LO, SIZE, pos)
     assert np.array_equal(got, want)
     assert len(np.unique(want)) == depth + 1  # every leaf is reached
+
+
+def test_context_outlives_its_guides(gpu):
+    # A context with live guides refuses to be destroyed (they use its stream);
+    # Context.close() destroys the guides it created first, so a guide object
+    # that outlives close() holds no handle to a freed context.
+    from rtlib_amd import _lib
+    from rtlib_amd.tracer import Context, MLPGuide, RTSTreeWrapper
+
+    c = Context(0)
+    tree = _tree(c)
+    mlp = MLPGuide(c, LO, HI, log2_table=12)
+    with pytest.raises(_lib.RtpgError, match="still alive"):
+        _lib.call("rtpg_context_destroy", c.h)
+    c.close()
+    assert c.h is None and tree.h is None and mlp.h is None
+    tree.Destroy()
+    mlp.close()
+    # the other order works too
+    c2 = Context(0)
+    t2 = RTSTreeWrapper(c2, LO, HI)
+    t2.Destroy()
+    c2.close()
+    assert c2.h is None

@@ -134,6 +134,7 @@ struct rtpg_mlp_t {
   rtpg::DBuf<float> w, w_m1, w_m2, w_grad;
   rtpg::DBuf<uint16_t> w_bf16;
   rtpg::DBuf<float> t, t_m1, t_m2, t_grad;
+  rtpg::DBuf<uint32_t> t_grad_pk;  // packed bf16 gradient of levels kMlpPkFirst.. (entries each)
   rtpg::DBuf<uint32_t> t_bf16;
   rtpg::DBuf<float> partial, loss_partial, loss;
   rtpg::DBuf<float4> samples;  // recorded by renders: capacity x 4
@@ -145,6 +146,7 @@ struct rtpg_mlp_t {
   rtpg::DBuf<float> cache_c;
   bool cache_stale = true;       // the network changed since the last fill
   size_t table_floats() const { return (size_t)rtpg::kMlpLevels * entries * rtpg::kMlpFeat; }
+  size_t pk_entries() const { return (size_t)(rtpg::kMlpLevels - rtpg::kMlpPkFirst) * entries; }
   uint32_t cache_entries() const { return cfg.cache_res * cfg.cache_res * cfg.cache_res * rtpg::kMlpCacheFaces; }
 };
 

@@ -19,10 +19,12 @@ hipError_t launch_mlp_train(const MlpView& m, const float4* samples, uint32_t n,
 hipError_t launch_mlp_reduce(const float* partial, const float* loss_partial, uint32_t grid, float* grad_w,
                              float* loss, hipStream_t s);
 // Adam on n fp32 parameters with gradient g * gscale; writes the bf16 copy
-// (w_bf16 for weights, or packed pairs into table_bf16 when pairs != 0) and
-// zeroes the gradient.
+// (w_bf16 for weights, or packed pairs into table_bf16 for the table) and
+// zeroes the gradient.  For the table, entries from pk_first on also take the
+// packed bf16 gradient gpk[entry - pk_first] (and zero it); gpk may be NULL.
 hipError_t launch_mlp_adam(float* p, float* m1, float* m2, float* g, float gscale, size_t n, float lr_t, float b1,
-                           float b2, float eps, uint16_t* w_bf16, uint32_t* table_bf16, hipStream_t s);
+                           float b2, float eps, uint16_t* w_bf16, uint32_t* table_bf16, uint32_t* gpk,
+                           size_t pk_first, hipStream_t s);
 // Queries: q = 3 float4 per sample ((x, roughness), (n, 0), (wo, 0)).
 hipError_t launch_mlp_forward(const MlpView& m, const float4* q, uint32_t n, float* raw, float* mix, hipStream_t s);
 // Draws one direction per sample from the mixture of launch_mlp_forward's

@@ -178,12 +178,16 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
       float px = fminf(fmaxf((in.p.x - m.lo[0]) * m.inv_size, 0.0f), 1.0f);
       float py = fminf(fmaxf((in.p.y - m.lo[1]) * m.inv_size, 0.0f), 1.0f);
       float pz = fminf(fmaxf((in.p.z - m.lo[2]) * m.inv_size, 0.0f), 1.0f);
-      // coarse levels 0-7: neighbouring samples share cells, so each row of
-      // 16 samples sums its runs before the atomics; fine levels 8-15 direct
+      // coarse levels below kMlpPkFirst: neighbouring samples share cells,
+      // so each row of 16 samples sums its runs before the fp32 atomics (the
+      // choice is uniform over a row: one level per row); finer levels send
+      // packed bf16 pairs directly
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         int l = 2 * g + q;
-        if (m.res[l] < 1024.0f) {
+        if (l >= kMlpPkFirst) {
+          if (valid) mlp_encode_level_grad_pk(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1]);
+        } else if (m.res[l] < 1024.0f) {
           mlp_encode_level_grad_runs(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1], valid);
         } else if (valid) {
           mlp_encode_level_grad(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1]);
@@ -192,7 +196,7 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
       if (valid) {
 #pragma unroll
         for (int q = 0; q < 2; ++q)
-          mlp_encode_level_grad(m, 8 + 2 * g + q, px, py, pz, dh[1][2 * q], dh[1][2 * q + 1]);
+          mlp_encode_level_grad_pk(m, 8 + 2 * g + q, px, py, pz, dh[1][2 * q], dh[1][2 * q + 1]);
       }
     }
   }
@@ -250,15 +254,24 @@ __device__ __forceinline__ float adam_one(float& p, float& m1, float& m2, float 
 }
 
 __global__ void k_mlp_adam(float* p, float* m1, float* m2, float* g, float gscale, size_t n, float lr_t, float b1,
-                           float b2, float eps, uint16_t* w_bf16, uint32_t* table_bf16) {
+                           float b2, float eps, uint16_t* w_bf16, uint32_t* table_bf16, uint32_t* gpk,
+                           size_t pk_first) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (table_bf16) {
     size_t e = 2 * i;
     if (e >= n) return;
+    // table entry i: fp32 gradient, plus the packed bf16 pair from pk_first on
+    float gp[2] = {0.0f, 0.0f};
+    if (gpk && i >= pk_first) {
+      uint32_t u = gpk[i - pk_first];
+      gpk[i - pk_first] = 0u;
+      gp[0] = bf2f((uint16_t)(u & 0xFFFFu));
+      gp[1] = bf2f((uint16_t)(u >> 16));
+    }
     float v[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      float gg = g[e + k] * gscale;
+      float gg = (g[e + k] + gp[k]) * gscale;
       g[e + k] = 0.0f;
       float pp = p[e + k], a = m1[e + k], b = m2[e + k];
       v[k] = adam_one(pp, a, b, gg, lr_t, b1, b2, eps);
@@ -428,10 +441,11 @@ hipError_t launch_mlp_reduce(const float* partial, const float* loss_partial, ui
 }
 
 hipError_t launch_mlp_adam(float* p, float* m1, float* m2, float* g, float gscale, size_t n, float lr_t, float b1,
-                           float b2, float eps, uint16_t* w_bf16, uint32_t* table_bf16, hipStream_t s) {
+                           float b2, float eps, uint16_t* w_bf16, uint32_t* table_bf16, uint32_t* gpk,
+                           size_t pk_first, hipStream_t s) {
   size_t threads = table_bf16 ? n / 2 : n;
   hipLaunchKernelGGL(k_mlp_adam, dim3(nblk(threads, 256)), dim3(256), 0, s, p, m1, m2, g, gscale, n, lr_t, b1, b2,
-                     eps, w_bf16, table_bf16);
+                     eps, w_bf16, table_bf16, gpk, pk_first);
   return hipGetLastError();
 }
 

@@ -32,10 +32,16 @@ constexpr float kMlpLogKappaMax = 8.0f;
 constexpr float kMlpSelMin = 0.05f;   // c = kMlpSelMin + kMlpSelSpan * sigmoid(t)
 constexpr float kMlpSelSpan = 0.9f;
 constexpr int kMlpMixF4 = 12;         // per-vertex mixture: 8 x (mu, kappa), pi[8], A[8]
+// Levels kMlpPkFirst.. (hashed at the default 2^19 entries: a few
+// contributions per entry and training step) accumulate their training
+// gradient as packed bf16 pairs, one atomic per entry for both features; the
+// dense coarse levels, where many samples share an entry, in fp32.
+constexpr int kMlpPkFirst = 5;
 
 struct MlpView {
   const uint32_t* table;  // [levels][T]: two bf16 features per entry
   float* grad;            // [levels][T][2] fp32 gradient (training)
+  uint32_t* grad_pk;      // [levels - kMlpPkFirst][T] bf16 pair gradient of the fine levels (training)
   const uint16_t* w;      // kMlpWeights bf16
   float lo[3];
   float inv_size;         // position -> [0,1]^3 of the cubic box
@@ -143,6 +149,29 @@ __device__ __forceinline__ void mlp_encode_level_grad(const MlpView& m, int l, f
     uint32_t i = mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz);
     if (g0 != 0.0f) unsafeAtomicAdd(gt + 2u * i, w * g0);
     if (g1 != 0.0f) unsafeAtomicAdd(gt + 2u * i + 1u, w * g1);
+  }
+}
+
+// The same scatter for a fine level l >= kMlpPkFirst into the packed bf16
+// gradient: both features of a corner in one global_atomic_pk_add_bf16 (the
+// contributions are rounded to bf16, the sum is kept in bf16; the fine levels'
+// entries receive a few contributions per training step).
+typedef __bf16 bf16x2_v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void mlp_encode_level_grad_pk(const MlpView& m, int l, float px, float py, float pz, float g0,
+                                                         float g1) {
+  MlpCell c = mlp_cell(m, l, px, py, pz);
+  bf16x2_v* gt = reinterpret_cast<bf16x2_v*>(m.grad_pk + (size_t)(l - kMlpPkFirst) * (m.tmask + 1u));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    float w = (dx ? c.fx : 1.0f - c.fx) * (dy ? c.fy : 1.0f - c.fy) * (dz ? c.fz : 1.0f - c.fz);
+    uint32_t i = mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz);
+    float a = w * g0, b = w * g1;
+    if (a != 0.0f || b != 0.0f) {
+      bf16x2_v v = {(__bf16)a, (__bf16)b};
+      __builtin_amdgcn_global_atomic_fadd_v2bf16(gt + i, v);
+    }
   }
 }
 

@@ -88,9 +88,9 @@ void adam_step(rtpg_mlp_t* m, uint32_t n) {
   float lr_t = (float)(c.lr * std::sqrt(1.0 - b2t) / (1.0 - b1t));
   float gs = 1.0f / (float)n;
   RTPG_HIP(launch_mlp_adam(m->w.p, m->w_m1.p, m->w_m2.p, m->w_grad.p, gs, kMlpWeights, lr_t, c.beta1, c.beta2, c.eps,
-                           m->w_bf16.p, nullptr, s));
+                           m->w_bf16.p, nullptr, nullptr, 0, s));
   RTPG_HIP(launch_mlp_adam(m->t.p, m->t_m1.p, m->t_m2.p, m->t_grad.p, gs, m->table_floats(), lr_t, c.beta1, c.beta2,
-                           c.eps, nullptr, m->t_bf16.p, s));
+                           c.eps, nullptr, m->t_bf16.p, m->t_grad_pk.p, (size_t)kMlpPkFirst * m->entries, s));
 }
 
 float download_loss(rtpg_mlp_t* m, uint32_t n) {
@@ -110,6 +110,7 @@ void reset_state(rtpg_mlp_t* m, uint64_t seed) {
   m->t_m1.zero(tf, s);
   m->t_m2.zero(tf, s);
   m->t_grad.zero(tf, s);
+  m->t_grad_pk.zero(m->pk_entries(), s);
   m->count.zero(1, s);
   m->steps = 0;
   RTPG_HIP(hipStreamSynchronize(s));
@@ -126,6 +127,7 @@ MlpView mlp_view_of(const rtpg_mlp_t* m, bool training) {
   MlpView v{};
   v.table = m->t_bf16.p;
   v.grad = training ? m->t_grad.p : nullptr;
+  v.grad_pk = training ? m->t_grad_pk.p : nullptr;
   v.w = m->w_bf16.p;
   for (int k = 0; k < 3; ++k) v.lo[k] = m->lo[k];
   v.inv_size = 1.0f / m->size;
@@ -191,6 +193,7 @@ int rtpg_mlp_create(rtpg_context ctx, const float aabb_min[3], const float aabb_
       m->t_m1.alloc(tf);
       m->t_m2.alloc(tf);
       m->t_grad.alloc(tf);
+      m->t_grad_pk.alloc(m->pk_entries());
       m->t_bf16.alloc(tf / 2);
       m->partial.alloc((size_t)kMlpTrainGrid * kMlpWeights);
       m->loss_partial.alloc(kMlpTrainGrid);
@@ -276,6 +279,7 @@ int rtpg_mlp_gradient(rtpg_mlp m, const float* pos, const float* wo, const float
     hipStream_t s = m->ctx->stream;
     upload_samples(m, pos, wo, nrm, rough, omega, weight, pdf_bsdf, n);
     m->t_grad.zero(m->table_floats(), s);
+    m->t_grad_pk.zero(m->pk_entries(), s);
     grad_pass(m, m->staging.p, n);
     float inv = 1.0f / (float)n;
     if (loss) *loss = download_loss(m, n);
@@ -286,9 +290,17 @@ int rtpg_mlp_gradient(rtpg_mlp m, const float* pos, const float* wo, const float
     if (grad_table) {
       size_t tf = m->table_floats();
       m->t_grad.download(grad_table, tf, s);
+      // fine levels: the packed bf16 pairs
+      std::vector<uint32_t> pk = m->t_grad_pk.to_vector(m->pk_entries(), s);
+      float* fine = grad_table + (size_t)kMlpPkFirst * m->entries * kMlpFeat;
+      for (size_t e = 0; e < pk.size(); ++e) {
+        fine[2 * e] += bf2f((uint16_t)(pk[e] & 0xFFFFu));
+        fine[2 * e + 1] += bf2f((uint16_t)(pk[e] >> 16));
+      }
       for (size_t k = 0; k < tf; ++k) grad_table[k] *= inv;
     }
     m->t_grad.zero(m->table_floats(), s);
+    m->t_grad_pk.zero(m->pk_entries(), s);
     RTPG_HIP(hipStreamSynchronize(s));
   });
 }

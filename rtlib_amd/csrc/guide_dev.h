@@ -291,13 +291,29 @@ RTPG_HD void grid_hash(const HashGridView& h, uint32_t ix, uint32_t iy, uint32_t
 }
 
 // GetPrvCellIndex (:298-319): read-only probe of the previous pass's table.
+// The slots are scanned in order up to the first empty one; the whole
+// 128-byte block is read at once (eight 16-byte loads issued together), and
+// the scan runs on bit masks: the result is the first slot holding the
+// checksum if no empty slot comes before it.
 RTPG_HD uint32_t grid_find_prv(const HashGridView& h, uint32_t bucket, uint32_t checksum) {
-  for (uint32_t i = 0; i < 32u; ++i) {
-    uint32_t c = h.prv[32u * bucket + i];
-    if (c == 0u) return kInvalidCell;
-    if (c == checksum) return 32u * bucket + i;
+  const uint4* blk = reinterpret_cast<const uint4*>(h.prv + 32u * bucket);
+  uint4 v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = blk[q];
+  uint32_t empty = 0u, hit = 0u;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint32_t c[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      empty |= (c[j] == 0u ? 1u : 0u) << (4 * q + j);
+      hit |= (c[j] == checksum ? 1u : 0u) << (4 * q + j);
+    }
   }
-  return kInvalidCell;
+  if (hit == 0u) return kInvalidCell;
+  const uint32_t first_hit = (uint32_t)__builtin_ctz(hit);
+  const uint32_t first_empty = empty ? (uint32_t)__builtin_ctz(empty) : 32u;
+  return first_hit < first_empty ? 32u * bucket + first_hit : kInvalidCell;
 }
 
 #if defined(__HIPCC__)

@@ -234,8 +234,9 @@ void rtpg_sdtree_t::build() {
   cp_f(dt.s_sw, dt.b_sw);
   cp_f(dt.s_sum, dt.b_sum);
   cp_f(dt.s_area, dt.b_area);
+  // no host wait: what follows (a render, the next reset's downloads) is
+  // ordered on the same stream
   publish();
-  RTPG_HIP(hipStreamSynchronize(s));
 }
 
 // RTSTreeWrapperT::Reset (:1645-1656): refine (RTSTreeT::Refine :1275-1311)
@@ -250,9 +251,15 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
     th = (size_t)((double)split_c * std::sqrt(std::pow(2.0, iter)));
   }
   uint32_t n = dt.n;
-  std::vector<float> bsw = dt.b_sw.to_vector(n, s);
-  std::vector<uint32_t> bcnt = dt.b_cnt.to_vector(n, s);
-  std::vector<uint32_t> scnt = dt.s_cnt.to_vector(n, s);
+  // the building weights and both trees' node counts, with one host wait
+  std::vector<float> bsw(n);
+  std::vector<uint32_t> bcnt(n), scnt(n);
+  if (n) {
+    RTPG_HIP(hipMemcpyAsync(bsw.data(), dt.b_sw.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    RTPG_HIP(hipMemcpyAsync(bcnt.data(), dt.b_cnt.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    RTPG_HIP(hipMemcpyAsync(scnt.data(), dt.s_cnt.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  }
+  RTPG_HIP(hipStreamSynchronize(s));
   // Host topology with per-leaf (building weight, source DTree).
   struct HN {
     bool leaf;
@@ -405,7 +412,6 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
   upload_stree();
   std::swap(dt, spare);
   publish();
-  RTPG_HIP(hipStreamSynchronize(s));
 }
 
 // ------------------------------------------------------------------ C ABI

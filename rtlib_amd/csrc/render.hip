@@ -83,25 +83,42 @@ __device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tm
 constexpr uint32_t kLeafEntry = 0x80000000u;
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // never a node (< 2^31) nor a leaf (first < 2^27)
 
+// A pointer into LDS (address space 3): accesses through it compile to
+// ds_read / ds_write instead of flat loads and stores, which take the slower
+// vector-memory path.  An entry is one 64-bit word (low: entry, high: bits
+// of its distance).
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
 struct StackRef {
-  uint2* lds;        // this lane's column of the LDS short stack
+  lds_u64* lds;      // this lane's column of the LDS short stack
   uint2* deep;       // this ray's column of the spill stack (stride = deep_stride)
   uint32_t deep_stride;
   uint32_t cap;      // total capacity
 };
 
+// The spill side uses nontemporal accesses: besides streaming the rarely
+// used spill entries, this keeps the compiler from merging the two sides into
+// one access through a selected generic pointer (a flat load or store, which
+// takes the slower vector-memory path even for the LDS entries).
 __device__ __forceinline__ void stack_push(const StackRef& s, uint32_t& sp, uint32_t e, float t) {
   uint2 v = make_uint2(e, fbits(t));
   if (sp < kShortStack) {
-    s.lds[sp * kBlock] = v;
+    s.lds[sp * kBlock] = ((unsigned long long)v.y << 32) | v.x;
   } else {
-    s.deep[(size_t)(sp - kShortStack) * s.deep_stride] = v;
+    uint2* d = s.deep + (size_t)(sp - kShortStack) * s.deep_stride;
+    __builtin_nontemporal_store(v.x, &d->x);
+    __builtin_nontemporal_store(v.y, &d->y);
   }
   ++sp;
 }
 __device__ __forceinline__ uint2 stack_pop(const StackRef& s, uint32_t& sp) {
   --sp;
-  return sp < kShortStack ? s.lds[sp * kBlock] : s.deep[(size_t)(sp - kShortStack) * s.deep_stride];
+  if (sp < kShortStack) {
+    unsigned long long w = s.lds[sp * kBlock];
+    return make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+  }
+  const uint2* d = s.deep + (size_t)(sp - kShortStack) * s.deep_stride;
+  return make_uint2(__builtin_nontemporal_load(&d->x), __builtin_nontemporal_load(&d->y));
 }
 
 #define RTPG_CS(a, b)                         \
@@ -240,7 +257,7 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
 __device__ __forceinline__ StackRef make_stack(uint2 (*lds)[kBlock], uint32_t* deep, uint32_t deep_stride,
                                                uint32_t slot, uint32_t cap) {
   StackRef s;
-  s.lds = &lds[0][threadIdx.x];
+  s.lds = (lds_u64*)&lds[0][threadIdx.x];
   s.deep = deep ? reinterpret_cast<uint2*>(deep) + slot : nullptr;
   s.deep_stride = deep_stride;
   s.cap = cap;
@@ -303,8 +320,11 @@ __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t a, 
   }
 }
 
+// 8 waves per SIMD (64 VGPRs): traversal hides its dependent node fetches
+// with occupancy.
 template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
   __shared__ uint2 lds[kShortStack][kBlock];
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t n_act = ps.count[depth];

@@ -199,7 +199,10 @@ struct Collapser {
     }
     BvhNode4& n = res.nodes[me];
     for (int i = 0; i < 4; ++i) {
-      if (i < nk) {
+      // an empty leaf (the tiny-scene path) is an empty slot
+      const bool used = i < nk && !(child_ref[i] < 0 && child_cnt[i] == 0);
+      n.pad[i] = 0;
+      if (used) {
         const Box& bx = bin[kids[i]].box;
         n.lo_x[i] = bx.lo[0];
         n.hi_x[i] = bx.hi[0];
@@ -207,14 +210,13 @@ struct Collapser {
         n.hi_y[i] = bx.hi[1];
         n.lo_z[i] = bx.lo[2];
         n.hi_z[i] = bx.hi[2];
-        n.child[i] = child_ref[i];
-        n.count[i] = child_cnt[i];
+        n.entry[i] = child_ref[i] >= 0 ? (uint32_t)child_ref[i]
+                                       : (kLeafEntry | ((child_cnt[i] - 1u) << 27) | (uint32_t)(~child_ref[i]));
       } else {
         n.lo_x[i] = n.hi_x[i] = kEmptyBox;
         n.lo_y[i] = n.hi_y[i] = kEmptyBox;
         n.lo_z[i] = n.hi_z[i] = kEmptyBox;
-        n.child[i] = 0;
-        n.count[i] = 0;
+        n.entry[i] = 0;
       }
     }
     return me;
@@ -330,15 +332,6 @@ BvhBuildResult build_bvh4(const float* verts, uint32_t nv, const uint32_t* idx, 
   }
   Collapser col(bin, refs, res);
   col.emit(0, 1, 0);
-  // Empty leaves from the tiny-scene path get a far-away box.
-  for (auto& n : res.nodes) {
-    for (int i = 0; i < 4; ++i) {
-      if (n.child[i] < 0 && n.count[i] == 0) {
-        n.lo_x[i] = n.hi_x[i] = n.lo_y[i] = n.hi_y[i] = n.lo_z[i] = n.hi_z[i] = kEmptyBox;
-        n.child[i] = 0;
-      }
-    }
-  }
   res.tris.resize(nt);
   res.order.resize(nt);
   for (uint32_t s = 0; s < nt; ++s) {

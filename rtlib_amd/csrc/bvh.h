@@ -12,10 +12,16 @@
 
 namespace rtpg {
 
+// A child of a node, as the traversal stacks it: an inner node is its index
+// (< 2^31); a leaf is kLeafEntry | (triangles - 1) << 27 | first triangle
+// (first < 2^27, at most 16 triangles).  Empty slots hold 0 and a box no ray
+// enters.
+constexpr uint32_t kLeafEntry = 0x80000000u;
+
 struct alignas(64) BvhNode4 {
   float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
-  int32_t child[4];    // >= 0: inner node index; < 0: leaf, first triangle = ~child
-  uint32_t count[4];   // triangles in a leaf child, 0 otherwise
+  uint32_t entry[4];   // the children as traversal entries
+  uint32_t pad[4];     // keeps the node at two 64-byte lines
 };
 static_assert(sizeof(BvhNode4) == 128, "BVH4 node must be 128 bytes");
 

@@ -78,9 +78,8 @@ __device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tm
   return true;
 }
 
-// Traversal stack entries: (node or leaf, entry distance of its box).  An
-// inner node is its index; a leaf is kLeafEntry | (count - 1) << 27 | first.
-constexpr uint32_t kLeafEntry = 0x80000000u;
+// Traversal stack entries: (node or leaf, entry distance of its box), the
+// node or leaf word as the BVH stores it (kLeafEntry in bvh.h).
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // never a node (< 2^31) nor a leaf (first < 2^27)
 
 // A pointer into LDS (address space 3): accesses through it compile to
@@ -174,8 +173,7 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
       if (kCount) ++wcnt[0];
       const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
       float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      int4 ch = reinterpret_cast<const int4*>(np)[6];
-      uint4 cn = reinterpret_cast<const uint4*>(np)[7];
+      uint4 ch = reinterpret_cast<const uint4*>(np)[6];
       float tk[4];
       uint32_t nk[4];
       int n_hit = 0;
@@ -189,10 +187,8 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
         float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
         bool hit = tn <= tf * 1.00000024f;
-        int32_t c = i == 0 ? ch.x : (i == 1 ? ch.y : (i == 2 ? ch.z : ch.w));
-        uint32_t cnt = i == 0 ? cn.x : (i == 1 ? cn.y : (i == 2 ? cn.z : cn.w));
         tk[i] = hit ? tn : 3.0e38f;
-        nk[i] = c >= 0 ? (uint32_t)c : (kLeafEntry | ((cnt - 1u) << 27) | (uint32_t)(~c));
+        nk[i] = i == 0 ? ch.x : (i == 1 ? ch.y : (i == 2 ? ch.z : ch.w));
         n_hit += hit ? 1 : 0;
       }
       if (n_hit == 0) {
@@ -207,7 +203,11 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
         } else {
           cur = nk[0];
           cur_t = tk[0];
-          for (int k = n_hit - 1; k >= 1; --k) stack_push(st, sp, nk[k], tk[k]);
+          // the other hit children, far to near (static indices: no
+          // register-array selects)
+#pragma unroll
+          for (int k = 3; k >= 1; --k)
+            if (k < n_hit) stack_push(st, sp, nk[k], tk[k]);
         }
       }
       if (cur != kNoEntry && (cur & kLeafEntry) && pleaf == kNoEntry) {
@@ -894,8 +894,10 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
   }
 }
 
+// Any-hit traversal held to 8 waves per SIMD like k_extend.
 template <bool kCount, bool kAdj = false>
-__global__ void __launch_bounds__(kBlock) k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdj ? 1 : 8)))
+k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
   __shared__ uint2 lds[kShortStack][kBlock];
   __shared__ LdsAcc3<kAdj ? 256u : 1u> acc_le;
   if (kAdj) {

@@ -34,7 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Msamples/s + relMSE at equal time: guided (SD-tree/MLP) vs plain PT, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 128      # one BVH4 node (four child boxes + child refs)
+NODE_BYTES = 112      # what the traversal reads of a 128-byte BVH4 node: four child boxes + child entries
 TRI_BYTES = 36        # three float3 vertices per tested triangle
 RAY_IN_BYTES = 32     # origin + direction
 HIT_OUT_BYTES = 16    # t, u, v, triangle

@@ -177,6 +177,7 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
       float tk[4];
       uint32_t nk[4];
       int n_hit = 0;
+      uint32_t hmask = 0;  // children whose box the ray enters
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         float lox = f4c(lx, i), hix = f4c(hx, i), loy = f4c(ly, i), hiy = f4c(hy, i);
@@ -190,17 +191,23 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
         tk[i] = hit ? tn : 3.0e38f;
         nk[i] = i == 0 ? ch.x : (i == 1 ? ch.y : (i == 2 ? ch.z : ch.w));
         n_hit += hit ? 1 : 0;
+        hmask |= hit ? 1u << i : 0u;
       }
       if (n_hit == 0) {
         pop_valid();
       } else {
-        RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
+        // Closest hit visits the children nearest first (a sorting network
+        // on the entry distances); any hit needs no order, as every entered
+        // box is visited or culled against the same tmax until a hit ends it.
+        if (!kAny) {
+          RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
+        }
         if (sp + (uint32_t)(n_hit - 1) > st.cap) {
           atomicOr(err, 1u);  // stack bound of the build violated: stop this ray
           cur = kNoEntry;
           pleaf = kNoEntry;
           sp = 0;
-        } else {
+        } else if (!kAny) {
           cur = nk[0];
           cur_t = tk[0];
           // the other hit children, far to near (static indices: no
@@ -208,6 +215,18 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
 #pragma unroll
           for (int k = 3; k >= 1; --k)
             if (k < n_hit) stack_push(st, sp, nk[k], tk[k]);
+        } else {
+          // the lowest entered child goes on, the others are stacked
+          const int first = __builtin_ctz(hmask);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            if (k == first) {
+              cur = nk[k];
+              cur_t = tk[k];
+            } else if (hmask & (1u << k)) {
+              stack_push(st, sp, nk[k], tk[k]);
+            }
+          }
         }
       }
       if (cur != kNoEntry && (cur & kLeafEntry) && pleaf == kNoEntry) {

@@ -197,9 +197,11 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
         pop_valid();
       } else {
         // Closest hit visits the children nearest first (a sorting network
-        // on the entry distances); any hit needs no order, as every entered
-        // box is visited or culled against the same tmax until a hit ends it.
-        if (!kAny) {
+        // on the entry distances) when some lane of the wave entered more
+        // than one; any hit needs no order, as every entered box is visited
+        // or culled against the same tmax until a hit ends it.
+        const bool sorted = !kAny && __any(n_hit > 1);
+        if (sorted) {
           RTPG_CS(0, 1) RTPG_CS(2, 3) RTPG_CS(0, 2) RTPG_CS(1, 3) RTPG_CS(1, 2)
         }
         if (sp + (uint32_t)(n_hit - 1) > st.cap) {
@@ -207,7 +209,7 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
           cur = kNoEntry;
           pleaf = kNoEntry;
           sp = 0;
-        } else if (!kAny) {
+        } else if (sorted) {
           cur = nk[0];
           cur_t = tk[0];
           // the other hit children, far to near (static indices: no

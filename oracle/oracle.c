@@ -266,6 +266,28 @@ void orc_grid_find(const orc_grid* g, uint32_t* cur, const uint32_t* prv, const 
   }
 }
 
+/* One position's slot for the integrator: FindFromPrv (read-only probe of the
+ * previous table, stops at an empty slot, GetPrvCellIndex :298-319) and
+ * FindFromCur (insert with compare-and-swap 0 -> checksum, the device branch
+ * of GetCurCellIndex :266-297; threads may insert concurrently). */
+static uint32_t grid_find_prv1(const uint32_t* prv, uint32_t bucket, uint32_t cs) {
+  for (uint32_t k = 0; k < 32u; ++k) {
+    uint32_t c = prv[32u * bucket + k];
+    if (c == 0u) return INVALID;
+    if (c == cs) return 32u * bucket + k;
+  }
+  return INVALID;
+}
+static uint32_t grid_find_cur1(uint32_t* cur, uint32_t bucket, uint32_t cs) {
+  for (uint32_t k = 0; k < 32u; ++k) {
+    uint32_t expected = 0u;
+    uint32_t idx = 32u * bucket + k;
+    if (__atomic_compare_exchange_n(&cur[idx], &expected, cs, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return idx;
+    if (expected == cs) return idx;
+  }
+  return INVALID;
+}
+
 /* ======================================================================
  * Morton quadtree, Utils/OPX7UtilsMorton.h:253-449, OPX7UtilsMortonBuild.cu
  * ==================================================================== */
@@ -300,50 +322,54 @@ void orc_mq_build(float* w, uint32_t level, uint32_t per_cell, uint32_t cells) {
   }
 }
 
-/* MortonQuadTreeT::SampleAndPdf :286-344 (pdf times 1/(4 pi), wrapper :486-500) */
+/* MortonQuadTreeT::SampleAndPdf :286-344 on one pyramid b, drawing from r:
+ * canonical (u, v) and the sphere density (pdf times 1/(4 pi), wrapper
+ * :486-500).  An empty pyramid (root <= 0) samples the square uniformly. */
+static float mq_sample1(const float* b, uint32_t level, pcg32* r, float* ou, float* ov) {
+  float total = b[0];
+  if (total <= 0.0f) {
+    *ou = rnd(r);
+    *ov = rnd(r);
+    return 1.0f * 0.25f * INV_PI_F;
+  }
+  uint32_t size = 4, offset = 1, code = 0;
+  for (uint32_t l = 0; l < level; ++l) {
+    float wv[4];
+    for (int k = 0; k < 4; ++k) wv[k] = b[offset + (code << 2) + (uint32_t)k];
+    uint32_t local = 0;
+    float partial = wv[0] + wv[2];
+    float boundary = partial / total;
+    float sample = rnd(r);
+    if (sample < boundary) {
+      sample /= boundary;
+      boundary = wv[0] / partial;
+    } else {
+      partial = wv[1] + wv[3];
+      sample = (sample - boundary) / (1.0f - boundary);
+      boundary = wv[1] / partial;
+      local |= 1u;
+    }
+    if (sample >= boundary) local |= 2u;
+    total = wv[local];
+    code = (code << 2) + local;
+    offset += size;
+    size *= 4u;
+  }
+  uint32_t xi, yi;
+  morton2_decode(code, &xi, &yi);
+  float nn = (float)(1u << level);
+  float a = rnd(r);
+  float c = rnd(r);
+  *ou = ((float)xi + a) / nn;
+  *ov = ((float)yi + c) / nn;
+  return total / b[0] * (nn * nn) * 0.25f * INV_PI_F;
+}
+
 void orc_mq_sample(const float* w, uint32_t per_cell, uint32_t level, const uint32_t* slot, uint32_t n,
                    uint64_t seed, float* out) {
   for (uint32_t i = 0; i < n; ++i) {
     pcg32 r = pcg32_seed(seed, i);
-    const float* b = w + (size_t)per_cell * slot[i];
-    float total = b[0];
-    if (total <= 0.0f) {
-      out[3 * i] = rnd(&r);
-      out[3 * i + 1] = rnd(&r);
-      out[3 * i + 2] = 1.0f * 0.25f * INV_PI_F;
-      continue;
-    }
-    uint32_t size = 4, offset = 1, code = 0;
-    for (uint32_t l = 0; l < level; ++l) {
-      float wv[4];
-      for (int k = 0; k < 4; ++k) wv[k] = b[offset + (code << 2) + (uint32_t)k];
-      uint32_t local = 0;
-      float partial = wv[0] + wv[2];
-      float boundary = partial / total;
-      float sample = rnd(&r);
-      if (sample < boundary) {
-        sample /= boundary;
-        boundary = wv[0] / partial;
-      } else {
-        partial = wv[1] + wv[3];
-        sample = (sample - boundary) / (1.0f - boundary);
-        boundary = wv[1] / partial;
-        local |= 1u;
-      }
-      if (sample >= boundary) local |= 2u;
-      total = wv[local];
-      code = (code << 2) + local;
-      offset += size;
-      size *= 4u;
-    }
-    uint32_t xi, yi;
-    morton2_decode(code, &xi, &yi);
-    float nn = (float)(1u << level);
-    float a = rnd(&r);
-    float c = rnd(&r);
-    out[3 * i] = ((float)xi + a) / nn;
-    out[3 * i + 1] = ((float)yi + c) / nn;
-    out[3 * i + 2] = total / b[0] * (nn * nn) * 0.25f * INV_PI_F;
+    out[3 * i + 2] = mq_sample1(w + (size_t)per_cell * slot[i], level, &r, &out[3 * i], &out[3 * i + 1]);
   }
 }
 
@@ -1382,7 +1408,8 @@ static int64_t trace(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int
  * wavefront kernels (NEE with one light sample or RIS, MIS by the power
  * heuristic, Russian roulette, SD-tree guided mixture and record).
  * ==================================================================== */
-enum { F_NEE = 1, F_RIS = 2, F_USE_TREE = 8, F_GUIDE = 16, F_FINAL = 32, F_MIS = 128, F_RR = 256 };
+enum { F_NEE = 1, F_RIS = 2, F_USE_GRID = 4, F_USE_TREE = 8, F_GUIDE = 16, F_FINAL = 32, F_LOCATE = 64, F_MIS = 128,
+       F_RR = 256 };
 enum { M_LAMBERT = 0, M_GGX = 1, M_PHONG = 2, M_GLASS = 3 };
 
 static float ggx_d(float c, float a2) {
@@ -1548,8 +1575,8 @@ typedef struct {
 
 enum { VTX_VALID = 1, VTX_DELTA = 2, VTX_BAD_BSDF = 4 };
 
-static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_sdtree* g, uint32_t pix, uint32_t smp,
-                            v3* Lout, adjrec* ar) {
+static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_sdtree* g, const orc_hashguide* hg,
+                            uint32_t pix, uint32_t smp, v3* Lout, adjrec* ar) {
   uint64_t rays = 0;
   pcg32 r = path_rng(p->seed, pix, p->sample_offset + smp);
   uint32_t x = pix % p->width, y = pix / p->width;
@@ -1565,7 +1592,9 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
   const int ris = nee && (p->flags & F_RIS);
   const int mis = nee && !ris && (p->flags & F_MIS);
   const int use_tree = g != NULL && (p->flags & F_USE_TREE);
-  const int record = use_tree && !(p->flags & F_FINAL);
+  /* the hash-grid guide of SimpleGuide2.cu (USE_GRID) */
+  const int use_grid = !use_tree && hg != NULL && (p->flags & F_USE_GRID);
+  const int record = (use_tree || use_grid) && !(p->flags & F_FINAL);
   const float alpha = p->fraction;
   v3 cbuf[64];
   vtx vb[64];
@@ -1680,11 +1709,27 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
           next_delta = 1;
           cont = max3(Tn) > 0.0f;
         } else {
-          uint32_t cell = INVALID;
-          if (use_tree) cell = stree_lookup1(g->stree, g->aabb_min, g->size, pos);
-          const int guided = use_tree && (p->flags & F_GUIDE) && cell != INVALID;
-          const float* gs = guided ? g->s_sums + 4 * (size_t)g->s_base[cell] : NULL;
-          const uint16_t* gc = guided ? g->s_child + 4 * (size_t)g->s_base[cell] : NULL;
+          /* cell: where this vertex records; gcell: the guide it samples from.
+           * The hash grid (SimpleGuide2.cu:319-323, 420-428) samples the
+           * previous table's slot and records there, or, in a Locate pass,
+           * into the slot a position missing from the previous table inserts
+           * into the current one. */
+          uint32_t cell = INVALID, gcell = INVALID;
+          if (use_tree) {
+            cell = stree_lookup1(g->stree, g->aabb_min, g->size, pos);
+            gcell = cell;
+          } else if (use_grid) {
+            float pp[3] = {pos.x, pos.y, pos.z};
+            uint32_t c3[3], bk, cs;
+            orc_grid_cell_hash(&hg->grid, pp, c3, &bk, &cs);
+            gcell = grid_find_prv1(hg->prv, bk, cs);
+            cell = gcell;
+            if ((p->flags & F_LOCATE) && gcell == INVALID) cell = grid_find_cur1(hg->cur, bk, cs);
+          }
+          const int guided = (use_tree || use_grid) && (p->flags & F_GUIDE) && gcell != INVALID;
+          const float* gs = guided && use_tree ? g->s_sums + 4 * (size_t)g->s_base[cell] : NULL;
+          const uint16_t* gc = guided && use_tree ? g->s_child + 4 * (size_t)g->s_base[cell] : NULL;
+          const float* mw = guided && use_grid ? hg->s_w + (size_t)hg->per_cell * gcell : NULL;
           if (nee) {
             if (ris) {
               float wsum = 0.0f, fa_y = 0.0f, dist_y = 0.0f;
@@ -1732,7 +1777,8 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
                   if (guided) {
                     float cu, cv;
                     dir_to_canonical(dir, &cu, &cv);
-                    float gp = dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], cu, cv);
+                    float gp = use_tree ? dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], cu, cv)
+                                        : mq_pdf1(mw, hg->level, cu, cv);
                     bp = (1.0f - alpha) * bp + alpha * gp;
                   }
                   w = power_h(pl, bp);
@@ -1755,7 +1801,13 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
             float us = rnd(&r);
             if (us < alpha) {
               float gu, gv;
-              dtree_sample1(gs, gc, g->s_sum[cell], g->s_sw[cell], &r, &gu, &gv, &tree_pdf);
+              if (use_tree) {
+                dtree_sample1(gs, gc, g->s_sum[cell], g->s_sw[cell], &r, &gu, &gv, &tree_pdf);
+              } else {
+                /* MortonQuadTreeWrapperT::SampleAndPdf (:486-500): the density
+                 * of the sampled bin */
+                tree_pdf = mq_sample1(mw, hg->level, &r, &gu, &gv);
+              }
               dir = nrm(canonical_to_dir(gu, gv));
             } else {
               dir = bsdf_sample(type, albedo, param, spec, n, wo, &r);
@@ -1767,7 +1819,9 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
           wo_pdf = be.pdf;
           if (guided) {
             dir_to_canonical(dir, &vx.cu, &vx.cv);
-            float gp = tree_pdf >= 0.0f ? tree_pdf : dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], vx.cu, vx.cv);
+            float gp = tree_pdf >= 0.0f ? tree_pdf
+                       : use_tree ? dtree_pdf1(gs, gc, g->s_sum[cell], g->s_sw[cell], vx.cu, vx.cv)
+                                  : mq_pdf1(mw, hg->level, vx.cu, vx.cv);
             wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
           } else if (record && cell != INVALID) {
             dir_to_canonical(dir, &vx.cu, &vx.cv);
@@ -1829,6 +1883,14 @@ static uint64_t render_path(const orc_scene* s, const orc_params* p, const orc_s
       float value = ((lx + ly + lz) / 3.0f) / v->wo_pdf;
       uint32_t cell = v->cell;
       float cu = v->cu, cv = v->cv;
+      if (use_grid) {
+        /* MortonTraceVertexT::Commit (OPX7UtilsMorton.h:550-585) ->
+         * MortonQuadTreeT::Record, leaf-only branch (:262-285) */
+        uint32_t nn = 1u << hg->level;
+        size_t at = (size_t)hg->per_cell * cell + level_offset(hg->level) + orc_morton2(canon_bin(cu, nn), canon_bin(cv, nn));
+        atomic_addf(&hg->b_w[at], value);
+        continue;
+      }
       if (g->filter) {
         /* stochastic box filters, own stream per (pixel, sample, vertex depth) */
         pcg32 fr = filter_rng(p->seed, pix, p->sample_offset + smp, dv);
@@ -1859,7 +1921,7 @@ void orc_adjoint(const orc_scene* s, const orc_params* p, const float* dldi, dou
                    dldi[3 * pix + 2] / (double)p->spp};
     for (uint32_t smp = 0; smp < p->spp; ++smp) {
       v3 L;
-      render_path(s, p, NULL, pix, smp, &L, ar);
+      render_path(s, p, NULL, NULL, pix, smp, &L, ar);
       for (uint32_t k = 0; k < ar->nh; ++k) {
         const float* cs[2] = {&ar->e[k].x, &ar->n[k].x};
         int32_t ls[2] = {ar->el[k], ar->nl[k]};
@@ -1892,6 +1954,7 @@ typedef struct {
   const orc_scene* s;
   const orc_params* p;
   const orc_sdtree* g;
+  const orc_hashguide* hg;
   float* accum;
   uint32_t next;
   uint64_t rays;
@@ -1911,7 +1974,7 @@ static void* worker(void* arg) {
       float sx = 0.0f, sy = 0.0f, sz = 0.0f;
       for (uint32_t smp = 0; smp < j->p->spp; ++smp) {
         v3 L;
-        rays += render_path(j->s, j->p, j->g, pix, smp, &L, NULL);
+        rays += render_path(j->s, j->p, j->g, j->hg, pix, smp, &L, NULL);
         sx += L.x;
         sy += L.y;
         sz += L.z;
@@ -1975,11 +2038,13 @@ void orc_debug_frame(const orc_scene* s, const orc_params* p, uint32_t type, flo
   }
 }
 
-uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum) {
+static uint64_t render_job(const orc_scene* s, const orc_params* p, const orc_sdtree* guide,
+                           const orc_hashguide* hg, float* accum) {
   job j;
   j.s = s;
   j.p = p;
   j.g = guide;
+  j.hg = hg;
   j.accum = accum;
   j.next = 0;
   j.rays = 0;
@@ -1991,4 +2056,66 @@ uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* g
   for (uint32_t i = 0; i < nth; ++i) pthread_join(th[i], NULL);
   pthread_mutex_destroy(&j.mu);
   return j.rays;
+}
+
+uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum) {
+  return render_job(s, p, guide, NULL, accum);
+}
+
+uint64_t orc_render_grid(const orc_scene* s, const orc_params* p, const orc_hashguide* guide, float* accum) {
+  return render_job(s, p, NULL, guide, accum);
+}
+
+/* ======================================================================
+ * pass controllers: RTSTreeControllerT::BegTrace/EndTrace
+ * (Utils/OPX7UtilsPathGuiding.h:1805-1882) and RTMortonQuadTreeControllerT::
+ * BegTrace/EndTrace (Utils/OPX7UtilsMorton.h:806-905), one launch at a time
+ * as RTLibExtOPX7TestApplication::TracePipeline drives them (:2155-2231,
+ * including its hash-grid snapshot when GetSamplePerTmp() is 0 after EndTrace).
+ * ==================================================================== */
+uint32_t orc_ctrl_schedule(int morton, uint32_t budget, uint32_t iter_built, uint32_t iter_locate, float ratio,
+                           uint32_t spl, uint32_t cap, uint32_t* state, uint32_t* iter, uint32_t* pass, uint32_t* ev) {
+  uint32_t per_all = 0, per_tmp = 0, cur = 0, sfp = 0;
+  uint32_t remain = ((budget - 1u + spl) / spl) * spl;
+  uint32_t n = 0;
+  int executing = 1;
+  while (executing && n < cap) {
+    uint32_t e = 0;
+    /* BegTrace */
+    if (per_tmp == 0) {
+      remain -= sfp;
+      uint64_t want = (uint64_t)(1ull << (cur < 63u ? cur : 63u)) * spl;
+      sfp = want < remain ? (uint32_t)want : remain;
+      if ((remain - sfp < 2u * sfp) || ((float)per_all >= ratio * (float)budget)) sfp = remain;
+      if (!morton && remain > sfp) e |= 1u;
+    }
+    uint32_t st;
+    if (cur > iter_built) {
+      st = remain > sfp ? 1u : 2u;
+    } else if (!morton || cur > iter_locate) {
+      st = 0u;
+    } else {
+      st = 3u;
+    }
+    state[n] = st;
+    iter[n] = cur;
+    pass[n] = sfp;
+    /* EndTrace */
+    per_all += spl;
+    per_tmp += spl;
+    if (per_tmp >= sfp) {
+      e |= 2u;
+      ++cur;
+      if (morton && cur > iter_built) e |= 4u;
+      per_tmp = 0;
+    }
+    if (morton && per_tmp == 0) e |= 8u;
+    if (per_all > budget) {
+      executing = 0;
+      per_all = 0;
+    }
+    ev[n] = e;
+    ++n;
+  }
+  return n;
 }

@@ -131,14 +131,41 @@ typedef struct orc_sdtree {
   const uint16_t* b_child;
   uint32_t filter; /* recording filters: 1 spatial box, 2 directional box (stochastic form) */
 } orc_sdtree;
+/* Hash-grid guide view for the integrator (SimpleGuide2.cu): the grid, its
+ * current table (inserted into in a Locate pass) and previous table (probed),
+ * and the Morton pyramids of level `level` (per_cell floats per slot):
+ * sampling pyramids read, building pyramids recorded into. */
+typedef struct orc_hashguide {
+  orc_grid grid;
+  uint32_t* cur;
+  const uint32_t* prv;
+  uint32_t level, per_cell;
+  const float* s_w;
+  float* b_w;
+} orc_hashguide;
 /* Debug frame of type 0-6 or 8 (raw quantity per pixel, 3 floats) and the
  * hit triangle per pixel (-1: miss). */
 void orc_debug_frame(const orc_scene* s, const orc_params* p, uint32_t type, float* out3, int32_t* tri_out);
 /* accum: 3*W*H floats, added to; returns the number of radiance rays traced. */
 uint64_t orc_render(const orc_scene* s, const orc_params* p, const orc_sdtree* guide, float* accum);
+/* The same integrator guided by the hash grid (RTPG flags USE_GRID, LOCATE,
+ * GUIDE_SAMPLE, FINAL as in rtpg_render). */
+uint64_t orc_render_grid(const orc_scene* s, const orc_params* p, const orc_hashguide* guide, float* accum);
 /* FP64-accumulated gradient of sum_pixels dldi . image w.r.t. per-triangle
  * albedo (3*nt, Lambertian triangles) and per-light radiance (3*nl). */
 void orc_adjoint(const orc_scene* s, const orc_params* p, const float* dldi, double* grad_alb, double* grad_le);
+
+/* ---- pass controllers ----
+ * RTSTreeControllerT (morton = 0) or RTMortonQuadTreeControllerT (morton = 1)
+ * driven one launch of spl samples at a time, as the application drives them,
+ * until the controller stops executing.  Per launch k: the state in force
+ * (0 Record, 1 RecordAndSample, 2 Sample, 3 Locate), the iteration, the
+ * samples of the current pass, and event bits: 1 the SD-tree is Reset before
+ * the launch, 2 the launch ends a pass (SD-tree Build), 4 Morton Update and
+ * pyramid build, 8 the application's hash-grid snapshot.  Returns the number
+ * of launches (at most cap). */
+uint32_t orc_ctrl_schedule(int morton, uint32_t budget, uint32_t iter_built, uint32_t iter_locate, float ratio,
+                           uint32_t spl, uint32_t cap, uint32_t* state, uint32_t* iter, uint32_t* pass, uint32_t* ev);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,11 @@ class Grid(C.Structure):
                 ("slots", C.c_uint32), ("key_mode", C.c_uint32)]
 
 
+class HashGuideS(C.Structure):
+    _fields_ = [("grid", Grid), ("cur", u32p), ("prv", u32p), ("level", C.c_uint32), ("per_cell", C.c_uint32),
+                ("s_w", f32p), ("b_w", f32p)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -80,6 +85,11 @@ def lib():
         L.orc_debug_frame.argtypes = [C.c_void_p, C.POINTER(Params), C.c_uint32, f32p, C.POINTER(C.c_int32)]
         L.orc_render.restype = C.c_uint64
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(SdTree), f32p]
+        L.orc_render_grid.restype = C.c_uint64
+        L.orc_render_grid.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(HashGuideS), f32p]
+        L.orc_ctrl_schedule.restype = C.c_uint32
+        L.orc_ctrl_schedule.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                                        C.c_uint32, u32p, u32p, u32p, u32p]
         L.orc_adjoint.argtypes = [C.c_void_p, C.POINTER(Params), f32p, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
         _lib = L
@@ -399,7 +409,8 @@ def tex_sample(texels, tex, u, v):
 
 def render(scene: Scene, camera_uvw, width, height, spp, flags, max_depth=8, rr_depth=3, num_candidates=32, seed=0,
            sample_offset=0, bg=(0, 0, 0), fraction=0.3, px_range=None, threads=None, guide=None, accum=None):
-    """Renders with the CPU integrator; returns (accum H x W x 3 sums, rays)."""
+    """Renders with the CPU integrator; returns (accum H x W x 3 sums, rays).
+    guide: a Guide (SD-tree) or a HashGuide (hash grid + Morton pyramids)."""
     p = Params()
     p.width, p.height, p.flags, p.max_depth, p.rr_depth = width, height, flags, max_depth, rr_depth
     p.num_candidates, p.spp, p.sample_offset, p.seed = num_candidates, spp, sample_offset, seed
@@ -411,6 +422,9 @@ def render(scene: Scene, camera_uvw, width, height, spp, flags, max_depth=8, rr_
     p.threads = threads or os.cpu_count() or 1
     if accum is None:
         accum = np.zeros((height, width, 3), np.float32)
+    if isinstance(guide, HashGuide):
+        rays = lib().orc_render_grid(scene.h, C.byref(p), C.byref(guide.struct), _p(accum))
+        return accum, rays
     gp = None
     if guide is not None:
         gp = C.byref(guide.struct)
@@ -484,3 +498,48 @@ class Guide:
         g.b_child = _p(self.b_child, u16p)
         g.filter = filter
         self.struct = g
+
+
+class HashGuide:
+    """Binds hash-grid tables and Morton pyramids to the integrator: cur is
+    inserted into in a Locate pass, prv probed, the sampling pyramids s_w read
+    and the building pyramids b_w recorded into (all in place)."""
+
+    def __init__(self, origin, size, bounds, slots, key_mode, cur, prv, level, s_w, b_w):
+        self.cur = np.ascontiguousarray(cur, np.uint32)
+        self.prv = np.ascontiguousarray(prv, np.uint32)
+        self.s_w = np.ascontiguousarray(s_w, np.float32)
+        self.b_w = np.ascontiguousarray(b_w, np.float32)
+        assert self.cur.shape == (slots,) and self.prv.shape == (slots,)
+        per_cell = ((1 << (2 * (level + 1))) - 1) // 3
+        assert self.s_w.shape == (slots, per_cell) and self.b_w.shape == (slots, per_cell)
+        g = HashGuideS()
+        g.grid = grid(origin, size, bounds, slots, key_mode)
+        g.cur, g.prv = _p(self.cur, u32p), _p(self.prv, u32p)
+        g.level, g.per_cell = level, per_cell
+        g.s_w, g.b_w = _p(self.s_w), _p(self.b_w)
+        self.struct = g
+
+
+def ctrl_schedule(morton, budget, iter_built=0, iter_locate=0, ratio=0.5, spl=1, cap=1 << 16):
+    """Launch-by-launch schedule of the reference's pass controllers (see
+    orc_ctrl_schedule): list of (state, iteration, pass samples, events)."""
+    st, it, ps, ev = (np.zeros(cap, np.uint32) for _ in range(4))
+    n = lib().orc_ctrl_schedule(int(morton), budget, iter_built, iter_locate, float(ratio), spl, cap, _p(st, u32p),
+                                _p(it, u32p), _p(ps, u32p), _p(ev, u32p))
+    return [(int(st[k]), int(it[k]), int(ps[k]), int(ev[k])) for k in range(n)]
+
+
+def ctrl_passes(morton, budget, iter_built=0, iter_locate=0, ratio=0.5, spl=1):
+    """The schedule grouped into passes: list of (state, samples) of every pass
+    that traces samples, in order (the controllers' extra launch after the
+    budget, whose pass holds no samples, is left out)."""
+    out = []
+    for state, _, pass_spp, ev in ctrl_schedule(morton, budget, iter_built, iter_locate, ratio, spl):
+        if pass_spp == 0:
+            continue
+        if not out or out[-1][2]:
+            out.append([state, 0, False])
+        out[-1][1] += spl
+        out[-1][2] = bool(ev & 2)
+    return [(s, n) for s, n, _ in out]

@@ -40,6 +40,130 @@ def test_ggx_furnace_energy_bounds(orc):
     assert np.all(center <= 1.0 + 0.02) and center.mean() > 0.9
 
 
+def test_ggx_furnace_normal_incidence_closed_form(orc):
+    # GGX with F0 = 1 seen head-on (camera on the axis of a quad at z = -2, 10
+    # degree field of view, background 1, one bounce, BSDF sampling only):
+    # the sample weight F G (o.h) / (cos_o cos_h) is at most 1 and the
+    # reflectance at near-normal incidence is 1 - (shadowing loss), above
+    # 0.97 at roughness 0.1.  A dropped factor 2 in G1 halves it.
+    b = scenes._Builder()
+    b.quad((-2, -2, -2), (2, -2, -2), (2, 2, -2), (-2, 2, -2), (1.0, 1.0, 1.0), mat=scenes.MAT_GGX, param=0.1)
+    cam = {"eye": (0, 0, 0), "lookat": (0, 0, -1), "up": (0, 1, 0), "fovy": 10.0, "width": 8, "height": 8}
+    img, _ = _render(orc, b.finish(cam), 256, 0, bg=(1.0, 1.0, 1.0), max_depth=2)
+    assert np.all(img <= 1.0 + 1e-6) and 0.97 < img.mean() <= 1.0, img.mean()
+
+
+def _facing_quad(mat, albedo, spec=None, param=0.0):
+    b = scenes._Builder()
+    b.quad((-2, -2, -2), (2, -2, -2), (2, 2, -2), (-2, 2, -2), albedo, mat=mat, param=param)
+    cam = {"eye": (0, 0, 0), "lookat": (0, 0, -1), "up": (0, 1, 0), "fovy": 12.0, "width": 8, "height": 8}
+    sp = None if spec is None else np.tile(np.asarray(spec, np.float32)[None], (2, 1))
+    return b.finish(cam, spec=sp), cam
+
+
+def test_phong_furnace_closed_form(orc):
+    # The reference's normalised Phong (SimpleTrace.cu:24-44, 271-298) under a
+    # white background, one bounce, BSDF sampling only, on a quad seen almost
+    # head-on: the diffuse lobe returns its albedo a exactly in expectation,
+    # the specular lobe ((s+2)/(s+1)) (s+1)/(2 pi) cos^s about the mirror
+    # direction returns k_s cos(alpha) at incidence alpha while the lobe stays
+    # above the horizon.  So E[L] = a + k_s cos(alpha) per pixel.  A dropped
+    # (s+2)/(s+1), a wrong lobe normalisation or a lobe-selection probability
+    # that differs between sampling and density all move the mean.
+    sc, cam = _facing_quad(scenes.MAT_PHONG, (0.6, 0.6, 0.6), spec=(0.3, 0.3, 0.3, 30.0))
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 1.0)
+    spp = 4096
+    img, _ = _render(orc, sc, spp, 0, bg=(1.0, 1.0, 1.0), max_depth=2)
+    # incidence per pixel from the pixel centres (jitter averages over the
+    # pixel; cos alpha varies by < 1e-4 across one)
+    ys, xs = np.mgrid[0:8, 0:8]
+    dx, dy = 2 * (xs + 0.5) / 8 - 1, 2 * (ys + 0.5) / 8 - 1
+    d = dx[..., None] * uvw[1] + dy[..., None] * uvw[2] + uvw[3]
+    cos_a = -d[..., 2] / np.linalg.norm(d, axis=-1)
+    want = 0.6 + 0.3 * cos_a
+    # one-sample weight spread is below 1.5, so 64 x 4096 samples give a
+    # standard error of the mean under 1e-3
+    assert abs(img[..., 0].mean() - want.mean()) < 3e-3, (img[..., 0].mean(), want.mean())
+
+
+def test_dielectric_furnace_exact(orc):
+    # A clear glass sphere (specular colour 1, ior 1.5, SimpleTrace.cu:347-407)
+    # under background 1: reflection and refraction are each chosen with the
+    # probability that is their Fresnel weight, so every path that leaves the
+    # sphere returns exactly 1 and only paths cut off at the depth limit
+    # return 0.  A weight that is not exactly 1 on either branch shows here.
+    sc = scenes.furnace_sphere(albedo=1.0, mat=scenes.MAT_GLASS, roughness=1.5, width=16, height=16)
+    spp = 16
+    img, _ = _render(orc, sc, spp, 0, bg=(1.0, 1.0, 1.0), max_depth=64)
+    v = img[..., 0] * spp
+    assert np.all(v == np.round(v)) and np.all(img <= 1.0)
+    assert (img == 1.0).mean() > 0.97 and img.mean() > 0.995, ((img == 1.0).mean(), img.mean())
+
+
+def test_dielectric_reflectance_normal_incidence(orc):
+    # A glass interface (ior 1.5) seen head-on with an emitter of radiance 1
+    # behind the camera and a black background: a camera path returns 1 when
+    # it takes the reflection branch, whose probability is Schlick's
+    # F0 + (1 - F0)(1 - cos)^5 with F0 = ((1 - eta)/(1 + eta))^2 = 0.04 at
+    # cos ~ 1.  Halving the reflection probability or a wrong F0 moves the mean
+    # by 50 standard errors.
+    b = scenes._Builder()
+    b.quad((-2, -2, -2), (2, -2, -2), (2, 2, -2), (-2, 2, -2), (1.0, 1.0, 1.0), mat=scenes.MAT_GLASS, param=1.5)
+    b.quad((-3, -3, 1), (-3, 3, 1), (3, 3, 1), (3, -3, 1), (0, 0, 0), emission=(1.0, 1.0, 1.0))  # faces -z
+    cam = {"eye": (0, 0, 0), "lookat": (0, 0, -1), "up": (0, 1, 0), "fovy": 12.0, "width": 8, "height": 8}
+    img, _ = _render(orc, b.finish(cam), 2048, 0, max_depth=4)
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 1.0)
+    ys, xs = np.mgrid[0:8, 0:8]
+    d = (2 * (xs + 0.5) / 8 - 1)[..., None] * uvw[1] + (2 * (ys + 0.5) / 8 - 1)[..., None] * uvw[2] + uvw[3]
+    c = -d[..., 2] / np.linalg.norm(d, axis=-1)
+    want = 0.04 + 0.96 * (1 - c) ** 5
+    se = np.sqrt(0.04 * 0.96 / (64 * 2048))
+    assert abs(img[..., 0].mean() - want.mean()) < 4 * se, (img[..., 0].mean(), want.mean(), se)
+
+
+def _ggx_cornell():
+    sc = scenes.cornell(24, 24)
+    light = sc["emission"].sum(1) > 0
+    rng = np.random.default_rng(3)
+    sc["mat"] = np.where(light, scenes.MAT_LAMBERT, scenes.MAT_GGX).astype(np.uint32)
+    sc["albedo"][:, 3] = np.where(light, 0.0, rng.uniform(0.15, 0.5, light.size))
+    return sc, None
+
+
+def _phong_glass_two_lights():
+    # Phong walls and box, a glass box, and two light meshes of different area
+    # (0.2 and 0.09 m^2) picked by the reference's uniform mesh-then-triangle rule
+    sc, lm = scenes.cornell_materials(24, 24)
+    light = sc["emission"].sum(1) > 0
+    ph = (sc["mat"] == scenes.MAT_LAMBERT) & ~light
+    sc["mat"][ph] = scenes.MAT_PHONG
+    sc["albedo"][ph, :3] *= 0.6
+    sc["spec"][ph] = (0.3, 0.3, 0.3, 8.0)
+    return sc, lm
+
+
+@pytest.mark.parametrize("make", [_ggx_cornell, _phong_glass_two_lights], ids=["ggx", "phong_glass_mesh_lights"])
+def test_estimators_agree_on_glossy_and_mesh_lights(orc, make):
+    # BSDF sampling alone, NEE, NEE + MIS and RIS estimate the same image.  On
+    # the GGX box the NDF D enters NEE's f but cancels from BSDF sampling's
+    # f cos / pdf, so a wrong D splits them (a 10% error in D: z ~ 5); a light
+    # selection density that differs from the selection itself biases every
+    # NEE estimator (10%: z ~ 5).  8 seeds x 256 spp per estimator.
+    sc, lm = make()
+    cam = sc["camera"]
+    uvw = scenes.camera_uvw(cam["eye"], cam["lookat"], cam["up"], cam["fovy"], 1.0)
+    S = orc.Scene(sc, light_mesh=lm)
+    est = {}
+    for flags in (RR, NEE, NEE | MIS | RR, NEE | RIS):
+        means = [orc.render(S, uvw, 24, 24, 256, flags, max_depth=8, num_candidates=8, seed=100 + k)[0].mean() / 256
+                 for k in range(8)]
+        est[flags] = (np.mean(means), np.std(means, ddof=1) / np.sqrt(len(means)))
+    m0, s0 = est[RR]
+    for flags, (m, se) in est.items():
+        z = (m - m0) / np.hypot(se, s0)
+        assert abs(z) < 3.5, (flags, m, m0, z)
+
+
 @pytest.mark.parametrize("flags,rtol", [(RR, 0.05), (NEE, 0.02), (NEE | RIS, 0.02)])
 def test_cornell_estimators_agree(orc, flags, rtol):
     sc = scenes.cornell(width=24, height=24)

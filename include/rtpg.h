@@ -210,6 +210,18 @@ int rtpg_scene_lights(rtpg_context ctx, uint32_t* light_tri);
  * hits: 4*n floats (t, u, v, primitive id bits; t = inf on a miss). */
 int rtpg_trace_closest(rtpg_context ctx, const float* rays, uint32_t n, float* hits);
 int rtpg_trace_occluded(rtpg_context ctx, const float* rays, uint32_t n, uint32_t* occluded);
+/* rtpg_trace_closest without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  rays (8*n floats) and hits (4*n floats) are DEVICE
+ * pointers owned by the caller, in the layouts of rtpg_trace_closest.  The
+ * traversal's spill stack and stack-bound flag use a fixed 32 MiB device array
+ * of the library (one per GPU, held until the process ends); rays are traced in
+ * launches of as many as its stacks hold, so n is not limited, but a scene
+ * whose stack bound leaves room for fewer than 256 rays is refused
+ * (RTPG_ERR_LIMIT).  Calls on one GPU share that array: calls that overlap on
+ * one GPU (other threads, other contexts' streams) race.  Stream-ordered on the
+ * context's stream; returns after the stream has drained. */
+int rtpg_trace_closest_noalloc(rtpg_context ctx, const float* rays, uint32_t n, float* hits);
 
 /* --- SD-tree ----------------------------------------------------------------
  * Replaces RTSTreeWrapperT (Utils/OPX7UtilsPathGuiding.h:1333-1767). */

@@ -94,6 +94,7 @@ SIGNATURES = {
     "rtpg_mlp_info": [c_vp, c_u64p, c_fp],
     "rtpg_trace_closest": [c_vp, c_fp, C.c_uint32, c_fp],
     "rtpg_trace_occluded": [c_vp, c_fp, C.c_uint32, c_u32p],
+    "rtpg_trace_closest_noalloc": [c_vp, c_vp, C.c_uint32, c_vp],
     "rtpg_sdtree_create": [c_vp, c_fp, c_fp, C.c_uint32, C.c_float, C.POINTER(c_vp)],
     "rtpg_sdtree_destroy": [c_vp],
     "rtpg_sdtree_clear": [c_vp],

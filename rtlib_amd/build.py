@@ -18,7 +18,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "librtpg.so"
 
 SOURCES = ["bvh.cpp", "api.cpp", "sdtree_host.cpp", "hashgrid_host.cpp", "mlp_host.cpp", "render.hip", "sdtree.hip",
-           "hashgrid.hip", "mlp.hip"]
+           "hashgrid.hip", "mlp.hip", "trace_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 

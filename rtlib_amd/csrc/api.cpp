@@ -77,6 +77,7 @@ int rtpg_context_create(int device, rtpg_context* out) {
     RTPG_HIP(hipGetDeviceCount(&count));
     require(device >= 0 && device < count, "device index out of range");
     RTPG_HIP(hipSetDevice(device));
+    scratch_setup(device);
     auto* c = new rtpg_context_t();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);

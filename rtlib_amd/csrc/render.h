@@ -36,7 +36,7 @@ struct PathSoA {
   uint32_t* count;   // live paths per depth [max_depth + 2]
   uint32_t* scount;  // shadow rays per depth [max_depth + 2]
   float4* shadow;    // 3 x float4 per shadow ray
-  uint32_t* deep;    // traversal spill stack: (entry, distance) pairs, 2 x (cap - 16) x n words
+  uint32_t* deep;    // traversal spill stack: (entry, distance) pairs, 2 x (cap - kShortStack) x n words
   uint32_t* err;     // sticky error bits
   // guide vertices, [depth][pid]
   float4* vA;        // throughput after the bounce, woPdf
@@ -87,6 +87,10 @@ constexpr int kMaxDepthEv = 64;
 
 hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n, float4* hits, uint32_t* occl,
                              bool any, uint32_t* deep, uint32_t* err, hipStream_t s);
+
+// Records this device's address of the allocation-free ray queries' scratch
+// array (trace_noalloc.hip); called once per context creation, device current.
+void scratch_setup(int device);
 
 hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, uchar4* frame, hipStream_t s);
 

@@ -501,6 +501,7 @@ extern "C" int rtpg_adjoint(rtpg_context ctx, const rtpg_render_params* prm, con
     require(!(prm->flags & RTPG_FLAG_RR), "the adjoint needs Russian roulette off (its survival depends on albedo)");
     RTPG_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
+    ctx->p_err.zero(1, s);  // the stack-bound flag reports this call only
     uint32_t npix = ctx->width * ctx->height;
     uint32_t nl = std::max(ctx->n_lights, 1u);
     // buffers kept by the context between calls (they only grow)
@@ -618,6 +619,8 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       ml = mlp_view_of(prm->mlp, false);
     }
     hipStream_t s = ctx->stream;
+    // the stack-bound flag reports this call only
+    ctx->p_err.zero(1, s);
     ctx->p_stats.zero(4, s);
     const bool timing = (ctx->prof_mode & 1u) != 0;
     const bool counting = (ctx->prof_mode & 2u) != 0;
@@ -668,8 +671,7 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));
       for (int k = 0; k < 3; ++k) ctx->kernel_launches[k] += prm->max_depth;
       ctx->kernel_launches[3] += 1;
-      if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL) && !(guide == 2 && (prm->flags & RTPG_FLAG_LOCATE)))
-        ctx->kernel_launches[4] += guide == 3 ? 1 : 2;
+      if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL)) ctx->kernel_launches[4] += guide == 3 ? 1 : 2;
       if (guide == 3 && (prm->flags & RTPG_FLAG_GUIDE_SAMPLE) && prm->mlp->cfg.cache_res == 0)
         ctx->kernel_launches[1] += prm->max_depth;
       done += cs;

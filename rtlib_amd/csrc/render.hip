@@ -5,8 +5,9 @@
 // (SimpleTrace.cu:45-103, 214-425, 625-630; SimpleGuide.cu:48-156, 287-511;
 // SimpleGuide2.cu:48-153, 285-523).  Paths live in structure-of-arrays HBM
 // buffers; between bounces the surviving paths are compacted into a queue with
-// one atomic per wave (ballot + popcount), so each bounce launches only as
-// many work-items as there are live paths.
+// one atomic per work-group (ballot + popcount, combined in LDS), so the work
+// of each bounce is the live paths only; the grid of every bounce is sized
+// for the whole chunk and work-groups past the live count exit at once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -685,18 +686,22 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
           vflags = kVtxDelta;
           cont = max3(Tn) > 0.0f;
         } else {
+          // vcell: where this vertex records; gcell: the guide it samples from
+          uint32_t gcell = kInvalidCell;
           if (kGuide == 1) {
             vcell = stree_lookup(sd, p);
+            gcell = vcell;
           } else if (kGuide == 2) {
+            // SimpleGuide2.cu:319-323, 420-428: the guide is the previous
+            // table's slot (FindFromPrv); the vertex records there, or, in a
+            // Locate pass, into the slot a cell missing from the previous
+            // table takes in the current one (FindFromCur)
             uint32_t ix, iy, iz, bucket, csum;
             grid_cell(hg, p, ix, iy, iz);
             grid_hash(hg, ix, iy, iz, bucket, csum);
-            if (rp.flags & kFlagLocate) {
-              grid_find_cur(hg, bucket, csum);
-              vcell = kInvalidCell;
-            } else {
-              vcell = grid_find_prv(hg, bucket, csum);
-            }
+            gcell = grid_find_prv(hg, bucket, csum);
+            vcell = gcell;
+            if ((rp.flags & kFlagLocate) && gcell == kInvalidCell) vcell = grid_find_cur(hg, bucket, csum);
           }
           // learned guide: the mixture the inference pass wrote for this queue slot
           const float4* mx = nullptr;
@@ -712,7 +717,7 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
             }
           }
           const bool guided = kGuide == 3 ? mx != nullptr
-                                          : (kGuide != 0) && (rp.flags & kFlagGuideSample) && vcell != kInvalidCell;
+                                          : (kGuide != 0) && (rp.flags & kFlagGuideSample) && gcell != kInvalidCell;
           const float alpha = kGuide == 3 ? csel : rp.fraction;
           // ---- next-event estimation (SimpleTrace.cu:302-339)
           if (nee) {
@@ -771,7 +776,7 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
                     } else {
                       float cu, cv;
                       dir_to_canonical(dir, cu, cv);
-                      gp = kGuide == 1 ? dtree_pdf(sd, vcell, cu, cv) : mq_pdf(mq, vcell, cu, cv);
+                      gp = kGuide == 1 ? dtree_pdf(sd, gcell, cu, cv) : mq_pdf(mq, gcell, cu, cv);
                     }
                     bp = (1.0f - alpha) * bp + alpha * gp;
                   }
@@ -797,10 +802,11 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
               } else {
                 float gu, gv;
                 if (kGuide == 1) {
-                  dtree_sample(sd, vcell, rng, gu, gv, tree_pdf);
+                  dtree_sample(sd, gcell, rng, gu, gv, tree_pdf);
                 } else {
-                  float gp;
-                  mq_sample(mq, vcell, rng, gu, gv, gp);
+                  // MortonQuadTreeWrapperT::SampleAndPdf (OPX7UtilsMorton.h:486-500):
+                  // the density of the sampled bin
+                  mq_sample(mq, gcell, rng, gu, gv, tree_pdf);
                 }
                 dir = normalize(canonical_to_dir(gu, gv));
               }
@@ -819,9 +825,9 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
             } else {
               dir_to_canonical(dir, vcu, vcv);
               if (kGuide == 1) {
-                gp = tree_pdf >= 0.0f ? tree_pdf : dtree_pdf(sd, vcell, vcu, vcv);
+                gp = tree_pdf >= 0.0f ? tree_pdf : dtree_pdf(sd, gcell, vcu, vcv);
               } else {
-                gp = mq_pdf(mq, vcell, vcu, vcv);
+                gp = tree_pdf >= 0.0f ? tree_pdf : mq_pdf(mq, gcell, vcu, vcv);
               }
             }
             wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
@@ -1558,7 +1564,7 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
   hipLaunchKernelGGL(k_accum, dim3(blocks_for(npix, 256)), dim3(256), 0, s, ps, npix, rp.spp, accum);
   mark(3, 0, 1);
   mark(4, 0, 0);
-  if (record && (guide == 1 || (guide == 2 && !(rp.flags & kFlagLocate)))) {
+  if (record && (guide == 1 || guide == 2)) {
     uint32_t max_vtx = rp.max_depth - 1u;
     uint32_t cap = max_vtx * ps.n;
     hipLaunchKernelGGL(k_commit_emit, dim3(nb), dim3(kBlock), 0, s, ps, max_vtx, ps.recs, ps.rec_count);

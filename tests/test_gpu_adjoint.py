@@ -34,17 +34,24 @@ def test_adjoint_matches_oracle_fp64(ctx, orc):
 
 
 def test_adjoint_matches_central_differences(ctx):
+    # L2 loss against the target of the adjoint workload: the image rendered
+    # with albedo + N(0, 0.1), clipped to [0, 1].  Central differences (h =
+    # 1e-3) of the loss of GPU renders with the same random streams, on 64
+    # random (triangle, channel) parameters, each within 1e-2 relative.  At
+    # this h the float32 image's rounding limits the difference quotient of the
+    # smallest gradients to a few 1e-3 (the FP64 oracle gradient shows the
+    # same spread against its own differences; h = 1e-2 agrees to 2e-3).
     sc = _setup(ctx)
     rng = np.random.default_rng(1)
-    target = rng.uniform(0, 0.3, (H, W, 3)).astype(np.float32)
     alb = sc["albedo"][:, :3].copy()
 
-    def render(a):
+    def render(a, spp=8, seed=7):
         ctx.set_albedo(a)
         ctx.clear()
-        ctx.render(spp=8, flags=FLAGS, max_depth=6, seed=7)
+        ctx.render(spp=spp, flags=FLAGS, max_depth=6, seed=seed)
         return ctx.image().astype(np.float64)
 
+    target = render(np.clip(alb + rng.normal(0.0, 0.1, alb.shape), 0.0, 1.0).astype(np.float32), spp=64, seed=99)
     img = render(alb)
     dldi = (2.0 * (img - target)).astype(np.float32)
     ga, _ = ctx.adjoint(dldi, spp=8, flags=FLAGS, max_depth=6, seed=7)
@@ -56,17 +63,19 @@ def test_adjoint_matches_central_differences(ctx):
     lambert = [t for t in range(nt) if sc["mat"][t] == 0 and sc["emission"][t].sum() == 0]
     picks = [(int(t), int(c)) for t, c in zip(rng.choice(lambert, 64), rng.integers(0, 3, 64))]
     h = 1e-3
-    scale = np.abs(ga).max()
-    bad = []
+    rel = []
     for t, c in picks:
         ap, am = alb.copy(), alb.copy()
         ap[t, c] += h
         am[t, c] -= h
         fd = (loss(ap) - loss(am)) / (2 * h)
-        if abs(fd - ga[t, c]) > 1e-2 * abs(ga[t, c]) + 1e-3 * scale:
-            bad.append((t, c, fd, float(ga[t, c])))
+        # a channel no path scatters from has fd = grad = 0 exactly
+        err = 0.0 if fd == 0.0 and ga[t, c] == 0.0 else abs(fd - ga[t, c]) / abs(fd)
+        rel.append((float(err), t, c, fd, float(ga[t, c])))
     ctx.set_albedo(alb)
-    assert not bad, bad
+    rel.sort(reverse=True)
+    print("adjoint vs central differences, worst relative errors:", rel[:4], flush=True)
+    assert rel[0][0] < 1e-2, rel[:4]
 
 
 def test_torch_autograd_wrapper(ctx):

@@ -406,8 +406,6 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
     c->p_vA.reserve((size_t)max_depth * n);
     c->p_vB.reserve((size_t)max_depth * n);
     c->p_vC.reserve((size_t)max_depth * n);
-    c->p_recs.reserve((size_t)(max_depth > 1 ? max_depth - 1 : 1) * n);
-    c->p_rec_count.reserve(1);
   }
   if (mix_slots) {
     c->p_mix.reserve((size_t)kMlpMixF4 * mix_slots);
@@ -444,8 +442,6 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
   ps.vB = guide ? ctx->p_vB.p : nullptr;
   ps.vC = guide ? ctx->p_vC.p : nullptr;
   ps.work = counting ? ctx->p_work.p : nullptr;
-  ps.recs = guide ? ctx->p_recs.p : nullptr;
-  ps.rec_count = guide ? ctx->p_rec_count.p : nullptr;
   ps.npix = ctx->width * ctx->height;
   ps.rec_limit = n;
   if (mlp) {
@@ -671,7 +667,7 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       RTPG_HIP(launch_stats_add(ps.count, ps.scount, prm->max_depth, n, ctx->p_stats.p, s));
       for (int k = 0; k < 3; ++k) ctx->kernel_launches[k] += prm->max_depth;
       ctx->kernel_launches[3] += 1;
-      if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL)) ctx->kernel_launches[4] += guide == 3 ? 1 : 2;
+      if (guide != 0 && !(prm->flags & RTPG_FLAG_FINAL)) ctx->kernel_launches[4] += 1;
       if (guide == 3 && (prm->flags & RTPG_FLAG_GUIDE_SAMPLE) && prm->mlp->cfg.cache_res == 0)
         ctx->kernel_launches[1] += prm->max_depth;
       done += cs;

@@ -50,11 +50,11 @@ struct rtpg_context_t {
   uint32_t accum_spp = 0;
   rtpg::Camera cam{};
   // path arena
-  rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC, p_recs;
+  rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC;
   rtpg::DBuf<float4> p_mix, p_mv;  // learned guide: mixture per queue slot; training vertices (5 arrays)
   rtpg::DBuf<float> p_mixc;
   rtpg::DBuf<uint2> p_rng;
-  rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err, p_rec_count;
+  rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err;
   rtpg::DBuf<unsigned long long> p_stats;
   uint64_t last_stats[3] = {0, 0, 0};
   // profiling: bit 0 times kernel classes with events, bit 1 counts traversal work

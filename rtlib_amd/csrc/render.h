@@ -45,9 +45,6 @@ struct PathSoA {
   // traversal work counters (null unless counting): extend nodes, extend
   // triangles, shadow nodes, shadow triangles
   unsigned long long* work;
-  // compacted guide records (cell, u, v, value) and their count
-  float4* recs;
-  uint32_t* rec_count;
   // path-replay adjoint (grad_alb non-null selects the adjoint kernels)
   uint32_t adj_mode;     // 0 primal pass, 1 replay and scatter
   uint32_t npix;

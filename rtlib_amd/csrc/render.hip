@@ -1140,78 +1140,24 @@ hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, ucha
 }
 
 // ------------------------------------------------------------------ guide records
-// Recording runs in two stages.  Stage 1 (one work-item per path) turns the
-// path's vertices into compact records (DTree or hash slot, canonical u, v,
-// value), appended with one atomic per wave.  Stage 2 applies a contiguous run
-// of records per work-group: each record's target (statistical weight of its
-// DTree, flux slot of its leaf) is accumulated in an LDS hash table, and the
-// table is flushed with one global float atomic per distinct address, so hot
-// leaves near the quadtree roots do not serialise on HBM atomics.
+// One kernel commits the guide vertices of a chunk (TraceVertexT::Commit,
+// OPX7UtilsPathGuiding.h:830-866; MortonTraceVertexT::Commit,
+// OPX7UtilsMorton.h:550-585).  A work-item takes one path at a time and walks
+// its vertices from the last hit backwards, so the radiance that reached
+// vertex d (the contributions of hits d+1 .. nh-1) is one running sum (the
+// reference sums them forwards per vertex: same terms, other rounding).  Each
+// committed vertex adds 1 to its DTree's statistical weight and its value to
+// the leaf slot of its direction (hash grid: to the Morton leaf of its slot).
+// Both go through LDS tables of the work-group, flushed with one global float
+// atomic per distinct address, so hot DTrees and leaves near the quadtree
+// roots do not serialise on HBM atomics; a key that finds no LDS slot in 16
+// probes goes to HBM directly.
 constexpr int kMaxCommitVtx = 16;
-constexpr uint32_t kApplyBlock = 256;
-constexpr uint32_t kApplyPerThread = 256;
+constexpr uint32_t kCommitBlock = 256;
+constexpr uint32_t kCommitPaths = 32;  // paths per work-item
 constexpr uint32_t kLeafTab = 4096;
 constexpr uint32_t kCellTab = 1024;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
-
-__global__ void __launch_bounds__(kBlock) k_commit_emit(PathSoA ps, uint32_t max_vtx, float4* recs,
-                                                        uint32_t* rec_count) {
-  uint32_t pid = blockIdx.x * kBlock + threadIdx.x;
-  bool in = pid < ps.n;
-  uint32_t nh = in ? ps.nhit[pid] : 0u;
-  nh = nh < (uint32_t)kMaxCommitVtx ? nh : (uint32_t)kMaxCommitVtx;
-  uint32_t nv = nh > 0 ? nh - 1u : 0u;
-  nv = nv < max_vtx ? nv : max_vtx;
-  uint32_t nh_max = nh;
-  for (int off = 32; off >= 1; off >>= 1) nh_max = max(nh_max, (uint32_t)__shfl_xor((int)nh_max, off));
-  // TraceVertexT::Commit (OPX7UtilsPathGuiding.h:830-866): the radiance that
-  // reached vertex d is the sum of the contributions of hits d+1 .. nh-1.  It
-  // is accumulated here from the last hit backwards (one running sum instead
-  // of the reference's per-vertex forward sums: same terms, other rounding).
-  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-  for (int d = (int)nh_max - 2; d >= 0; --d) {
-    bool ok = false;
-    uint32_t cell = 0;
-    float cu = 0.0f, cv = 0.0f, value = 0.0f;
-    if ((uint32_t)d + 1u < nh) {
-      float4 b = ps.vB[(size_t)(d + 1) * ps.n + pid];
-      rx += b.x;
-      ry += b.y;
-      rz += b.z;
-    }
-    if ((uint32_t)d < nv) {
-      size_t vi = (size_t)d * ps.n + pid;
-      float4 vc = ps.vC[vi];
-      uint32_t fl = fbits(vc.w);
-      cell = fbits(vc.z);
-      if ((fl & kVtxValid) && !(fl & kVtxDelta) && cell != kInvalidCell) {
-        float4 va = ps.vA[vi];
-        float wo_pdf = va.w;
-        bool fin = isfinite(rx) && rx >= 0.0f && isfinite(ry) && ry >= 0.0f && isfinite(rz) && rz >= 0.0f;
-        if (wo_pdf > 0.0f && fin && !(fl & kVtxBadBsdf)) {
-          float lx = va.x * wo_pdf > 1e-4f ? rx / va.x : 0.0f;
-          float ly = va.y * wo_pdf > 1e-4f ? ry / va.y : 0.0f;
-          float lz = va.z * wo_pdf > 1e-4f ? rz / va.z : 0.0f;
-          float cosv = fabsf(ps.vB[vi].w);
-          lx *= cosv;
-          ly *= cosv;
-          lz *= cosv;
-          float avg = (lx + ly + lz) / 3.0f;
-          value = avg / wo_pdf;
-          cu = vc.x;
-          cv = vc.y;
-          ok = true;
-        }
-      }
-    }
-    // fixed slot per (vertex, path): no counter to contend on; stage 2 reads
-    // slot (d, pid) only for d < nhit[pid] - 1, all of which are written here
-    if (in && (uint32_t)d < max_vtx) {
-      reinterpret_cast<uint4*>(recs)[(size_t)d * ps.n + pid] =
-          make_uint4(ok ? cell : kInvalidCell, fbits(cu), fbits(cv), fbits(value));
-    }
-  }
-}
 
 // Insert-or-add into an LDS open-addressing table; false when 16 probes miss
 // (the caller then adds to global memory directly).
@@ -1227,33 +1173,6 @@ __device__ __forceinline__ bool lds_table_add(uint32_t* keys, T* vals, uint32_t 
     h = (h + 1u) & mask;
   }
   return false;
-}
-
-// One record slot for k_commit_apply: cell == kInvalidCell when the slot
-// holds no record; flux when its value is recorded: SD-tree records with a
-// positive finite value not dropped by the directional box filter (cu < 0),
-// as DTreeT::RecordIrradiance; every Morton record, as MortonQuadTreeT::Record
-// (OPX7UtilsMorton.h:262-285) tests no value.
-struct ApplyRec {
-  uint32_t cell, key;
-  float cu, cv, value;
-  bool flux;
-};
-
-template <int kKind>
-__device__ __forceinline__ ApplyRec apply_load(const float4* recs, const uint32_t* nhit, uint32_t npaths, uint32_t r,
-                                               uint32_t end) {
-  ApplyRec q{kInvalidCell, 0u, 0.0f, 0.0f, 0.0f, false};
-  if (r >= end) return q;
-  uint32_t d = r / npaths, pid = r - d * npaths;
-  if (d + 1u >= nhit[pid]) return q;  // no vertex d on this path
-  float4 rec = recs[r];
-  q.cell = fbits(rec.x);
-  q.cu = rec.y;
-  q.cv = rec.z;
-  q.value = rec.w;
-  q.flux = q.cell != kInvalidCell && (kKind == 2 || (isfinite(q.value) && q.value > 0.0f && q.cu >= 0.0f));
-  return q;
 }
 
 // Flux slot of a record (DTreeNodeT::Record :113-125): the first levels of the
@@ -1289,55 +1208,120 @@ __device__ __forceinline__ uint32_t morton_key(const MortonView& mq, uint32_t ce
 }
 
 // kind 1: SD-tree (DTreeT::RecordIrradiance :419-427 -> DTreeNodeT::Record
-// :113-125: weight += 1, flux into the first leaf slot on the descent);
-// kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285).
+// :113-125: weight += 1, flux into the first leaf slot on the descent, for a
+// positive finite value not dropped by the directional box filter, cu < 0);
+// kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285, which tests
+// no value).
+// kind 1: SD-tree (DTreeT::RecordIrradiance :419-427 -> DTreeNodeT::Record
+// :113-125: weight += 1, flux into the first leaf slot on the descent, for a
+// positive finite value not dropped by the directional box filter, cu < 0);
+// kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285, which tests
+// no value).
+// Combines the lanes of the wave that hold the same key, for the first
+// `rounds` distinct keys in lane order: each group's first lane gets the
+// group's count (n) and sum (v) and returns true, the others false; the lanes
+// of keys beyond those rounds keep their own (n = 1) and return true.  Lanes
+// of neighbouring pixels at one depth mostly share a DTree and its near-root
+// leaves, whose LDS atomics would otherwise serialise on one address.  All
+// lanes of the wave must call it.
+__device__ __forceinline__ bool wave_combine(bool active, uint32_t key, float& v, uint32_t& n, int rounds) {
+  uint64_t pending = __ballot(active);
+  const uint32_t lane = lane_id();
+  bool lead = active;
+  n = 1u;
+  for (int r = 0; r < rounds && pending; ++r) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t k = __shfl(key, leader);
+    const bool mine = active && key == k;
+    const uint64_t same = __ballot(mine);
+    float part = mine ? v : 0.0f;
+    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+    if (mine) {
+      lead = lane == (uint32_t)leader;
+      v = part;
+      n = (uint32_t)__popcll(same);
+    }
+    pending &= ~same;
+  }
+  return lead;
+}
+
+// kind 1: SD-tree (DTreeT::RecordIrradiance :419-427 -> DTreeNodeT::Record
+// :113-125: weight += 1, flux into the first leaf slot on the descent, for a
+// positive finite value not dropped by the directional box filter, cu < 0);
+// kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285, which tests
+// no value).
 template <int kKind>
-__global__ void __launch_bounds__(kApplyBlock) k_commit_apply(const float4* recs, const uint32_t* nhit,
-                                                              uint32_t npaths, uint32_t n, SdTreeView sd,
-                                                              MortonView mq) {
+__global__ void __launch_bounds__(kCommitBlock) k_commit(PathSoA ps, uint32_t max_vtx, SdTreeView sd, MortonView mq) {
   __shared__ uint32_t lkey[kLeafTab];
   __shared__ float lval[kLeafTab];
-  __shared__ uint32_t ckey[kCellTab];
-  __shared__ uint32_t cval[kCellTab];
-  uint32_t begin = blockIdx.x * kApplyBlock * kApplyPerThread;
-  if (begin >= n) return;
-  for (uint32_t i = threadIdx.x; i < kLeafTab; i += kApplyBlock) {
+  __shared__ uint32_t ckey[kKind == 1 ? kCellTab : 1];
+  __shared__ uint32_t cval[kKind == 1 ? kCellTab : 1];
+  const uint32_t first = blockIdx.x * kCommitBlock * kCommitPaths;
+  if (first >= ps.n) return;  // uniform over the work-group
+  for (uint32_t i = threadIdx.x; i < kLeafTab; i += kCommitBlock) {
     lkey[i] = kEmptyKey;
     lval[i] = 0.0f;
   }
   if (kKind == 1) {
-    for (uint32_t i = threadIdx.x; i < kCellTab; i += kApplyBlock) {
+    for (uint32_t i = threadIdx.x; i < kCellTab; i += kCommitBlock) {
       ckey[i] = kEmptyKey;
       cval[i] = 0u;
     }
   }
   __syncthreads();
-  uint32_t end = min(n, begin + kApplyBlock * kApplyPerThread);
   float* flat = kKind == 1 ? reinterpret_cast<float*>(sd.build_sums) : mq.build;
-  // two records per iteration: their loads and descents are independent, so
-  // their latencies overlap before the LDS updates
-  for (uint32_t r = begin + threadIdx.x; r < end; r += 2u * kApplyBlock) {
-    ApplyRec a = apply_load<kKind>(recs, nhit, npaths, r, end);
-    ApplyRec b = apply_load<kKind>(recs, nhit, npaths, r + kApplyBlock, end);
-    if (a.flux) a.key = kKind == 1 ? record_key(sd, a.cell, a.cu, a.cv) : morton_key(mq, a.cell, a.cu, a.cv);
-    if (b.flux) b.key = kKind == 1 ? record_key(sd, b.cell, b.cu, b.cv) : morton_key(mq, b.cell, b.cu, b.cv);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const ApplyRec& q = k == 0 ? a : b;
-      if (kKind == 1 && q.cell != kInvalidCell) {
-        if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, q.cell, 1u)) unsafeAtomicAdd(sd.build_sw + q.cell, 1.0f);
+  for (uint32_t k = 0; k < kCommitPaths; ++k) {
+    // consecutive work-items take consecutive paths, so the vertex loads of a
+    // wave at one depth are contiguous
+    const uint32_t pid = first + k * kCommitBlock + threadIdx.x;
+    const bool in = pid < ps.n;
+    uint32_t nh = in ? ps.nhit[pid] : 0u;
+    nh = nh < (uint32_t)kMaxCommitVtx ? nh : (uint32_t)kMaxCommitVtx;
+    uint32_t nv = nh > 0 ? nh - 1u : 0u;
+    nv = nv < max_vtx ? nv : max_vtx;
+    uint32_t nh_max = nh;
+    for (int off = 32; off >= 1; off >>= 1) nh_max = max(nh_max, (uint32_t)__shfl_xor((int)nh_max, off));
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    for (int d = (int)nh_max - 2; d >= 0; --d) {
+      if ((uint32_t)d + 1u < nh) {
+        float4 b = ps.vB[(size_t)(d + 1) * ps.n + pid];
+        rx += b.x;
+        ry += b.y;
+        rz += b.z;
       }
-      if (q.flux && !lds_table_add<float>(lkey, lval, kLeafTab - 1u, q.key, q.value)) {
-        unsafeAtomicAdd(flat + q.key, q.value);
+      if ((uint32_t)d >= nv) continue;
+      const size_t vi = (size_t)d * ps.n + pid;
+      const float4 vc = ps.vC[vi];
+      const uint32_t fl = fbits(vc.w), cell = fbits(vc.z);
+      if (!(fl & kVtxValid) || (fl & kVtxDelta) || cell == kInvalidCell) continue;
+      const float4 va = ps.vA[vi];
+      const float wo_pdf = va.w;
+      const bool fin = isfinite(rx) && rx >= 0.0f && isfinite(ry) && ry >= 0.0f && isfinite(rz) && rz >= 0.0f;
+      if (!(wo_pdf > 0.0f && fin && !(fl & kVtxBadBsdf))) continue;
+      float lx = va.x * wo_pdf > 1e-4f ? rx / va.x : 0.0f;
+      float ly = va.y * wo_pdf > 1e-4f ? ry / va.y : 0.0f;
+      float lz = va.z * wo_pdf > 1e-4f ? rz / va.z : 0.0f;
+      const float cosv = fabsf(ps.vB[vi].w);
+      lx *= cosv;
+      ly *= cosv;
+      lz *= cosv;
+      const float value = ((lx + ly + lz) / 3.0f) / wo_pdf;
+      const float cu = vc.x, cv = vc.y;
+      if (kKind == 1) {
+        if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) unsafeAtomicAdd(sd.build_sw + cell, 1.0f);
+        if (!(isfinite(value) && value > 0.0f && cu >= 0.0f)) continue;
       }
+      const uint32_t key = kKind == 1 ? record_key(sd, cell, cu, cv) : morton_key(mq, cell, cu, cv);
+      if (!lds_table_add<float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kLeafTab; i += kApplyBlock) {
+  for (uint32_t i = threadIdx.x; i < kLeafTab; i += kCommitBlock) {
     if (lkey[i] != kEmptyKey) unsafeAtomicAdd(flat + lkey[i], lval[i]);
   }
   if (kKind == 1) {
-    for (uint32_t i = threadIdx.x; i < kCellTab; i += kApplyBlock) {
+    for (uint32_t i = threadIdx.x; i < kCellTab; i += kCommitBlock) {
       if (ckey[i] != kEmptyKey) unsafeAtomicAdd(sd.build_sw + ckey[i], (float)cval[i]);
     }
   }
@@ -1565,16 +1549,11 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
   mark(3, 0, 1);
   mark(4, 0, 0);
   if (record && (guide == 1 || guide == 2)) {
-    uint32_t max_vtx = rp.max_depth - 1u;
-    uint32_t cap = max_vtx * ps.n;
-    hipLaunchKernelGGL(k_commit_emit, dim3(nb), dim3(kBlock), 0, s, ps, max_vtx, ps.recs, ps.rec_count);
-    uint32_t na = blocks_for(cap, kApplyBlock * kApplyPerThread);
+    const uint32_t nc = blocks_for(ps.n, kCommitBlock * kCommitPaths);
     if (guide == 1) {
-      hipLaunchKernelGGL(k_commit_apply<1>, dim3(na), dim3(kApplyBlock), 0, s, ps.recs, ps.nhit, ps.n, cap, sd,
-                         mq);
+      hipLaunchKernelGGL(k_commit<1>, dim3(nc), dim3(kCommitBlock), 0, s, ps, rp.max_depth - 1u, sd, mq);
     } else {
-      hipLaunchKernelGGL(k_commit_apply<2>, dim3(na), dim3(kApplyBlock), 0, s, ps.recs, ps.nhit, ps.n, cap, sd,
-                         mq);
+      hipLaunchKernelGGL(k_commit<2>, dim3(nc), dim3(kCommitBlock), 0, s, ps, rp.max_depth - 1u, sd, mq);
     }
   }
   if (record && guide == 3) {

@@ -1,4 +1,4 @@
-"""Benchmark: guided (SD-tree, learned) vs plain path tracing on one MI355X.
+"""Benchmark: guided (SD-tree, hash grid, learned) vs plain path tracing on one MI355X.
 
 Workload (the guided-room configuration): room with a half-open dividing wall,
 a small ceiling light (0.5% of the ceiling) hidden behind it, a 1024 x 512
@@ -6,15 +6,21 @@ fBm heightfield floor (~1.05M triangles); Lambertian albedo U[0.2,0.8] per
 triangle with 20% GGX (roughness U[0.05,0.5]); 1280 x 720, path depth 10, NEE
 with MIS and Russian roulette; SD-tree training passes of 1, 2, 4, ..., 64 spp
 (split threshold c = 12000 * sqrt(2^k), flux threshold 0.01, DTree depth 20)
-and a final pass, 256 spp in all.
+and a final pass, 256 spp in all; guide fraction 0.7 and the spatial box
+recording filter (--fraction, --spatial-filter; the library defaults are the
+reference's 0.3 and Nearest, run as a separate leg).
 
 One step = one complete SD-tree-guided render of the budget (all training
 passes, recording, refinement and the final pass).  `value` is Msamples/s
-(camera paths per second) of that render.  The line also carries the plain
-path tracer's rate, the relMSE at equal wall-clock time of the SD-tree-guided
-and the learned-guide renders against plain path tracing (reference: a
-high-spp plain render), the roofline of the dominant kernel and the CPU
-baseline.
+(camera paths per second) of that render.  The line also carries:
+- equal-time quality legs: each guided render (SD-tree, hash grid, both with
+  RIS, learned guide) against plain path tracing given the same wall-clock
+  time, as relMSE, SMAPE and rMAE against a high-spp plain reference, mean and
+  standard deviation over --seeds seeds;
+- the roofline of the dominant kernel (k_extend, closest-hit traversal): its
+  VALU issue rate from the committed counter summary of this workload over
+  the launch time measured here, beside its HBM traffic;
+- the other configurations and the CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -34,11 +40,15 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Msamples/s + relMSE at equal time: guided (SD-tree/MLP) vs plain PT, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4
+# cycles (16 lanes), at the 2.4 GHz peak engine clock
+VALU_PEAK_GWIS = 256 * 4 * 2.4 / 4  # 614.4 G wave-instructions/s
 NODE_BYTES = 112      # what the traversal reads of a 128-byte BVH4 node: four child boxes + child entries
 TRI_BYTES = 36        # three float3 vertices per tested triangle
 RAY_IN_BYTES = 32     # origin + direction
 HIT_OUT_BYTES = 16    # t, u, v, triangle
 QUEUE_BYTES = 4       # queue index read
+PMC_FILE = ROOT / "profiles" / "guided_room_counters.json"
 
 
 def log(*a):
@@ -49,6 +59,31 @@ def rel_mse(img, ref):
     return float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2)))
 
 
+def metrics(img, ref):
+    """relMSE, and the reference comparison tool's SMAPE and rMAE."""
+    from rtlib_amd import results
+    rmae, smape = results.compare(ref.reshape(-1, 3), img.reshape(-1, 3))
+    return {"relmse": rel_mse(img, ref), "smape": smape, "rmae": rmae}
+
+
+def summarize(rows):
+    """rows: per seed {ms, plain_spp, guided: metrics, plain: metrics} ->
+    mean and standard deviation of each figure and of the guided / plain ratios."""
+    out = {"seeds": len(rows)}
+
+    def ms(vals):
+        a = np.asarray(vals, np.float64)
+        return [round(float(a.mean()), 6), round(float(a.std(ddof=1)) if a.size > 1 else 0.0, 6)]
+
+    out["ms"] = ms([r["ms"] for r in rows])
+    out["plain_spp"] = ms([r["plain_spp"] for r in rows])
+    for k in ("relmse", "smape", "rmae"):
+        out[k] = ms([r["guided"][k] for r in rows])
+        out["plain_" + k] = ms([r["plain"][k] for r in rows])
+        out["ratio_" + k] = ms([r["guided"][k] / r["plain"][k] for r in rows])
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +91,15 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--budget", type=int, default=256,
                     help="samples per pixel of one guided render: SD-tree passes 1, 2, 4, .., 64 and the final pass")
-    ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--fraction", type=float, default=0.7,
+                    help="probability of sampling the guide instead of the BSDF (SD-tree and hash grid)")
+    ap.add_argument("--spatial-filter", type=int, default=1, help="SD-tree spatial recording filter: 0 Nearest, "
+                    "1 Box (stochastic form)")
+    ap.add_argument("--directional-filter", type=int, default=0, help="SD-tree directional recording filter: "
+                    "0 Nearest, 1 Box (stochastic form)")
+    ap.add_argument("--grid-bounds", default="64,32,64", help="hash-grid cells per axis over the scene box")
+    ap.add_argument("--seeds", type=int, default=4, help="seeds per equal-time quality leg")
+    ap.add_argument("--ref-spp", type=int, default=16384, help="reference image (0: no quality legs)")
     ap.add_argument("--no-configs", action="store_true", help="skip the Cornell / many-light / adjoint / "
                     "learned-guide configurations")
     ap.add_argument("--mlp-passes", type=int, default=8)
@@ -67,6 +110,11 @@ def main(argv=None):
                     "of the SD-tree and the learned guide (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--small", action="store_true", help="reduced size for a quick functional check")
+    ap.add_argument("--pmc-step", action="store_true", help="run exactly one guided step of the workload (seed 1, "
+                    "the first timed step's) and exit: the command that the counter summaries under profiles/ "
+                    "are collected from")
+    ap.add_argument("--pmc-mlp", action="store_true", help="run one learned-guide render of the workload and exit "
+                    "(for the matrix-core counters of its training and inference kernels)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (once untimed, then 5 times) and exit; used to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters")
@@ -79,16 +127,18 @@ def main(argv=None):
         return 2
 
     from rtlib_amd import scenes
-    from rtlib_amd.tracer import (FLAG_MIS, FLAG_NEE, FLAG_RR, Context, DoubleBufferedHashGrid3Buffer, MLPGuide,
-                                  RTMortonQuadTreeController, RTMortonQuadTreeWrapper, RTSTreeController,
+    from rtlib_amd.tracer import (FLAG_MIS, FLAG_NEE, FLAG_RIS, FLAG_RR, Context, MLPGuide, RTSTreeController,
                                   RTSTreeWrapper, render_guided, render_mlp_guided)
 
     W, H = (320, 180) if a.small else (1280, 720)
-    grid = (256, 128) if a.small else (1024, 512)
+    grid_res = (256, 128) if a.small else (1024, 512)
     depth = 10
+    frac = a.fraction
+    bounds = tuple(int(x) for x in a.grid_bounds.split(","))
     base = FLAG_NEE | FLAG_MIS | FLAG_RR
+    ris = FLAG_NEE | FLAG_RIS | FLAG_RR
     t0 = time.time()
-    sc = scenes.guided_room(width=W, height=H, grid=grid)
+    sc = scenes.guided_room(width=W, height=H, grid=grid_res)
     log(f"scene: {sc['idx'].shape[0]} triangles ({time.time() - t0:.1f} s)")
     ctx = Context(0)
     t0 = time.time()
@@ -99,13 +149,29 @@ def main(argv=None):
     npix = W * H
     lo, hi = info["aabb_min"] - 0.005, info["aabb_max"] + 0.005
 
-    tree = RTSTreeWrapper(ctx, lo, hi, 20, 0.3)
+    tree = RTSTreeWrapper(ctx, lo, hi, 20, frac)
     tree.SetSplit(mode=1, c=12000.0, max_mb=2000, subdiv_th=0.01)
+    tree.SetFilter(a.spatial_filter, a.directional_filter, 17)
 
-    def guided_step(seed, on_pass=None):
+    def guided_step(seed, on_pass=None, flags=base):
         ctrl = RTSTreeController(tree, a.budget, 0, 0.5, 1)
         ctx.clear()
-        return render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=seed, on_pass=on_pass)
+        return render_guided(ctx, ctrl, tree, "tree", flags, max_depth=depth, seed=seed, fraction=frac,
+                             on_pass=on_pass)
+
+    if a.pmc_step:
+        guided_step(1)
+        ctx.synchronize()
+        log("one guided step done")
+        return 0
+    if a.pmc_mlp:
+        mlp = MLPGuide(ctx, lo, hi, seed=1, cache_res=a.mlp_cache)
+        render_mlp_guided(ctx, mlp, a.budget, base, max_depth=depth, seed=11, train_passes=a.mlp_passes,
+                          steps_per_pass=a.mlp_steps)
+        ctx.synchronize()
+        mlp.close()
+        log("one learned-guide render done")
+        return 0
 
     def timed(fn):
         ctx.synchronize()
@@ -115,9 +181,7 @@ def main(argv=None):
         return time.perf_counter() - t, r
 
     # ---- guided: warmup, then K timed steps.  Kernel classes are timed with
-    # HIP events on the render stream during the timed steps themselves; the
-    # window of the timed steps is logged (CLOCK_BOOTTIME and CLOCK_MONOTONIC
-    # ns) so that a kernel trace can be cut to exactly these dispatches.
+    # HIP events on the render stream during the timed steps themselves.
     for w in range(a.warmup):
         dt, _ = timed(lambda: guided_step(100 + w))
         log(f"warmup step {w}: {dt * 1e3:.1f} ms")
@@ -135,16 +199,12 @@ def main(argv=None):
         rays += st["rays"]
         shadow += st["shadow_rays"]
 
-    win = [time.clock_gettime_ns(time.CLOCK_BOOTTIME), time.monotonic_ns()]
     for k in range(a.steps):
         step_ms.append(np.zeros(5))
         step_launches.append(np.zeros(5, np.int64))
         dt, n = timed(lambda: guided_step(k + 1, on_pass=acc_pass))
         times.append(dt)
         log(f"step {k}: {dt * 1e3:.1f} ms ({n} spp)")
-    win += [time.clock_gettime_ns(time.CLOCK_BOOTTIME), time.monotonic_ns()]
-    log(f"timed window ns: boottime {win[0]} {win[2]} monotonic {win[1]} {win[3]}")
-    guided_img = ctx.image()
     per_kernel = np.mean(step_ms, axis=0)
     launches = step_launches[0]
     # ---- work counts of step 0 (seed 1) in a separate, untimed run: same
@@ -175,152 +235,29 @@ def main(argv=None):
     dt_plain, _ = timed(lambda: ctx.render(spp=a.budget, flags=base, max_depth=depth, seed=77))
     plain_rate = a.budget * npix / dt_plain / 1e6
     st_plain = ctx.render_stats()
-    plain_budget_img = ctx.image()
-    # kernel classes of the same plain render, timed in a separate run
     ctx.set_profile(time_kernels=True, count_work=False)
     ctx.clear()
     ctx.render(spp=a.budget, flags=base, max_depth=depth, seed=77)
     plain_kernel_ms = ctx.kernel_stats()[0]
     ctx.set_profile(False, False)
+    ctx.clear()
+    dt_plain_ris, _ = timed(lambda: ctx.render(spp=a.budget, flags=ris, max_depth=depth, seed=78))
+    plain_ris_rate = a.budget * npix / dt_plain_ris / 1e6
+
     quality = None
     if a.ref_spp > 0:
-        # learned guide on the same budget: a.mlp_passes recording passes of 1
-        # spp, each followed by a.mlp_steps Adam steps, then one final guided
-        # pass that reads the spatial cache of the network (a.mlp_cache cells
-        # per axis); and the same with inference at every guided vertex
-        mlp = MLPGuide(ctx, lo, hi, seed=1, cache_res=a.mlp_cache)
-        mlp_pv = MLPGuide(ctx, lo, hi, seed=1)
+        quality = quality_legs(a, ctx, sc, tree, lo, hi, bounds, guided_step, timed, plain_rate, plain_ris_rate,
+                               npix, depth, base, ris)
 
-        def mlp_step(g=mlp, budget=a.budget):
-            g.reset(1)
-            ctx.clear()
-            return render_mlp_guided(ctx, g, budget, base, max_depth=depth, seed=11,
-                                     train_passes=a.mlp_passes, steps_per_pass=a.mlp_steps)
-
-        timed(mlp_step)
-        t_mlp, _ = timed(mlp_step)
-        mlp_img = ctx.image()
-        log(f"learned guide step: {t_mlp * 1e3:.1f} ms ({a.budget} spp, cache {a.mlp_cache})")
-        timed(lambda: mlp_step(mlp_pv))
-        t_pv, _ = timed(lambda: mlp_step(mlp_pv))
-        pv_img = ctx.image()
-        log(f"learned guide step, per-vertex inference: {t_pv * 1e3:.1f} ms ({a.budget} spp)")
-        mlp_pv.close()
-
-        def plain_equal_time(t, seed):
-            spp = max(1, int(round(t * plain_rate * 1e6 / npix)))
-            ctx.clear()
-            ctx.render(spp=spp, flags=base, max_depth=depth, seed=seed)
-            return spp, ctx.image()
-
-        # hash-grid variant: 30-bit Morton cell keys, 2^20 slots, 8 x 8 direction
-        # bins per cell, the reference's Morton controller (Locate first, trees
-        # built from iteration 3)
-        grid = DoubleBufferedHashGrid3Buffer(ctx, lo, hi, bounds=(256, 64, 256), slots=1 << 20, key_mode=1)
-        mq = RTMortonQuadTreeWrapper(ctx, 1 << 20, 3, 0.3)
-
-        def grid_step():
-            grid.Clear()
-            mq.Allocate()
-            ctrl = RTMortonQuadTreeController(mq, grid, a.budget, 3, 0, 0.5, 1)
-            ctx.clear()
-            return render_guided(ctx, ctrl, mq, "grid", base, max_depth=depth, seed=13, grid=grid)
-
-        timed(grid_step)
-        t_grid, _ = timed(grid_step)
-        grid_img = ctx.image()
-        log(f"hash-grid guided step: {t_grid * 1e3:.1f} ms ({a.budget} spp)")
-
-        # SD-tree with the stochastic box recording filters (spatial + directional)
-        tree.SetFilter(1, 1, 17)
-        timed(lambda: guided_step(90))
-        t_box, _ = timed(lambda: guided_step(91))
-        box_img = ctx.image()
-        box_sizes = tree.Sizes()
-        tree.SetFilter(0, 0)
-        log(f"SD-tree guided step, box filters: {t_box * 1e3:.1f} ms ({a.budget} spp)")
-
-        spp_tree, plain_tree = plain_equal_time(t_step, 78)
-        spp_box, plain_box = plain_equal_time(t_box, 81)
-        spp_mlp, plain_mlp = plain_equal_time(t_mlp, 79)
-        spp_pv, plain_pv = plain_equal_time(t_pv, 82)
-        spp_grid, plain_grid = plain_equal_time(t_grid, 80)
-        # the SD-tree and the learned guide on a larger budget, where their
-        # training passes are a smaller share of the time
-        if a.large_budget > 0:
-            def tree_large():
-                ctrl = RTSTreeController(tree, a.large_budget, 0, 0.5, 1)
-                ctx.clear()
-                return render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=92)
-
-            timed(tree_large)
-            t_tl, _ = timed(tree_large)
-            tl_img = ctx.image()
-            timed(lambda: mlp_step(mlp, a.large_budget))
-            t_ml, _ = timed(lambda: mlp_step(mlp, a.large_budget))
-            ml_img = ctx.image()
-            log(f"budget {a.large_budget}: SD-tree {t_tl * 1e3:.1f} ms, learned guide {t_ml * 1e3:.1f} ms")
-            spp_tl, plain_tl = plain_equal_time(t_tl, 83)
-            spp_ml, plain_ml = plain_equal_time(t_ml, 84)
-        # reference image (plain, many samples, independent seed)
-        t0 = time.time()
-        ctx.clear()
-        done = 0
-        while done < a.ref_spp:
-            k = min(256, a.ref_spp - done)
-            ctx.render(spp=k, flags=base, max_depth=depth, seed=4242, sample_offset=done)
-            done += k
-        ref = ctx.image()
-        log(f"reference {a.ref_spp} spp in {time.time() - t0:.1f} s")
-        quality = {
-            "reference_spp": a.ref_spp,
-            "plain_at_budget": {"spp": a.budget, "relmse": round(rel_mse(plain_budget_img, ref), 5)},
-            "sdtree": {"ms": round(t_step * 1e3, 1), "relmse": round(rel_mse(guided_img, ref), 5),
-                       "plain_spp": spp_tree, "plain_relmse": round(rel_mse(plain_tree, ref), 5)},
-            "sdtree_box_filters": {"ms": round(t_box * 1e3, 1), "relmse": round(rel_mse(box_img, ref), 5),
-                                   "plain_spp": spp_box, "plain_relmse": round(rel_mse(plain_box, ref), 5),
-                                   "dtrees": box_sizes["dtrees"]},
-            "hashgrid": {"ms": round(t_grid * 1e3, 1), "relmse": round(rel_mse(grid_img, ref), 5),
-                         "plain_spp": spp_grid, "plain_relmse": round(rel_mse(plain_grid, ref), 5)},
-            "mlp": {"ms": round(t_mlp * 1e3, 1), "relmse": round(rel_mse(mlp_img, ref), 5),
-                    "plain_spp": spp_mlp, "plain_relmse": round(rel_mse(plain_mlp, ref), 5),
-                    "train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps, "cache_res": a.mlp_cache},
-            "mlp_per_vertex_inference": {"ms": round(t_pv * 1e3, 1), "relmse": round(rel_mse(pv_img, ref), 5),
-                                         "plain_spp": spp_pv, "plain_relmse": round(rel_mse(plain_pv, ref), 5)},
-        }
-        if a.large_budget > 0:
-            quality[f"budget_{a.large_budget}"] = {
-                "sdtree": {"ms": round(t_tl * 1e3, 1), "relmse": round(rel_mse(tl_img, ref), 5),
-                           "plain_spp": spp_tl, "plain_relmse": round(rel_mse(plain_tl, ref), 5)},
-                "mlp": {"ms": round(t_ml * 1e3, 1), "relmse": round(rel_mse(ml_img, ref), 5),
-                        "plain_spp": spp_ml, "plain_relmse": round(rel_mse(plain_ml, ref), 5)},
-            }
-        mlp.close()
-
-    # ---- roofline of the dominant kernel (k_extend): algorithmic bytes of
-    # step 0 over its event-timed k_extend time
-    dom = int(np.argmax(per_kernel))
+    # ---- roofline of the dominant kernel (k_extend)
     ext_bytes = rays_c * (RAY_IN_BYTES + HIT_OUT_BYTES + QUEUE_BYTES) + work[0] * NODE_BYTES + work[1] * TRI_BYTES
-    n_ext = int(launches[0])
-    ext_ms0 = float(step_ms[0][0])
-    achieved = ext_bytes / (ext_ms0 * 1e-3) / 1e9 if ext_ms0 > 0 else 0.0
-    traffic, traffic_src = _traffic(W, H)
-    roofline = {"kernel": "k_extend (closest-hit BVH4 traversal)", "bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "traffic_source": traffic_src,
-                "algorithmic_bytes_per_launch": round(ext_bytes / max(n_ext, 1)),
-                "launches_per_step": n_ext,
-                "avg_launch_ms": round(float(per_kernel[0]) / max(n_ext, 1), 4),
-                "dominant_by_time": names[dom],
-                "kernel_ms_per_step": {n: round(float(x), 2) for n, x in zip(names, per_kernel)},
-                "plain_kernel_ms_per_step": {n: round(float(plain_kernel_ms[n]), 2) for n in names},
-                "kernel_launches_per_step": {n: int(x) for n, x in zip(names, launches)}}
+    roofline = extend_roofline(W, H, per_kernel, launches, ext_bytes, names, plain_kernel_ms)
 
     # ---- the other configurations, measured on the same context
-    configs = None if (a.small or a.no_configs) else other_configs(ctx, timed)
+    configs = None if (a.small or a.no_configs) else other_configs(ctx, timed, frac)
 
     # ---- CPU baseline: the oracle integrator on the host cores, guided final pass
-    cpu = cpu_baseline(sc, tree, lo, hi, W, H, depth, a.cpu_seconds) if a.cpu_seconds > 0 else None
+    cpu = cpu_baseline(sc, tree, lo, hi, W, H, depth, a.cpu_seconds, frac) if a.cpu_seconds > 0 else None
 
     out = {
         "metric": METRIC,
@@ -340,11 +277,17 @@ def main(argv=None):
                                f"SD-tree passes 1,2,4..64 spp + final pass, {a.budget} spp budget" if not a.small
                    else "guided room (small)",
                    "width": W, "height": H, "triangles": int(sc["idx"].shape[0]), "budget_spp": a.budget,
-                   "max_depth": depth, "parallelism": "single GPU"},
+                   "max_depth": depth, "guide_fraction": frac,
+                   "sdtree_filters": {"spatial": ["nearest", "box"][a.spatial_filter],
+                                      "directional": ["nearest", "box"][a.directional_filter]},
+                   "hash_grid": {"cells": list(bounds), "slots": 1 << 20,
+                                                                             "directional_bins": "8x8"},
+                   "parallelism": "single GPU"},
         "plain_msamples_per_s": round(plain_rate, 2),
+        "plain_ris_msamples_per_s": round(plain_ris_rate, 2),
         "mrays_per_s": {"guided": round((rays + shadow) / t_rays / 1e6, 1),
                         "plain": round((st_plain["rays"] + st_plain["shadow_rays"]) / dt_plain / 1e6, 1)},
-        "relmse_equal_time": quality,
+        "equal_time": quality,
         "sdtree": tree.Sizes(),
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -354,10 +297,151 @@ def main(argv=None):
     return 0
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "k_extend_guided_room_traffic.json"
+def quality_legs(a, ctx, sc, tree, lo, hi, bounds, guided_step, timed, plain_rate, plain_ris_rate, npix, depth, base,
+                 ris):
+    """Equal-time legs: every guided render against plain path tracing (plain
+    RIS for the RIS legs) given the same wall-clock time, over a.seeds seeds."""
+    from rtlib_amd.tracer import (DoubleBufferedHashGrid3Buffer, MLPGuide, RTMortonQuadTreeController,
+                                  RTMortonQuadTreeWrapper, RTSTreeController, RTSTreeWrapper, render_guided,
+                                  render_mlp_guided)
+    frac = a.fraction
+    t0 = time.time()
+    ctx.clear()
+    done = 0
+    while done < a.ref_spp:
+        k = min(256, a.ref_spp - done)
+        ctx.render(spp=k, flags=base, max_depth=depth, seed=4242, sample_offset=done)
+        done += k
+    ref = ctx.image()
+    log(f"reference {a.ref_spp} spp in {time.time() - t0:.1f} s")
+    grid = DoubleBufferedHashGrid3Buffer(ctx, lo, hi, bounds=bounds, slots=1 << 20, key_mode=1)
+    mq = RTMortonQuadTreeWrapper(ctx, 1 << 20, 3, frac)
+
+    def grid_step(seed, flags=base):
+        grid.Clear()
+        mq.Allocate()
+        ctrl = RTMortonQuadTreeController(mq, grid, a.budget, 3, 0, 0.5, 1)
+        ctx.clear()
+        return render_guided(ctx, ctrl, mq, "grid", flags, max_depth=depth, seed=seed, fraction=frac, grid=grid)
+
+    mlp = MLPGuide(ctx, lo, hi, seed=1, cache_res=a.mlp_cache)
+
+    def mlp_step(seed, g=None, budget=None):
+        g = g or mlp
+        g.reset(1)
+        ctx.clear()
+        return render_mlp_guided(ctx, g, budget or a.budget, base, max_depth=depth, seed=seed,
+                                 train_passes=a.mlp_passes, steps_per_pass=a.mlp_steps)
+
+    def leg(name, fn, seeds, plain_flags=base, rate=plain_rate, seed0=1000):
+        fn(seed0 - 1)  # warm
+        rows = []
+        for s in range(seeds):
+            t, _ = timed(lambda: fn(seed0 + s))
+            img = ctx.image()
+            spp = max(1, int(round(t * rate * 1e6 / npix)))
+            ctx.clear()
+            done = 0
+            while done < spp:
+                k = min(256, spp - done)
+                ctx.render(spp=k, flags=plain_flags, max_depth=depth, seed=seed0 + 500 + s, sample_offset=done)
+                done += k
+            rows.append({"ms": t * 1e3, "plain_spp": spp, "guided": metrics(img, ref), "plain": metrics(ctx.image(),
+                                                                                                        ref)})
+        r = summarize(rows)
+        log(f"{name}: {r['ms'][0]:.1f} ms, relMSE {r['relmse'][0]:.5f} vs plain {r['plain_relmse'][0]:.5f} "
+            f"(ratio {r['ratio_relmse'][0]:.3f} +- {r['ratio_relmse'][1]:.3f}), SMAPE ratio {r['ratio_smape'][0]:.3f}")
+        return r
+
+    out = {"reference_spp": a.ref_spp, "budget_spp": a.budget}
+    ctx.clear()
+    ctx.render(spp=a.budget, flags=base, max_depth=depth, seed=77)
+    out["plain_at_budget"] = metrics(ctx.image(), ref)
+    out["sdtree"] = leg("SD-tree", lambda s: guided_step(s), a.seeds, seed0=1000)
+    out["hashgrid"] = leg("hash grid", lambda s: grid_step(s), a.seeds, seed0=2000)
+    out["sdtree_ris"] = leg("SD-tree + RIS", lambda s: guided_step(s, flags=ris), a.seeds, plain_flags=ris,
+                            rate=plain_ris_rate, seed0=3000)
+    out["hashgrid_ris"] = leg("hash grid + RIS", lambda s: grid_step(s, flags=ris), a.seeds, plain_flags=ris,
+                              rate=plain_ris_rate, seed0=4000)
+    out["mlp"] = leg("learned guide (spatial cache)", lambda s: mlp_step(s), a.seeds, seed0=5000)
+    out["mlp"].update({"train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps, "cache_res": a.mlp_cache})
+    pv = MLPGuide(ctx, lo, hi, seed=1)
+    out["mlp_per_vertex_inference"] = leg("learned guide (per-vertex inference)", lambda s: mlp_step(s, g=pv), 1,
+                                          seed0=6000)
+    pv.close()
+    # the SD-tree with the reference's defaults: fraction 0.3, Nearest filters
+    ref_tree = RTSTreeWrapper(ctx, lo, hi, 20, 0.3)
+    ref_tree.SetSplit(mode=1, c=12000.0, max_mb=2000, subdiv_th=0.01)
+
+    def ref_tree_step(s):
+        ctrl = RTSTreeController(ref_tree, a.budget, 0, 0.5, 1)
+        ctx.clear()
+        return render_guided(ctx, ctrl, ref_tree, "tree", base, max_depth=depth, seed=s, fraction=0.3)
+
+    out["sdtree_reference_defaults"] = leg("SD-tree, fraction 0.3, Nearest", ref_tree_step, 2, seed0=7000)
+    out["sdtree_reference_defaults"].update({"fraction": 0.3, "filters": "nearest"})
+    ref_tree.Destroy()
+    if a.large_budget > 0:
+        def tree_large(s):
+            ctrl = RTSTreeController(tree, a.large_budget, 0, 0.5, 1)
+            ctx.clear()
+            return render_guided(ctx, ctrl, tree, "tree", base, max_depth=depth, seed=s, fraction=frac)
+
+        out[f"budget_{a.large_budget}"] = {
+            "sdtree": leg(f"SD-tree {a.large_budget} spp", tree_large, 1, seed0=8000),
+            "mlp": leg(f"learned guide {a.large_budget} spp", lambda s: mlp_step(s, budget=a.large_budget), 1,
+                       seed0=9000)}
+    mlp.close()
+    return out
 
 
-def other_configs(ctx, timed):
+def load_counters(W, H):
+    """Per-dispatch counter averages of one guided step of this workload,
+    from the committed rocprofv3 --pmc summary (or None)."""
+    try:
+        t = json.loads(PMC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if t.get("width") != W or t.get("height") != H:
+        return None
+    return t
+
+
+def extend_roofline(W, H, per_kernel, launches, ext_bytes, names, plain_kernel_ms):
+    """k_extend (closest-hit BVH4 traversal) against the VALU issue roofline:
+    SQ_INSTS_VALU per launch (counter summary of this workload) over the
+    launch time measured in this run.  Beside it, its HBM traffic (FETCH_SIZE
+    and WRITE_SIZE per launch, each scaled by its reading for a copy kernel of
+    known size) and the algorithmic bytes it touches (rays, visited nodes,
+    tested triangles), most of which hit L2 and the MALL."""
+    n_ext = int(launches[0])
+    avg_s = float(per_kernel[0]) / max(n_ext, 1) * 1e-3
+    pmc = load_counters(W, H)
+    r = {"kernel": "k_extend (closest-hit BVH4 traversal)", "bound": "valu", "unit": "G wave-instructions/s",
+         "peak": VALU_PEAK_GWIS, "achieved": None, "frac": None, "traffic": None,
+         "launches_per_step": n_ext, "avg_launch_ms": round(avg_s * 1e3, 4),
+         "kernel_ms_per_step": {n: round(float(x), 2) for n, x in zip(names, per_kernel)},
+         "plain_kernel_ms_per_step": {n: round(float(plain_kernel_ms[n]), 2) for n in names},
+         "kernel_launches_per_step": {n: int(x) for n, x in zip(names, launches)},
+         "touched_bytes_per_launch": round(ext_bytes / max(n_ext, 1)),
+         "touched_gbs": round(ext_bytes / max(n_ext, 1) / avg_s / 1e9, 1) if avg_s > 0 else None}
+    if pmc is not None and avg_s > 0:
+        e = pmc["kernels"]["k_extend"]
+        valu = e["SQ_INSTS_VALU"]
+        r["achieved"] = round(valu / avg_s / 1e9, 1)
+        r["frac"] = round(valu / avg_s / 1e9 / VALU_PEAK_GWIS, 4)
+        r["valu_per_launch"] = valu
+        r["valu_per_wave"] = round(valu / e["SQ_WAVES"], 1)
+        traffic = e["FETCH_SIZE"] * 1024 * pmc["fetch_scale"] + e["WRITE_SIZE"] * 1024 * pmc["write_scale"]
+        r["traffic"] = round(traffic)
+        r["hbm"] = {"gbs": round(traffic / avg_s / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
+                    "frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic_over_touched": round(traffic / max(ext_bytes / max(n_ext, 1), 1), 3)}
+        r["counters_source"] = f"profiles/{PMC_FILE.name}"
+    return r
+
+
+def other_configs(ctx, timed, frac):
     """Throughput (and, for the learned guide, equal-time relMSE) of the other
     benchmark configurations: Cornell box plain path tracing (256^2, 64 spp,
     depth 8, RR after 3); many-light RIS (2000 icospheres, 2.56M tris, 4096
@@ -405,14 +489,14 @@ def other_configs(ctx, timed):
     t_mlp, _ = timed(mlp_step)
     img_mlp = ctx.image()
     mlp.close()
-    tree = RTSTreeWrapper(ctx, lo, hi, 20, 0.3)
+    tree = RTSTreeWrapper(ctx, lo, hi, 20, frac)
     tree.SetSplit(mode=1, c=12000.0)
 
     def tree_step():
         tree.Clear()
         ctrl = RTSTreeController(tree, 64, 0, 0.5, 1)
         ctx.clear()
-        return render_guided(ctx, ctrl, tree, "tree", nee_mis_rr, max_depth=8, seed=22)
+        return render_guided(ctx, ctrl, tree, "tree", nee_mis_rr, max_depth=8, seed=22, fraction=frac)
 
     timed(tree_step)
     t_tree, _ = timed(tree_step)
@@ -426,7 +510,7 @@ def other_configs(ctx, timed):
                      "plain_relmse": round(rel_mse(ctx.image(), ref), 6)}
     ctx.clear()
     ctx.render(spp=64, flags=nee_mis_rr, max_depth=8, seed=24)
-    res["plain_at_budget"] = {"spp": 64, "relmse": round(rel_mse(ctx.image(), ref), 6)}
+    res["plain_at_budget"] = {"spp": 64, "ms": round(t * 1e3, 2), "relmse": round(rel_mse(ctx.image(), ref), 6)}
     out["cornell_guided_equal_time"] = res
     log(f"cornell guided: {res}")
     # many-light RIS
@@ -482,20 +566,6 @@ def other_configs(ctx, timed):
     return out
 
 
-def _traffic(W, H):
-    """HBM bytes per k_extend launch from the committed rocprofv3 --pmc
-    summary of this workload (FETCH_SIZE and WRITE_SIZE per dispatch, each
-    scaled by the counter's reading for a copy kernel of known size)."""
-    try:
-        t = json.loads(TRAFFIC_FILE.read_text())
-    except (OSError, ValueError):
-        return None, None
-    if t.get("width") != W or t.get("height") != H:
-        return None, None
-    b = t["fetch_kib_per_dispatch"] * 1024 * t["fetch_scale"] + t["write_kib_per_dispatch"] * 1024 * t["write_scale"]
-    return round(b), TRAFFIC_FILE.name
-
-
 def copy_calibration(mib):
     import torch
     n = mib * (1 << 20) // 4
@@ -510,9 +580,10 @@ def copy_calibration(mib):
     return 0
 
 
-def cpu_baseline(sc, tree, lo, hi, W, H, depth, seconds):
+def cpu_baseline(sc, tree, lo, hi, W, H, depth, seconds, frac):
     """Times the oracle's CPU integrator on a bounded pixel sample of the same
-    guided render (final-pass state of the trained tree)."""
+    guided render (final-pass state of the trained tree), with one thread per
+    CPU this process may run on, at most the 16 CPUs a one-GPU box allots."""
     try:
         from oracle import oracle
         from rtlib_amd import scenes
@@ -526,13 +597,14 @@ def cpu_baseline(sc, tree, lo, hi, W, H, depth, seconds):
     t_build = time.time() - t0
     state = tree.Download()
     guide = oracle.Guide(state, lo, float(np.max(hi - lo)))
-    threads = min(os.cpu_count() or 1, 16)
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = min(avail, 16)
     flags = 1 | 128 | 256 | 8 | 16 | 32  # NEE, MIS, RR, tree, guide sampling, final
     # calibrate on one row, then size the sample to the time budget
     acc = np.zeros((H, W, 3), np.float32)
     t0 = time.perf_counter()
     oracle.render(osc, uvw, W, H, 1, flags, max_depth=depth, seed=5, guide=guide, px_range=(0, W),
-                  threads=threads, accum=acc)
+                  threads=threads, accum=acc, fraction=frac)
     t_row = max(time.perf_counter() - t0, 1e-3)
     rows = int(max(1, min(H, seconds / t_row)))
     stride = max(1, H // rows)
@@ -540,13 +612,14 @@ def cpu_baseline(sc, tree, lo, hi, W, H, depth, seconds):
     for r in range(0, H, stride):
         t0 = time.perf_counter()
         _, nr = oracle.render(osc, uvw, W, H, 1, flags, max_depth=depth, seed=5, guide=guide,
-                              px_range=(r * W, (r + 1) * W), threads=threads, accum=acc)
+                              px_range=(r * W, (r + 1) * W), threads=threads, accum=acc, fraction=frac)
         t_total += time.perf_counter() - t0
         total_px += W
         rays += nr
         if t_total > seconds:
             break
     return {"value": round(total_px / t_total / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(), "cpus_available": avail,
             "sample": f"{total_px} camera paths (1 spp on every {stride}th row of {W}x{H}), guided final pass "
                       f"with the GPU-trained SD-tree; BVH build {t_build:.1f} s excluded",
             "mrays_per_s": round(rays / t_total / 1e6, 3)}

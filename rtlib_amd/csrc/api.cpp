@@ -726,6 +726,7 @@ static int trace_common(rtpg_context ctx, const float* rays, uint32_t n, float* 
     d_rays.upload(reinterpret_cast<const float4*>(rays), 2 * (size_t)n, s);
     if (any) d_occ.alloc(n); else d_hits.alloc(n);
     if (ctx->max_stack > kShortStack) d_deep.alloc(2 * (size_t)(ctx->max_stack - kShortStack) * n);
+    ctx->p_err.zero(1, s);  // the stack-bound flag reports this call only
     RTPG_HIP(launch_trace_rays(ctx->scene_view(), d_rays.p, n, d_hits.p, d_occ.p, any,
                                ctx->max_stack > kShortStack ? d_deep.p : nullptr, ctx->p_err.p, s));
     if (any) d_occ.download(occl, n, s); else d_hits.download(reinterpret_cast<float4*>(hits), n, s);

@@ -82,23 +82,24 @@ struct rtpg_sdtree_t {
   int max_mb = 2000;
   float subdiv_th = 0.01f;
   bool octree = false;  // refine splits a leaf into 8 (RTSTree2T) instead of 2
-  // Host mirror of the spatial tree topology: leaf flag, first child / DTree.
-  struct SNode {
-    bool leaf;
-    uint32_t child;
-    uint32_t dtree;
-  };
-  std::vector<SNode> snodes;
-  rtpg::DBuf<uint32_t> d_stree;
-  rtpg::DBuf<uint32_t> d_jump;  // lookup table of the first 3 * kJumpBits levels
-  rtpg::DBuf<uint32_t> d_leaf_depth;  // per DTree: depth of its STree leaf
+  // Spatial tree on the device: node words (leaf: kLeafBit | DTree, inner:
+  // index of the first of its two children), the leaves in the refine's walk
+  // order, the depth of every DTree's leaf, and the lookup table of the first
+  // 3 * kJumpBits levels; the *2 buffers are the other half of the refine's
+  // ping-pong.
+  uint32_t n_snodes = 0;
+  rtpg::DBuf<uint32_t> d_stree, d_stree2;
+  rtpg::DBuf<uint32_t> d_order, d_order2;
+  rtpg::DBuf<uint32_t> d_leaf_depth, d_leaf_depth2;
+  rtpg::DBuf<uint32_t> d_jump;
   static constexpr uint32_t kJumpBits = 5;
   // recording filters (RecordFilter bits); rtpg_sdtree_record draws from
   // filter_rng(filter_seed, i, record_batch, 0), record_batch counting its calls
   uint32_t filter = 0;
   uint64_t filter_seed = 0;
   uint32_t record_batch = 0;
-  void upload_stree();
+  // host-side topology (import): node words, walk order and leaf depths uploaded, jump table built
+  void upload_stree(const std::vector<uint32_t>& words);
   // device DTree arrays (owned)
   struct Owned {
     uint32_t n = 0, b_nodes = 0, s_nodes = 0;
@@ -113,7 +114,11 @@ struct rtpg_sdtree_t {
   } dt, spare;  // spare: the other set of reset()'s ping-pong
   rtpg::DBuf<float> area_tmp;
   rtpg::DBuf<uint32_t> d_map, d_total;
-  rtpg::DBuf<uint32_t> d_err;
+  // refine temporaries: per old DTree (reset counts, leaves, splits, their
+  // prefix sums), per new node (leaf flag, rank, source, depth), counters
+  rtpg::DBuf<uint32_t> d_cnt_old, d_leaves, d_splits, d_leaves_excl;
+  rtpg::DBuf<uint32_t> d_is_leaf, d_rank, d_src, d_depth;
+  rtpg::DBuf<unsigned long long> d_stat;
 
   rtpg::SdTreeView view();
   void clear();

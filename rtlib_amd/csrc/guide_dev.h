@@ -8,7 +8,6 @@
 namespace rtpg {
 
 constexpr uint32_t kInvalidCell = 0xFFFFFFFFu;
-constexpr uint32_t kLeafBit = 0x80000000u;
 
 // ---------------------------------------------------------------- SD-tree
 // STreeT::GetDTreeWrapper (OPX7UtilsPathGuiding.h:639-648, descent :561-567).

@@ -3,6 +3,8 @@
 // spatial split, and record/sample/pdf/lookup entry kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.h"
 #include "guide_dev.h"
 #include "sdtree.h"
@@ -151,7 +153,7 @@ __global__ void k_dtree_reset(DTreeArrays src, DTreeArrays dst, const uint32_t* 
 #pragma clang fp contract(off)
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= dst.n) return;
-  uint32_t sj = map[j];
+  uint32_t sj = map ? map[j] : j;
   uint32_t sbase = src.s_base[sj];
   float total = src.s_sum[sj];
   uint32_t obase = pass ? dst.b_base[j] : 0u;
@@ -290,6 +292,183 @@ __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* tile_sums)
   if (g < n) out[g] += tile_sums[blockIdx.x];
 }
 
+// ------------------------------------------------------------------ STree refine
+// RTSTreeT::Refine (OPX7UtilsPathGuiding.h:1275-1311) and RTSTree2T's
+// SubDivide (:2062-2077) on the device.  The reference walks the tree depth
+// first from a stack (the second child is visited first) and appends the
+// children of every leaf it splits at the end of the node array, so:
+//  - a leaf of building weight w > th splits, and its children inherit w / 2
+//    (w / 8 for the octree) and are visited next: the leaf becomes a complete
+//    subtree of k splits, b^k leaves (b = 2 or 8) and 2 (b^k - 1) new nodes
+//    (a binary split adds 2 nodes and 1 leaf, an octree split 14 and 7);
+//  - the new nodes of the leaves form one block per split leaf, the blocks in
+//    the walk's order of the old leaves.
+// With the old leaves listed in that order (order[q]), an exclusive prefix
+// sum L of b^k gives every block's start, N_old + 2 (L[q] - q), and the
+// position of every new leaf in the next walk order; a second prefix sum over
+// the leaf flags of the new node array numbers the DTrees in node order
+// (RTSTreeWrapperT::Upload :1480-1512).
+
+// Bytes of RTSTreeWrapperT::GetApproxMemoryFootPrint summed over the nodes: a
+// leaf's wrapper holds its two DTrees (24 bytes a node and 48), an inner node
+// a default wrapper of two one-node trees (96 bytes).
+__global__ void k_stree_footprint(const uint32_t* stree, uint32_t n, const uint32_t* b_cnt, const uint32_t* s_cnt,
+                                  unsigned long long* stat) {
+  __shared__ unsigned long long part[256];
+  unsigned long long acc = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t w = stree[i];
+    if (w & kLeafBit) {
+      uint32_t d = w & ~kLeafBit;
+      acc += 24ull * ((unsigned long long)b_cnt[d] + s_cnt[d]) + 48ull;
+    } else {
+      acc += 96ull;
+    }
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(stat + kStatFootprint, part[0]);
+}
+
+// Split count k of the q-th leaf in walk order, b^k, and the sums the host
+// sizes the new arrays from: leaves, building nodes (b^k copies of the leaf's
+// reset tree) and sampling nodes (b^k copies of its sampling tree).
+__global__ void k_stree_split_count(const uint32_t* stree, const uint32_t* order, uint32_t n_leaves,
+                                    const float* b_sw, const uint32_t* reset_cnt, const uint32_t* s_cnt, float th,
+                                    uint32_t octree, int max_mb, unsigned long long* stat, uint32_t* leaves,
+                                    uint32_t* splits) {
+#pragma clang fp contract(off)
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_leaves) return;
+  bool refine = max_mb < 0 || stat[kStatFootprint] / 1000000ull < (unsigned long long)max_mb;
+  uint32_t d = stree[order[q]] & ~kLeafBit;
+  const uint32_t lb = octree ? 3u : 1u, kmax = octree ? 10u : 30u;
+  float w = b_sw[d];
+  uint32_t k = 0;
+  while (refine && w > th) {
+    if (k == kmax) {
+      atomicOr(reinterpret_cast<unsigned int*>(stat + kStatErr), kErrStreeSize);
+      break;
+    }
+    w = octree ? w / 8.0f : w / 2.0f;
+    ++k;
+  }
+  uint32_t nl = 1u << (lb * k);
+  leaves[q] = nl;
+  splits[q] = k;
+  atomicAdd(stat + kStatLeaves, (unsigned long long)nl);
+  atomicAdd(stat + kStatBuildNodes, (unsigned long long)nl * reset_cnt[d]);
+  atomicAdd(stat + kStatSampNodes, (unsigned long long)nl * s_cnt[d]);
+}
+
+// New node array, first the old nodes: leaf flag, source DTree and depth.
+__global__ void k_stree_copy(const uint32_t* stree, uint32_t n, const uint32_t* leaf_depth, uint32_t* out,
+                             uint32_t* is_leaf, uint32_t* src, uint32_t* depth) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w = stree[i];
+  out[i] = w;
+  is_leaf[i] = (w & kLeafBit) ? 1u : 0u;
+  if (w & kLeafBit) {
+    src[i] = w & ~kLeafBit;
+    depth[i] = leaf_depth[w & ~kLeafBit];
+  }
+}
+
+// One work-item per leaf g of the new tree (walk order): it finds its old
+// leaf q (last q with L[q] <= g) and its rank t among q's b^k leaves, and
+// descends q's subtree by the base-b digits of t, most significant first:
+// in a split unit placed at p (binary: the pair p, p + 1; octree: inner pairs
+// p..p+5 whose children are p + 2 + 2m, leaves p + 6..p + 13) the walk visits
+// the unit's leaves from the last node down, so digit u selects node
+// p + U - 1 - u (U = 2 or 14), and the subtree below it occupies
+// [p + U + u S, p + U + (u + 1) S) with S = 2 (b^(r-1) - 1) for r splits left.
+// The first leaf of a subtree writes its split unit; every node is written once.
+__global__ void k_stree_emit(const uint32_t* stree_old, uint32_t n_old, const uint32_t* order_old,
+                             uint32_t n_leaves_old, const uint32_t* leaves_excl, const uint32_t* splits,
+                             const uint32_t* leaf_depth, uint32_t octree, uint32_t n_leaves, uint32_t* out,
+                             uint32_t* is_leaf, uint32_t* src, uint32_t* depth, uint32_t* order_new) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_leaves) return;
+  uint32_t lo = 0, hi = n_leaves_old - 1u;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1u) / 2u;
+    if (leaves_excl[mid] <= g) lo = mid; else hi = mid - 1u;
+  }
+  const uint32_t q = lo, t = g - leaves_excl[q], k = splits[q];
+  uint32_t node = order_old[q];
+  const uint32_t d = stree_old[node] & ~kLeafBit;
+  uint32_t dep = leaf_depth[d];
+  if (k > 0u) {
+    const uint32_t lb = octree ? 3u : 1u, U = octree ? 14u : 2u;
+    uint32_t p = n_old + 2u * (leaves_excl[q] - q);
+    for (uint32_t r = k; r > 0u; --r) {
+      const uint32_t shift = lb * (r - 1u);
+      if ((t & ((1u << (lb * r)) - 1u)) == 0u) {
+        out[node] = p;
+        is_leaf[node] = 0u;
+        if (octree) {
+          for (uint32_t m = 0; m < 6u; ++m) {
+            out[p + m] = p + 2u + 2u * m;
+            is_leaf[p + m] = 0u;
+          }
+        }
+      }
+      const uint32_t u = (t >> shift) & ((1u << lb) - 1u);
+      const uint32_t S = 2u * ((1u << shift) - 1u);
+      node = p + U - 1u - u;
+      p = p + U + u * S;
+      dep += lb;
+    }
+  }
+  is_leaf[node] = 1u;
+  src[node] = d;
+  depth[node] = dep;
+  order_new[g] = node;
+}
+
+// Leaves numbered in node order: node word, source of each new DTree and the
+// depth of its leaf.
+__global__ void k_stree_finalize(uint32_t* out, uint32_t n, const uint32_t* is_leaf, const uint32_t* rank,
+                                 const uint32_t* src, const uint32_t* depth, uint32_t* map, uint32_t* leaf_depth) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !is_leaf[i]) return;
+  uint32_t r = rank[i];
+  out[i] = kLeafBit | r;
+  map[r] = src[i];
+  leaf_depth[r] = depth[i];
+}
+
+// Node counts of the new DTrees from their sources.
+__global__ void k_dtree_gather_counts(const uint32_t* map, uint32_t n, const uint32_t* s_cnt_old,
+                                      const uint32_t* reset_cnt, uint32_t* s_cnt, uint32_t* b_cnt) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t sj = map[j];
+  s_cnt[j] = s_cnt_old[sj];
+  b_cnt[j] = reset_cnt[sj];
+}
+
+// Lookup table: the node word after the first 3 G levels of the descent for
+// every cell of the 2^G cubed grid (child along an axis = bit G-1-level of the
+// cell coordinate; the leaf word where the descent ends earlier).
+__global__ void k_stree_jump(const uint32_t* stree, uint32_t G, uint32_t* jump) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = 1u << G;
+  if (c >= n * n * n) return;
+  uint32_t xyz[3] = {c % n, (c / n) % n, c / (n * n)};
+  uint32_t w = stree[0];
+  for (uint32_t k = 0; k < 3u * G && !(w & kLeafBit); ++k) {
+    uint32_t bit = (xyz[k % 3u] >> (G - 1u - k / 3u)) & 1u;
+    w = stree[w + bit];
+  }
+  jump[c] = w;
+}
+
 // DTreeT::Impl<Nearest>::RecordIrradiance (:419-427) at the DTree found by
 // the STree lookup of `pos`; with v.filter set, the stochastic box filters
 // (stree_box_jitter, dtree_box_jitter) move the record first, drawing five
@@ -390,6 +569,49 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, 
   e = hipGetLastError();
   hipFreeAsync(tile_sums, s);
   return e;
+}
+hipError_t launch_stree_footprint(const uint32_t* stree, uint32_t n, const uint32_t* b_cnt, const uint32_t* s_cnt,
+                                  unsigned long long* stat, hipStream_t s) {
+  uint32_t nb = std::min(nblk(n, 256), 1024u);
+  hipLaunchKernelGGL(k_stree_footprint, dim3(nb), dim3(256), 0, s, stree, n, b_cnt, s_cnt, stat);
+  return hipGetLastError();
+}
+hipError_t launch_stree_split_count(const uint32_t* stree, const uint32_t* order, uint32_t n_leaves,
+                                    const float* b_sw, const uint32_t* reset_cnt, const uint32_t* s_cnt, float th,
+                                    bool octree, int max_mb, unsigned long long* stat, uint32_t* leaves,
+                                    uint32_t* splits, hipStream_t s) {
+  hipLaunchKernelGGL(k_stree_split_count, dim3(nblk(n_leaves, 256)), dim3(256), 0, s, stree, order, n_leaves, b_sw,
+                     reset_cnt, s_cnt, th, octree ? 1u : 0u, max_mb, stat, leaves, splits);
+  return hipGetLastError();
+}
+hipError_t launch_stree_rebuild(const uint32_t* stree_old, uint32_t n_old, const uint32_t* order_old,
+                                uint32_t n_leaves_old, const uint32_t* leaves_excl, const uint32_t* splits,
+                                const uint32_t* leaf_depth_old, bool octree, uint32_t n_new, uint32_t n_leaves,
+                                StreeScratch t, uint32_t* stree_new, uint32_t* order_new, uint32_t* map,
+                                uint32_t* leaf_depth_new, hipStream_t s) {
+  hipLaunchKernelGGL(k_stree_copy, dim3(nblk(n_old, 256)), dim3(256), 0, s, stree_old, n_old, leaf_depth_old,
+                     stree_new, t.is_leaf, t.src, t.depth);
+  hipLaunchKernelGGL(k_stree_emit, dim3(nblk(n_leaves, 256)), dim3(256), 0, s, stree_old, n_old, order_old,
+                     n_leaves_old, leaves_excl, splits, leaf_depth_old, octree ? 1u : 0u, n_leaves, stree_new,
+                     t.is_leaf, t.src, t.depth, order_new);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = launch_exclusive_scan(t.is_leaf, t.rank, n_new, t.total, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_stree_finalize, dim3(nblk(n_new, 256)), dim3(256), 0, s, stree_new, n_new, t.is_leaf, t.rank,
+                     t.src, t.depth, map, leaf_depth_new);
+  return hipGetLastError();
+}
+hipError_t launch_dtree_gather_counts(const uint32_t* map, uint32_t n, const uint32_t* s_cnt_old,
+                                      const uint32_t* reset_cnt, uint32_t* s_cnt, uint32_t* b_cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_dtree_gather_counts, dim3(nblk(n, 256)), dim3(256), 0, s, map, n, s_cnt_old, reset_cnt, s_cnt,
+                     b_cnt);
+  return hipGetLastError();
+}
+hipError_t launch_stree_jump(const uint32_t* stree, uint32_t G, uint32_t* jump, hipStream_t s) {
+  uint32_t cells = 1u << (3u * G);
+  hipLaunchKernelGGL(k_stree_jump, dim3(nblk(cells, 256)), dim3(256), 0, s, stree, G, jump);
+  return hipGetLastError();
 }
 hipError_t launch_sdtree_record(SdTreeView v, const float* pos, const float* canon, const float* irr,
                                 const float* sw, uint32_t n, uint64_t seed, uint32_t batch, hipStream_t s) {

@@ -97,54 +97,34 @@ static void alloc_per_dtree(rtpg_sdtree_t::Owned& o, uint32_t n) {
   o.info.reserve(n);
 }
 
-static std::vector<uint32_t> encode_stree(const std::vector<rtpg_sdtree_t::SNode>& sn) {
-  std::vector<uint32_t> e(sn.size());
-  for (size_t i = 0; i < sn.size(); ++i) e[i] = sn[i].leaf ? (0x80000000u | sn[i].dtree) : sn[i].child;
-  return e;
-}
-
-// Node words after 3G levels of the descent for every cell of the 2^G cubed
-// grid (child index along an axis = bit G-1-level of the cell coordinate).
-static std::vector<uint32_t> build_jump(const std::vector<rtpg_sdtree_t::SNode>& sn, uint32_t G) {
-  const uint32_t n = 1u << G;
-  std::vector<uint32_t> jt((size_t)n * n * n);
-  for (uint32_t z = 0; z < n; ++z)
-    for (uint32_t y = 0; y < n; ++y)
-      for (uint32_t x = 0; x < n; ++x) {
-        uint32_t node = 0, c[3] = {x, y, z};
-        for (uint32_t k = 0; k < 3 * G && !sn[node].leaf; ++k) {
-          uint32_t bit = (c[k % 3] >> (G - 1 - k / 3)) & 1u;
-          node = sn[node].child + bit;
-        }
-        jt[((size_t)z * n + y) * n + x] = sn[node].leaf ? (0x80000000u | sn[node].dtree) : sn[node].child;
-      }
-  return jt;
-}
-
-// Depth of every DTree's leaf (root = 0), for the voxel size of the spatial
-// box filter.
-static std::vector<uint32_t> leaf_depths(const std::vector<rtpg_sdtree_t::SNode>& sn) {
-  std::vector<uint32_t> out;
+// Node words of a validated topology to the device, with the leaves in the
+// refine's walk order (depth first, second child first) and the depth of
+// every DTree's leaf (root = 0, for the voxel size of the spatial box filter);
+// the lookup table is built on the device.
+void rtpg_sdtree_t::upload_stree(const std::vector<uint32_t>& words) {
+  hipStream_t s = ctx->stream;
+  std::vector<uint32_t> order, depth;
   std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 0u}};  // (node, depth)
   while (!stack.empty()) {
     auto [i, d] = stack.back();
     stack.pop_back();
-    if (sn[i].leaf) {
-      if (sn[i].dtree >= out.size()) out.resize((size_t)sn[i].dtree + 1u, 0u);
-      out[sn[i].dtree] = d;
+    uint32_t w = words[i];
+    if (w & kLeafBit) {
+      uint32_t t = w & ~kLeafBit;
+      if (t >= depth.size()) depth.resize((size_t)t + 1u, 0u);
+      depth[t] = d;
+      order.push_back(i);
     } else {
-      stack.push_back({sn[i].child, d + 1u});
-      stack.push_back({sn[i].child + 1u, d + 1u});
+      stack.push_back({w, d + 1u});
+      stack.push_back({w + 1u, d + 1u});
     }
   }
-  return out;
-}
-
-void rtpg_sdtree_t::upload_stree() {
-  hipStream_t s = ctx->stream;
-  d_stree.upload(encode_stree(snodes), s);
-  d_jump.upload(build_jump(snodes, kJumpBits), s);
-  d_leaf_depth.upload(leaf_depths(snodes), s);
+  n_snodes = (uint32_t)words.size();
+  d_stree.upload(words, s);
+  d_order.upload(order, s);
+  d_leaf_depth.upload(depth, s);
+  d_jump.reserve((size_t)1 << (3 * kJumpBits));
+  RTPG_HIP(launch_stree_jump(d_stree.p, kJumpBits, d_jump.p, s));
 }
 
 SdTreeView rtpg_sdtree_t::view() {
@@ -185,8 +165,7 @@ void rtpg_sdtree_t::publish() {
 // whose DTrees (building, sampling) are one empty node each (RTDTreeT ctor :879-886).
 void rtpg_sdtree_t::clear() {
   hipStream_t s = ctx->stream;
-  snodes.assign(1, SNode{true, 0u, 0u});
-  upload_stree();
+  upload_stree({kLeafBit});
   alloc_per_dtree(dt, 1);
   std::vector<uint32_t> zero_u{0u}, one_u{1u};
   std::vector<float> zero_f{0.0f};
@@ -240,7 +219,8 @@ void rtpg_sdtree_t::build() {
 }
 
 // RTSTreeWrapperT::Reset (:1645-1656): refine (RTSTreeT::Refine :1275-1311)
-// then per leaf RTDTreeWrapperT::Reset(maxDepth, 0.01).
+// then per leaf RTDTreeWrapperT::Reset(maxDepth, 0.01), all on the device; the
+// host reads five counters once to size the new arrays.
 void rtpg_sdtree_t::reset(int iter, int spp) {
   if (iter <= 0) return;
   hipStream_t s = ctx->stream;
@@ -250,166 +230,67 @@ void rtpg_sdtree_t::reset(int iter, int spp) {
   } else {
     th = (size_t)((double)split_c * std::sqrt(std::pow(2.0, iter)));
   }
-  uint32_t n = dt.n;
-  // the building weights and both trees' node counts, with one host wait
-  std::vector<float> bsw(n);
-  std::vector<uint32_t> bcnt(n), scnt(n);
-  if (n) {
-    RTPG_HIP(hipMemcpyAsync(bsw.data(), dt.b_sw.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-    RTPG_HIP(hipMemcpyAsync(bcnt.data(), dt.b_cnt.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
-    RTPG_HIP(hipMemcpyAsync(scnt.data(), dt.s_cnt.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
-  }
-  RTPG_HIP(hipStreamSynchronize(s));
-  // Host topology with per-leaf (building weight, source DTree).
-  struct HN {
-    bool leaf;
-    uint8_t axis;
-    uint32_t child[2];
-    float sw;
-    int64_t src;  // DTree the wrapper was copied from; -1 for inner defaults
-  };
-  std::vector<HN> hn(snodes.size());
-  for (size_t i = 0; i < snodes.size(); ++i) {
-    hn[i].leaf = snodes[i].leaf;
-    hn[i].child[0] = snodes[i].child;
-    hn[i].child[1] = snodes[i].child + 1u;
-    hn[i].sw = snodes[i].leaf ? bsw[snodes[i].dtree] : 0.0f;
-    hn[i].src = snodes[i].leaf ? (int64_t)snodes[i].dtree : -1;
-    hn[i].axis = 0;
-  }
-  // axes by depth (root axis 0, children (axis+1)%3)
-  {
-    std::vector<std::pair<uint32_t, uint8_t>> st{{0u, (uint8_t)0}};
-    while (!st.empty()) {
-      auto [i, ax] = st.back();
-      st.pop_back();
-      hn[i].axis = ax;
-      if (!hn[i].leaf) {
-        st.push_back({hn[i].child[0], (uint8_t)((ax + 1) % 3)});
-        st.push_back({hn[i].child[1], (uint8_t)((ax + 1) % 3)});
-      }
-    }
-  }
-  bool refine = true;
-  if (max_mb >= 0) {
-    size_t fp = 0;
-    for (const HN& h : hn) {
-      if (h.leaf) {
-        fp += 24u * ((size_t)bcnt[(size_t)h.src] + scnt[(size_t)h.src]) + 48u;
-      } else {
-        fp += 96u;  // a default RTDTreeWrapper: two one-node trees
-      }
-    }
-    if (fp / 1000000 >= (size_t)max_mb) refine = false;
-  }
-  if (refine) {
-    std::vector<std::pair<size_t, int>> st{{0, 1}};
-    while (!st.empty()) {
-      auto [idx, depth] = st.back();
-      st.pop_back();
-      if (hn[idx].leaf) {
-        bool split = hn.size() < (size_t)std::numeric_limits<uint32_t>::max() - 1 && hn[idx].sw > (float)th;
-        if (split && !octree) {
-          size_t c = hn.size();
-          hn.resize(c + 2);
-          HN& cur = hn[idx];
-          for (int i = 0; i < 2; ++i) {
-            HN& ch = hn[c + i];
-            ch.leaf = true;
-            ch.axis = (uint8_t)((cur.axis + 1) % 3);
-            ch.sw = cur.sw / 2.0f;
-            ch.src = cur.src;
-            ch.child[0] = ch.child[1] = 0;
-            cur.child[i] = (uint32_t)(c + i);
-          }
-          cur.leaf = false;
-          cur.src = -1;
-          cur.sw = 0.0f;
-        } else if (split) {
-          // RTSTree2T::SubDivide (:2062-2077): eight children of half the
-          // edge along every axis, each with an eighth of the weight, here as
-          // three binary levels below the leaf (pairs at c, c+2..c+5, and the
-          // eight leaves at c+6..c+13), so lookups and filters are unchanged
-          size_t c = hn.size();
-          hn.resize(c + 14);
-          const float sw8 = hn[idx].sw / 8.0f;
-          const int64_t src = hn[idx].src;
-          const uint8_t ax = hn[idx].axis;
-          auto make_inner = [&](size_t i, size_t first, uint8_t axis) {
-            hn[i].leaf = false;
-            hn[i].axis = axis;
-            hn[i].child[0] = (uint32_t)first;
-            hn[i].child[1] = (uint32_t)first + 1u;
-            hn[i].sw = 0.0f;
-            hn[i].src = -1;
-          };
-          make_inner(idx, c, ax);
-          for (size_t k = 0; k < 2; ++k) make_inner(c + k, c + 2 + 2 * k, (uint8_t)((ax + 1) % 3));
-          for (size_t k = 0; k < 4; ++k) make_inner(c + 2 + k, c + 6 + 2 * k, (uint8_t)((ax + 2) % 3));
-          for (size_t k = 0; k < 8; ++k) {
-            HN& ch = hn[c + 6 + k];
-            ch.leaf = true;
-            ch.axis = ax;
-            ch.sw = sw8;
-            ch.src = src;
-            ch.child[0] = ch.child[1] = 0;
-          }
-        }
-      }
-      if (!hn[idx].leaf) {
-        uint32_t c0 = hn[idx].child[0], c1 = hn[idx].child[1];
-        st.push_back({c0, depth + 1});
-        st.push_back({c1, depth + 1});
-      }
-    }
-  }
-  if (hn.size() >= 0x80000000ull) throw Error(RTPG_ERR_LIMIT, "spatial tree exceeds 2^31 nodes");
-  // New DTree order: leaves in node order (reference Upload :1480-1512).
-  std::vector<uint32_t> map;
-  std::vector<SNode> ns(hn.size());
-  for (size_t i = 0; i < hn.size(); ++i) {
-    ns[i].leaf = hn[i].leaf;
-    ns[i].child = hn[i].leaf ? 0u : hn[i].child[0];
-    ns[i].dtree = 0;
-    if (hn[i].leaf) {
-      ns[i].dtree = (uint32_t)map.size();
-      map.push_back((uint32_t)hn[i].src);
-    }
-  }
-  uint32_t n2 = (uint32_t)map.size();
-  // Device: new DTree arrays in the spare set (swapped in at the end).
-  Owned& nd = spare;
-  alloc_per_dtree(nd, n2);
-  d_map.upload(map, s);
+  const uint32_t n = dt.n, nn = n_snodes;
+  // 1. reset node counts of every old DTree, split counts of the old leaves
+  //    in walk order and their prefix sums
+  d_stat.reserve(kStatWords);
+  d_stat.zero(kStatWords, s);
+  uint32_t* err = reinterpret_cast<uint32_t*>(d_stat.p + kStatErr);
+  d_cnt_old.reserve(n);
+  d_leaves.reserve(n);
+  d_splits.reserve(n);
+  d_leaves_excl.reserve(n);
   d_total.reserve(1);
-  std::vector<uint32_t> s_cnt2(n2);
-  for (uint32_t j = 0; j < n2; ++j) s_cnt2[j] = scnt[map[j]];
-  uint32_t s_total = 0;
-  std::vector<uint32_t> s_base2 = exclusive_prefix(s_cnt2, s_total);
-  nd.s_cnt.upload(s_cnt2, s);
-  nd.s_base.upload(s_base2, s);
-  nd.s_nodes = s_total;
-  nd.s_sums.reserve(s_total);
-  nd.s_child.reserve(s_total);
-  d_err.reserve(1);
-  d_err.zero(1, s);
   DTreeArrays src = dt.view();
+  DTreeArrays cnt_view = src;
+  cnt_view.b_cnt = d_cnt_old.p;
+  RTPG_HIP(launch_dtree_reset(src, cnt_view, nullptr, max_dtree_depth, subdiv_th, 0, err, s));
+  if (max_mb >= 0) RTPG_HIP(launch_stree_footprint(d_stree.p, nn, dt.b_cnt.p, dt.s_cnt.p, d_stat.p, s));
+  RTPG_HIP(launch_stree_split_count(d_stree.p, d_order.p, n, dt.b_sw.p, d_cnt_old.p, dt.s_cnt.p, (float)th, octree,
+                                    max_mb, d_stat.p, d_leaves.p, d_splits.p, s));
+  RTPG_HIP(launch_exclusive_scan(d_leaves.p, d_leaves_excl.p, n, d_total.p, s));
+  unsigned long long stat[kStatWords];
+  d_stat.download(stat, kStatWords, s);  // the one host wait of the refine
+  if (stat[kStatErr] & kErrResetStack) throw Error(RTPG_ERR_LIMIT, "DTree reset stack bound exceeded");
+  const unsigned long long n2 = stat[kStatLeaves];
+  const unsigned long long nn2 = (unsigned long long)nn + 2ull * (n2 - n);
+  if ((stat[kStatErr] & kErrStreeSize) || nn2 >= 0x80000000ull)
+    throw Error(RTPG_ERR_LIMIT, "spatial tree exceeds 2^31 nodes");
+  if (stat[kStatBuildNodes] > 0xFFFFFFFFull || stat[kStatSampNodes] > 0xFFFFFFFFull)
+    throw Error(RTPG_ERR_LIMIT, "DTree nodes exceed 2^32");
+  // 2. new node array, walk order, DTree sources and leaf depths
+  d_stree2.reserve(nn2);
+  d_order2.reserve(n2);
+  d_leaf_depth2.reserve(n2);
+  d_map.reserve(n2);
+  d_is_leaf.reserve(nn2);
+  d_rank.reserve(nn2);
+  d_src.reserve(nn2);
+  d_depth.reserve(nn2);
+  StreeScratch t{d_is_leaf.p, d_rank.p, d_src.p, d_depth.p, d_total.p};
+  RTPG_HIP(launch_stree_rebuild(d_stree.p, nn, d_order.p, n, d_leaves_excl.p, d_splits.p, d_leaf_depth.p, octree,
+                                (uint32_t)nn2, (uint32_t)n2, t, d_stree2.p, d_order2.p, d_map.p, d_leaf_depth2.p, s));
+  // 3. new DTree arrays in the spare set (swapped in at the end): sampling
+  //    trees copied from the sources, building trees reset from them
+  Owned& nd = spare;
+  alloc_per_dtree(nd, (uint32_t)n2);
+  RTPG_HIP(launch_dtree_gather_counts(d_map.p, (uint32_t)n2, dt.s_cnt.p, d_cnt_old.p, nd.s_cnt.p, nd.b_cnt.p, s));
+  RTPG_HIP(launch_exclusive_scan(nd.s_cnt.p, nd.s_base.p, (uint32_t)n2, d_total.p, s));
+  RTPG_HIP(launch_exclusive_scan(nd.b_cnt.p, nd.b_base.p, (uint32_t)n2, d_total.p, s));
+  nd.s_nodes = (uint32_t)stat[kStatSampNodes];
+  nd.b_nodes = (uint32_t)stat[kStatBuildNodes];
+  nd.s_sums.reserve(nd.s_nodes);
+  nd.s_child.reserve(nd.s_nodes);
+  nd.b_sums.reserve(nd.b_nodes);
+  nd.b_child.reserve(nd.b_nodes);
   DTreeArrays dstv = nd.view();
   RTPG_HIP(launch_dtree_copy_sampling(src, dstv, d_map.p, s));
-  RTPG_HIP(launch_dtree_reset(src, dstv, d_map.p, max_dtree_depth, subdiv_th, 0, d_err.p, s));
-  RTPG_HIP(launch_exclusive_scan(nd.b_cnt.p, nd.b_base.p, n2, d_total.p, s));
-  uint32_t b_total = 0;
-  d_total.download(&b_total, 1, s);
-  nd.b_nodes = b_total;
-  nd.b_sums.reserve(b_total);
-  nd.b_child.reserve(b_total);
-  dstv = nd.view();
-  RTPG_HIP(launch_dtree_reset(src, dstv, d_map.p, max_dtree_depth, subdiv_th, 1, d_err.p, s));
-  uint32_t err = 0;
-  d_err.download(&err, 1, s);
-  if (err) throw Error(RTPG_ERR_LIMIT, "DTree reset stack bound exceeded");
-  snodes.swap(ns);
-  upload_stree();
+  RTPG_HIP(launch_dtree_reset(src, dstv, d_map.p, max_dtree_depth, subdiv_th, 1, err, s));
+  RTPG_HIP(launch_stree_jump(d_stree2.p, kJumpBits, d_jump.p, s));
+  std::swap(d_stree, d_stree2);
+  std::swap(d_order, d_order2);
+  std::swap(d_leaf_depth, d_leaf_depth2);
+  n_snodes = (uint32_t)nn2;
   std::swap(dt, spare);
   publish();
 }
@@ -576,7 +457,7 @@ int rtpg_sdtree_pdf(rtpg_sdtree tree, const uint32_t* dtree, const float* canon,
 int rtpg_sdtree_sizes(rtpg_sdtree tree, uint32_t sizes[4]) {
   return guarded([&] {
     require(tree && sizes, "NULL argument");
-    sizes[0] = (uint32_t)tree->snodes.size();
+    sizes[0] = tree->n_snodes;
     sizes[1] = tree->dt.n;
     sizes[2] = tree->dt.b_nodes;
     sizes[3] = tree->dt.s_nodes;
@@ -590,10 +471,7 @@ int rtpg_sdtree_export(rtpg_sdtree tree, uint32_t* stree, uint32_t* b_cnt, uint3
     hipStream_t s = tree->ctx->stream;
     auto& dt = tree->dt;
     uint32_t n = dt.n;
-    if (stree) {
-      std::vector<uint32_t> e = encode_stree(tree->snodes);
-      std::memcpy(stree, e.data(), e.size() * 4);
-    }
+    if (stree) tree->d_stree.download(stree, tree->n_snodes, s);
     if (b_cnt) dt.b_cnt.download(b_cnt, n, s);
     if (s_cnt) dt.s_cnt.download(s_cnt, n, s);
     if (scal) {
@@ -614,7 +492,6 @@ int rtpg_sdtree_import(rtpg_sdtree tree, uint32_t n_stree, const uint32_t* stree
     require(tree && stree && b_cnt && s_cnt && scal && b_sums && b_child && s_sums && s_child, "NULL argument");
     require(n_stree >= 1 && n_dtrees >= 1, "empty tree");
     // Validate the topology: pairs of children, every leaf names a distinct DTree.
-    std::vector<rtpg_sdtree_t::SNode> sn(n_stree);
     std::vector<uint8_t> seen(n_dtrees, 0);
     uint32_t leaves = 0;
     for (uint32_t i = 0; i < n_stree; ++i) {
@@ -623,11 +500,9 @@ int rtpg_sdtree_import(rtpg_sdtree tree, uint32_t n_stree, const uint32_t* stree
         uint32_t d = w & 0x7FFFFFFFu;
         require(d < n_dtrees && !seen[d], "bad DTree index in stree");
         seen[d] = 1;
-        sn[i] = {true, 0u, d};
         ++leaves;
       } else {
         require(w > i && w + 1u < n_stree, "bad child index in stree");
-        sn[i] = {false, w, 0u};
       }
     }
     require(leaves == n_dtrees, "stree leaves must match n_dtrees");
@@ -668,8 +543,7 @@ int rtpg_sdtree_import(rtpg_sdtree tree, uint32_t n_stree, const uint32_t* stree
     o.s_sums.upload(reinterpret_cast<const float4*>(s_sums), st, s);
     o.s_child.upload(reinterpret_cast<const uint2*>(s_child), st, s);
     RTPG_HIP(hipStreamSynchronize(s));
-    tree->snodes.swap(sn);
-    tree->upload_stree();
+    tree->upload_stree(std::vector<uint32_t>(stree, stree + n_stree));
     tree->dt = std::move(o);
     tree->publish();
     RTPG_HIP(hipStreamSynchronize(s));

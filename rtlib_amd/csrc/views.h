@@ -124,7 +124,11 @@ struct Camera {
   float eye[3], u[3], v[3], w[3];
 };
 
-// SD-tree, device side (flattened; see sdtree.hip).
+// SD-tree, device side (flattened; see sdtree.hip).  Spatial tree node words:
+// a leaf holds kLeafBit | its DTree, an inner node the index of the first of
+// its two children.
+constexpr uint32_t kLeafBit = 0x80000000u;
+
 struct SdTreeView {
   const uint32_t* stree;   // bit 31 set: leaf, low bits DTree index; else first child
   const uint32_t* jump;    // 2^(3 jump_bits) node words after 3 jump_bits levels (or null)

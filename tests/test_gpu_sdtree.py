@@ -83,6 +83,42 @@ def test_octree_lifecycle_matches_oracle(ctx, orc):
     assert np.array_equal(tree.Lookup(pos), orc.stree_lookup(st["stree"], LO, SIZE, pos))
 
 
+@pytest.mark.parametrize("octree", [False, True])
+def test_refine_many_levels_in_one_reset(ctx, orc, octree):
+    # The first reset splits the root leaf 11 times (binary) or 4 times
+    # (octree) in one refine, the next ones split some thousands of leaves a
+    # few levels each: block offsets, walk order and DTree numbering of the
+    # device refine against the oracle's sequential walk, bit for bit.
+    rng = np.random.default_rng(11 if octree else 12)
+    tree = _tree(ctx)
+    tree.SetSplit(mode=1, c=40.0)
+    tree.SetOctree(octree)
+    st = tree.Download()
+    for it in range(4):
+        if it > 0:
+            tree.Reset(it, 1)
+            st = orc.sdtree_reset(st, it, 1, mode=1, c=40.0, octree=octree)
+            _assert_state_equal(tree.Download(), st)
+        st = _pass(tree, orc, st, rng, 60000 * (it + 1))
+    sz = tree.Sizes()
+    assert sz["dtrees"] > 2000, sz
+    pos = rng.uniform(0, 1, (20000, 3)).astype(np.float32) * np.array([4, 2, 3], np.float32)
+    assert np.array_equal(tree.Lookup(pos), orc.stree_lookup(st["stree"], LO, SIZE, pos))
+
+
+def test_refine_respects_memory_cap(ctx, orc):
+    # max_mb = 0: the footprint test (footprint / 10^6 >= max_mb) always
+    # holds, so the reset keeps the spatial tree and only resets the DTrees.
+    rng = np.random.default_rng(5)
+    tree = _tree(ctx)
+    tree.SetSplit(mode=0, c=40.0, max_mb=0)
+    st = _pass(tree, orc, tree.Download(), rng, 8000)
+    tree.Reset(1, 1)
+    st = orc.sdtree_reset(st, 1, 1, mode=0, c=40.0, max_mb=0)
+    _assert_state_equal(tree.Download(), st)
+    assert st["stree"].tolist() == [0x80000000]
+
+
 def test_lookup_sample_pdf_match_oracle(ctx, orc):
     rng = np.random.default_rng(1)
     tree = _tree(ctx)

@@ -144,6 +144,8 @@ struct rtpg_mlp_t {
   rtpg::DBuf<float> partial, loss_partial, loss;
   rtpg::DBuf<float4> samples;  // recorded by renders: capacity x 4
   rtpg::DBuf<uint32_t> count;
+  rtpg::DBuf<float4> ordered;  // training windows in spatial order
+  rtpg::DBuf<uint32_t> sort_hist, sort_offs, sort_total;
   rtpg::DBuf<float4> staging;
   rtpg::DBuf<float> out;
   uint64_t steps = 0;

@@ -9,6 +9,7 @@ namespace rtpg {
 
 constexpr uint32_t kMlpTrainBlock = 256;   // 4 waves: 64 samples per iteration
 constexpr uint32_t kMlpTrainGrid = 512;    // persistent work-groups of a training step
+constexpr uint32_t kMlpSortCells = 64;     // cells per axis of the training windows' spatial order
 
 // One training sample (four float4): (x, roughness), (n, bsdf pdf at w),
 // (wo, weight = f Li |cos| / p_sample averaged over RGB), (w, 0).
@@ -16,6 +17,11 @@ hipError_t launch_mlp_init(float* w_master, uint16_t* w_bf16, float* table, uint
                            uint64_t seed, hipStream_t s);
 hipError_t launch_mlp_train(const MlpView& m, const float4* samples, uint32_t n, float* partial, float* loss_partial,
                             uint32_t grid, hipStream_t s);
+// Copies n samples to out in spatial order (buckets of the Morton code on a
+// kMlpSortCells^3 grid over the guide's box); hist and offs hold
+// kMlpSortCells^3 words, total one.
+hipError_t launch_mlp_spatial_order(const MlpView& m, const float4* samples, uint32_t n, uint32_t* hist,
+                                    uint32_t* offs, uint32_t* total, float4* out, hipStream_t s);
 hipError_t launch_mlp_reduce(const float* partial, const float* loss_partial, uint32_t grid, float* grad_w,
                              float* loss, hipStream_t s);
 // Adam on n fp32 parameters with gradient g * gscale; writes the bf16 copy

@@ -3,6 +3,7 @@
 // Adam, initialisation and the query kernels used by the C ABI.
 #include "mlp.h"
 #include "mlp_tile.h"
+#include "sdtree.h"
 
 namespace rtpg {
 
@@ -34,10 +35,11 @@ __device__ __forceinline__ void dw_tiles(const uint16_t* sa, const uint16_t* sd,
     int k = wave + 4 * i;
     int t = k / NU, u = k % NU;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      bf16x4_t A = *reinterpret_cast<const bf16x4_t*>(sd + (16 * t + r) * kSP + 16 * ks + 4 * g);
-      bf16x4_t B = *reinterpret_cast<const bf16x4_t*>(sa + (16 * u + r) * kSP + 16 * ks + 4 * g);
-      acc[i] = mfma16(A, B, acc[i]);
+    for (int ks = 0; ks < 4; ks += 2) {
+      const uint16_t* pa = sd + (16 * t + r) * kSP + 16 * ks + 4 * g;
+      const uint16_t* pb = sa + (16 * u + r) * kSP + 16 * ks + 4 * g;
+      acc[i] = mfma32(*reinterpret_cast<const bf16x4_t*>(pa), *reinterpret_cast<const bf16x4_t*>(pa + 16),
+                      *reinterpret_cast<const bf16x4_t*>(pb), *reinterpret_cast<const bf16x4_t*>(pb + 16), acc[i]);
     }
   }
 }
@@ -54,6 +56,39 @@ __device__ __forceinline__ void dw_store(float* dst, const f32x4_t* acc, int wav
   }
 }
 
+// Spatial order of a training window: samples bucketed by the Morton code of
+// their position on a 64^3 grid over the guide's box, so that the 16 samples
+// of a wave row mostly share the cells of the coarse levels and the per-row
+// run sums replace most of their table-gradient atomics.  The order inside a
+// bucket is the atomic arrival order (a step's gradient is a sum).
+__device__ __forceinline__ uint32_t mlp_sort_key(const MlpView& m, float4 s0) {
+  const float n = (float)kMlpSortCells;
+  float q[3] = {(s0.x - m.lo[0]) * m.inv_size * n, (s0.y - m.lo[1]) * m.inv_size * n,
+                (s0.z - m.lo[2]) * m.inv_size * n};
+  uint32_t c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = (uint32_t)fminf(fmaxf(floorf(q[k]), 0.0f), n - 1.0f);
+  return morton3_30(c[0], c[1], c[2]);
+}
+
+__global__ void k_mlp_bucket_count(MlpView m, const float4* smp, uint32_t n, uint32_t* hist) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(hist + mlp_sort_key(m, smp[4u * i]), 1u);
+}
+
+// offs: exclusive prefix sums of the bucket counts (advanced by the scatter).
+__global__ void k_mlp_bucket_scatter(MlpView m, const float4* smp, uint32_t n, uint32_t* offs, float4* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 s0 = smp[4u * i];
+  uint32_t d = atomicAdd(offs + mlp_sort_key(m, s0), 1u);
+  out[4u * d] = s0;
+  out[4u * d + 1u] = smp[4u * i + 1u];
+  out[4u * d + 2u] = smp[4u * i + 2u];
+  out[4u * d + 3u] = smp[4u * i + 3u];
+}
+
 // One training pass over n samples: loss = mean_i -w_i log(c q(w_i) + (1 - c) p_bsdf_i)
 // (the KL divergence of the mixture from the radiance-weighted target up to
 // a constant).  Each work-group writes its weight-gradient sum (unscaled) and
@@ -65,6 +100,14 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
   __shared__ uint16_t sd[kMlpHidden * kSP];
   __shared__ float hs[4][16 * kHeadPitch];
   __shared__ float lred[4];
+  __shared__ uint32_t gkeys[kMlpGradTab];
+  __shared__ float gvals[2 * kMlpGradTab];
+  for (uint32_t i = threadIdx.x; i < kMlpGradTab; i += blockDim.x) {
+    gkeys[i] = kMlpEmptyKey;
+    gvals[2 * i] = 0.0f;
+    gvals[2 * i + 1] = 0.0f;
+  }
+  const MlpGradTable tab{m.grad ? gkeys : nullptr, gvals};
   mlp_load_weights(lw, m.w);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -81,7 +124,10 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
   }
   float lsum = 0.0f;
   float* row = hs[wave] + r * kHeadPitch;
-  for (uint32_t chunk = blockIdx.x; chunk * 64u < n; chunk += gridDim.x) {
+  // a contiguous slice of the (spatially ordered) samples per work-group
+  const uint32_t chunks = (n + 63u) / 64u;
+  const uint32_t c_end = (uint32_t)(((uint64_t)chunks * (blockIdx.x + 1u)) / gridDim.x);
+  for (uint32_t chunk = (uint32_t)(((uint64_t)chunks * blockIdx.x) / gridDim.x); chunk < c_end; ++chunk) {
     uint32_t idx = chunk * 64u + 16u * (uint32_t)wave + (uint32_t)r;
     bool valid = idx < n;
     MlpInput in{V3{0, 0, 0}, V3{0, 0, 1}, V3{0, 0, 1}, 0.0f};
@@ -178,26 +224,42 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
       float px = fminf(fmaxf((in.p.x - m.lo[0]) * m.inv_size, 0.0f), 1.0f);
       float py = fminf(fmaxf((in.p.y - m.lo[1]) * m.inv_size, 0.0f), 1.0f);
       float pz = fminf(fmaxf((in.p.z - m.lo[2]) * m.inv_size, 0.0f), 1.0f);
-      // coarse levels below kMlpPkFirst: neighbouring samples share cells,
-      // so each row of 16 samples sums its runs before the fp32 atomics (the
-      // choice is uniform over a row: one level per row); finer levels send
-      // packed bf16 pairs directly
+      // levels below 1024 cells per axis: neighbouring samples (the window
+      // is in spatial order) share cells, so each row of 16 samples sums its
+      // runs before the atomics (fp32 below kMlpPkFirst, packed bf16 pairs
+      // from there; the choice is uniform over a row: one level per row);
+      // finer levels send every sample's contributions directly
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        int l = 2 * g + q;
-        if (l >= kMlpPkFirst) {
-          if (valid) mlp_encode_level_grad_pk(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1]);
-        } else if (m.res[l] < 1024.0f) {
-          mlp_encode_level_grad_runs(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1], valid);
-        } else if (valid) {
-          mlp_encode_level_grad(m, l, px, py, pz, dh[0][2 * q], dh[0][2 * q + 1]);
+      for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int l = 8 * b + 2 * g + q;
+          const float g0 = dh[b][2 * q], g1 = dh[b][2 * q + 1];
+          if (m.res[l] < 1024.0f) {
+            if (l >= kMlpPkFirst) {
+              mlp_encode_level_grad_runs<true>(m, l, px, py, pz, g0, g1, valid);
+            } else {
+              mlp_encode_level_grad_runs<false>(m, l, px, py, pz, g0, g1, valid, tab);
+            }
+          } else if (valid) {
+            if (l >= kMlpPkFirst) {
+              mlp_encode_level_grad_pk(m, l, px, py, pz, g0, g1);
+            } else {
+              mlp_encode_level_grad(m, l, px, py, pz, g0, g1);
+            }
+          }
         }
       }
-      if (valid) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-          mlp_encode_level_grad_pk(m, 8 + 2 * g + q, px, py, pz, dh[1][2 * q], dh[1][2 * q + 1]);
-      }
+    }
+  }
+  if (m.grad) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kMlpGradTab; i += blockDim.x) {
+      const uint32_t key = gkeys[i];
+      if (key == kMlpEmptyKey) continue;
+      float* gt = m.grad + (size_t)key * 2u;
+      if (gvals[2 * i] != 0.0f) unsafeAtomicAdd(gt, gvals[2 * i]);
+      if (gvals[2 * i + 1] != 0.0f) unsafeAtomicAdd(gt + 1, gvals[2 * i + 1]);
     }
   }
   float* dst = partial + (size_t)blockIdx.x * kMlpWeights;
@@ -430,6 +492,19 @@ hipError_t launch_mlp_init(float* w_master, uint16_t* w_bf16, float* table, uint
 hipError_t launch_mlp_train(const MlpView& m, const float4* samples, uint32_t n, float* partial, float* loss_partial,
                             uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(kMlpTrainBlock), 0, s, m, samples, n, partial, loss_partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_spatial_order(const MlpView& m, const float4* samples, uint32_t n, uint32_t* hist,
+                                    uint32_t* offs, uint32_t* total, float4* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t buckets = kMlpSortCells * kMlpSortCells * kMlpSortCells;
+  hipError_t e = hipMemsetAsync(hist, 0, buckets * sizeof(uint32_t), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mlp_bucket_count, dim3(nblk(n, 256)), dim3(256), 0, s, m, samples, n, hist);
+  e = launch_exclusive_scan(hist, offs, buckets, total, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mlp_bucket_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, m, samples, n, offs, out);
   return hipGetLastError();
 }
 

@@ -175,15 +175,45 @@ __device__ __forceinline__ void mlp_encode_level_grad_pk(const MlpView& m, int l
   }
 }
 
-// The same scatter for coarse levels, where neighbouring samples often share a
-// grid cell.  Each row of 16 lanes holds 16 samples at one level (lane r of
-// the row: sample r; every lane of the row takes part, valid = false lanes
+// Work-group table of gradient sums in LDS: key level x entries + entry (the
+// entry's pair in the fp32 gradient), two fp32 features per slot, 16 linear
+// probes; the training kernel flushes it to the fp32 gradient at its end.  A work-group's samples are a
+// contiguous slice of a spatially ordered window, so they touch few coarse
+// entries; a full table makes the caller add to global memory directly.
+constexpr uint32_t kMlpGradTab = 1024;
+constexpr uint32_t kMlpEmptyKey = 0xFFFFFFFFu;
+struct MlpGradTable {
+  uint32_t* keys;  // kMlpGradTab, or NULL: no table
+  float* vals;     // 2 x kMlpGradTab
+};
+
+__device__ __forceinline__ bool mlp_tab_add(const MlpGradTable& t, uint32_t key, float a, float b) {
+  uint32_t h = (key * 0x9E3779B1u) & (kMlpGradTab - 1u);
+  for (int probe = 0; probe < 16; ++probe) {
+    uint32_t prev = atomicCAS(&t.keys[h], kMlpEmptyKey, key);
+    if (prev == kMlpEmptyKey || prev == key) {
+      if (a != 0.0f) atomicAdd(&t.vals[2u * h], a);
+      if (b != 0.0f) atomicAdd(&t.vals[2u * h + 1u], b);
+      return true;
+    }
+    h = (h + 1u) & (kMlpGradTab - 1u);
+  }
+  return false;
+}
+
+// The same scatter for levels where neighbouring samples often share a grid
+// cell.  Each row of 16 lanes holds 16 samples at one level (lane r of the
+// row: sample r; every lane of the row takes part, valid = false lanes
 // contribute nothing).  Runs of consecutive lanes in the same cell sum their
 // 8 corner x 2 feature contributions with a segmented suffix scan and the
-// first lane of a run issues one atomic per corner and feature for the whole
+// first lane of a run issues one atomic per corner (kPacked: one packed bf16
+// pair into grad_pk, for levels >= kMlpPkFirst; else one fp32 add per
+// feature, into the work-group table tab when one is given) for the whole
 // run.  Needs res[l] < 1024 (the cell key packs 10 bits per axis).
+template <bool kPacked>
 __device__ __forceinline__ void mlp_encode_level_grad_runs(const MlpView& m, int l, float px, float py, float pz,
-                                                           float g0, float g1, bool valid) {
+                                                           float g0, float g1, bool valid,
+                                                           const MlpGradTable& tab = MlpGradTable{nullptr, nullptr}) {
   MlpCell c = mlp_cell(m, l, px, py, pz);
   const uint32_t lane = __lane_id(), r = lane & 15u;
   const uint32_t key = valid ? (c.x | (c.y << 10) | (c.z << 20)) : 0xFFFFFFFFu;
@@ -213,13 +243,28 @@ __device__ __forceinline__ void mlp_encode_level_grad_runs(const MlpView& m, int
     }
   }
   if (valid && (r == 0u || kp != key)) {
-    float* gt = m.grad + (size_t)l * (m.tmask + 1u) * 2u;
+    if (kPacked) {
+      bf16x2_v* gt = reinterpret_cast<bf16x2_v*>(m.grad_pk + (size_t)(l - kMlpPkFirst) * (m.tmask + 1u));
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
-      uint32_t i = mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz);
-      if (v[2 * k] != 0.0f) unsafeAtomicAdd(gt + 2u * i, v[2 * k]);
-      if (v[2 * k + 1] != 0.0f) unsafeAtomicAdd(gt + 2u * i + 1u, v[2 * k + 1]);
+      for (int k = 0; k < 8; ++k) {
+        uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+        uint32_t i = mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz);
+        if (v[2 * k] == 0.0f && v[2 * k + 1] == 0.0f) continue;
+        if (tab.keys && mlp_tab_add(tab, (uint32_t)l * (m.tmask + 1u) + i, v[2 * k], v[2 * k + 1])) continue;
+        bf16x2_v pv = {(__bf16)v[2 * k], (__bf16)v[2 * k + 1]};
+        __builtin_amdgcn_global_atomic_fadd_v2bf16(gt + i, pv);
+      }
+    } else {
+      float* gt = m.grad + (size_t)l * (m.tmask + 1u) * 2u;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+        uint32_t i = mlp_index(m, l, c.x + dx, c.y + dy, c.z + dz);
+        if (v[2 * k] == 0.0f && v[2 * k + 1] == 0.0f) continue;
+        if (tab.keys && mlp_tab_add(tab, (uint32_t)l * (m.tmask + 1u) + i, v[2 * k], v[2 * k + 1])) continue;
+        if (v[2 * k] != 0.0f) unsafeAtomicAdd(gt + 2u * i, v[2 * k]);
+        if (v[2 * k + 1] != 0.0f) unsafeAtomicAdd(gt + 2u * i + 1u, v[2 * k + 1]);
+      }
     }
   }
 }

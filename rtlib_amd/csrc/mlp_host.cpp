@@ -339,10 +339,22 @@ int rtpg_mlp_train_recorded(rtpg_mlp m, uint32_t steps, float* loss, uint32_t* u
     }
     uint32_t b = std::min(n, m->cfg.batch);
     uint32_t windows = n / b;  // whole batches in recording order
+    // each window the steps use, copied in spatial order (the gradient
+    // scatter sums runs of samples that share a grid cell)
+    const uint32_t used_windows = std::min(windows, std::max(steps, 1u));
+    const uint32_t buckets = kMlpSortCells * kMlpSortCells * kMlpSortCells;
+    m->ordered.reserve(4 * (size_t)b * used_windows);
+    m->sort_hist.reserve(buckets);
+    m->sort_offs.reserve(buckets);
+    m->sort_total.reserve(1);
+    const MlpView v = mlp_view_of(m, false);
+    for (uint32_t w = 0; w < used_windows; ++w)
+      RTPG_HIP(launch_mlp_spatial_order(v, m->samples.p + 4 * (size_t)w * b, b, m->sort_hist.p, m->sort_offs.p,
+                                        m->sort_total.p, m->ordered.p + 4 * (size_t)w * b, s));
     // every step's loss sum stays on the device until the last step
     m->loss.reserve(std::max(steps, 1u));
     for (uint32_t k = 0; k < steps; ++k) {
-      const float4* smp = m->samples.p + 4 * (size_t)((k % windows) * b);
+      const float4* smp = m->ordered.p + 4 * (size_t)((k % used_windows) * b);
       grad_pass(m, smp, b, k);
       adam_step(m, b);
     }

@@ -1,18 +1,26 @@
-// Wave-level MLP tiles on v_mfma_f32_16x16x16_bf16 (gfx950 matrix cores).
+// Wave-level MLP tiles on the gfx950 matrix cores: v_mfma_f32_16x16x32_bf16
+// for pairs of 16-wide k blocks, v_mfma_f32_16x16x16_bf16 for a lone block.
 //
-// Operand layout of the instruction, lane l = 16 g + r (r < 16, g < 4):
+// Operand layout of the 16-wide form, lane l = 16 g + r (r < 16, g < 4):
 //   A (16 x 16, rows x k): lane holds A[r][4g + j], j < 4
 //   B (16 x 16, k x cols): lane holds B[4g + j][r]
 //   D (16 x 16):           lane holds D[4g + j][r]
-// With weights as A and activations as B (columns = 16 samples), the D tile of
-// one layer holds, per lane, sample r and features 4g + j of a 16-feature
-// block: the B operand of the next layer.  Device code only.
+// The 32-wide form takes 8 k values per lane on A and B.  A lane gives it
+// its 4 values of block 2b followed by its 4 of block 2b + 1, on A and B
+// alike: the sum over k is the same whichever k a lane holds as long as its A
+// and B values pair up, so two blocks fuse into one instruction without
+// moving data.  With weights as A and activations as B (columns = 16
+// samples), the D tile of one layer holds, per lane, sample r and features
+// 4g + j of a 16-feature block: the B operand of the next layer.  Device code
+// only.
 #pragma once
 #include "mlp_dev.h"
 
 namespace rtpg {
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_v __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 // LDS copy of the weights: row pitches padded by 8 bf16 against bank conflicts.
@@ -27,6 +35,14 @@ constexpr int kHeadPitch = 49;                           // floats per sample ro
 
 __device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Two 16-wide k blocks in one instruction: A = (a0 | a1), B = (b0 | b1).
+__device__ __forceinline__ f32x4_t mfma32(bf16x4_t a0, bf16x4_t a1, bf16x4_t b0, bf16x4_t b1, f32x4_t c) {
+  const bf16x8_t a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  const bf16x8_t b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_v, a), __builtin_bit_cast(bf16x8_v, b), c,
+                                                 0, 0, 0);
 }
 
 // Copies the row-major bf16 weights into the padded LDS layout (whole block).
@@ -71,11 +87,14 @@ __device__ __forceinline__ void dense_fwd(const uint16_t* W, int pitch, const bf
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t* row = W + (16 * t + r) * pitch + 4 * g;
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) {
-      bf16x4_t a = *reinterpret_cast<const bf16x4_t*>(W + (16 * t + r) * pitch + 16 * kb + 4 * g);
-      acc = mfma16(a, x[kb], acc);
+    for (int kb = 0; kb + 1 < NKB; kb += 2) {
+      bf16x4_t a0 = *reinterpret_cast<const bf16x4_t*>(row + 16 * kb);
+      bf16x4_t a1 = *reinterpret_cast<const bf16x4_t*>(row + 16 * kb + 16);
+      acc = mfma32(a0, a1, x[kb], x[kb + 1], acc);
     }
+    if (NKB & 1) acc = mfma16(*reinterpret_cast<const bf16x4_t*>(row + 16 * (NKB - 1)), x[NKB - 1], acc);
     y[t] = acc;
   }
 }
@@ -85,19 +104,21 @@ __device__ __forceinline__ void dense_fwd(const uint16_t* W, int pitch, const bf
 template <int NU, int NKB>
 __device__ __forceinline__ void dense_bwd(const uint16_t* W, int pitch, const bf16x4_t* dz, f32x4_t* dh, int lane) {
   const int r = lane & 15, g = lane >> 4;
+  auto col = [&](int kb, int u) {
+    const uint16_t* c = W + (16 * kb + 4 * g) * pitch + 16 * u + r;
+    bf16x4_t a;
+    a[0] = (short)c[0];
+    a[1] = (short)c[pitch];
+    a[2] = (short)c[2 * pitch];
+    a[3] = (short)c[3 * pitch];
+    return a;
+  };
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
     f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) {
-      const uint16_t* c = W + (16 * kb + 4 * g) * pitch + 16 * u + r;
-      bf16x4_t a;
-      a[0] = (short)c[0];
-      a[1] = (short)c[pitch];
-      a[2] = (short)c[2 * pitch];
-      a[3] = (short)c[3 * pitch];
-      acc = mfma16(a, dz[kb], acc);
-    }
+    for (int kb = 0; kb + 1 < NKB; kb += 2) acc = mfma32(col(kb, u), col(kb + 1, u), dz[kb], dz[kb + 1], acc);
+    if (NKB & 1) acc = mfma16(col(NKB - 1, u), dz[NKB - 1], acc);
     dh[u] = acc;
   }
 }

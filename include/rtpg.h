@@ -46,6 +46,7 @@ extern "C" {
 #define RTPG_FLAG_MIS 128u
 #define RTPG_FLAG_RR 256u
 #define RTPG_FLAG_USE_MLP 512u
+#define RTPG_FLAG_MOMENTS 1024u /* per-pixel moments of this render's samples (rtpg_render_relvar) */
 
 /* Material types (per triangle). */
 #define RTPG_MAT_LAMBERT 0u
@@ -152,7 +153,8 @@ typedef struct rtpg_render_params {
   uint64_t seed;
   float bg[3];             /* background radiance (reference MissData.BgColor) */
   float fraction;          /* guide-vs-BSDF mixture (reference SdTree.Fraction) */
-  rtpg_sdtree sdtree;      /* guide for RTPG_FLAG_USE_TREE, else NULL */
+  rtpg_sdtree sdtree;      /* guide for RTPG_FLAG_USE_TREE; with RTPG_FLAG_USE_MLP (and not FINAL) the
+                              learned guide's training vertices also record into it; else NULL */
   rtpg_hashgrid hashgrid;  /* spatial hash for RTPG_FLAG_USE_GRID */
   rtpg_mqtree mqtree;      /* direction quadtrees for RTPG_FLAG_USE_GRID */
   uint32_t max_paths;      /* paths in flight per chunk (0 = default 32M; at most 64M) */
@@ -175,6 +177,15 @@ int rtpg_render(rtpg_context ctx, const rtpg_render_params* params);
  * accumulation-buffer order. */
 int rtpg_render_debug(rtpg_context ctx, const rtpg_render_params* params, uint32_t frame_type, float* value,
                       uint8_t* rgba);
+/* Per-sample relative variance of the last render made with RTPG_FLAG_MOMENTS
+ * (at least 2 samples per pixel): the mean over pixels of s^2 / (m^2 + 0.01),
+ * m and s^2 the mean and unbiased variance of the pixel's samples of
+ * y = (r + g + b) / 3.  Times the render's cost per sample it ranks
+ * estimators by efficiency: the learned guide's fallback decision
+ * (rtlib_amd.tracer.render_mlp_guided) compares it between plain, SD-tree-
+ * and learned-guide renders.  No reference counterpart (the reference judges
+ * methods offline with RTLibExtOPX7Comp.cpp). */
+int rtpg_render_relvar(rtpg_context ctx, double* relvar);
 /* Last render's counters: [0] paths, [1] radiance rays, [2] shadow rays. */
 int rtpg_render_stats(rtpg_context ctx, uint64_t stats[3]);
 /* Profiling of later renders (no reference counterpart; the reference only

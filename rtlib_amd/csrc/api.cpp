@@ -609,6 +609,8 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     MortonView mq{};
     MlpView ml{};
     if (guide == 1) sd = sdtree_view_of(prm->sdtree);
+    // a learned-guide render that records also records into params->sdtree
+    if (guide == 3 && prm->sdtree && !(prm->flags & RTPG_FLAG_FINAL)) sd = sdtree_view_of(prm->sdtree);
     if (guide == 2) guide_views(prm, ctx, hg, mq);
     if (guide == 3) {
       if (prm->flags & RTPG_FLAG_GUIDE_SAMPLE) mlp_cache_refresh(prm->mlp);
@@ -618,6 +620,12 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     // the stack-bound flag reports this call only
     ctx->p_err.zero(1, s);
     ctx->p_stats.zero(4, s);
+    const bool moments = (prm->flags & RTPG_FLAG_MOMENTS) != 0;
+    if (moments) {
+      ctx->moments.reserve(2 * (size_t)npix);
+      ctx->moments.zero(2 * (size_t)npix, s);
+      ctx->moments_spp = prm->spp;
+    }
     const bool timing = (ctx->prof_mode & 1u) != 0;
     const bool counting = (ctx->prof_mode & 2u) != 0;
     // events of up to kEvChunks chunks are read back together, so timing adds
@@ -661,6 +669,7 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
       PathSoA ps = make_soa(ctx, n, prm->max_depth, guide != 0, counting, guide == 3 ? prm->mlp : nullptr);
       RenderParams rp = make_rp(ctx, prm, cs, done);
       if (guide == 3) ps.mlp_phase = rp.sample_offset % ps.mlp_stride;
+      ps.moments = moments ? ctx->moments.p : nullptr;
       if (timing && ev_pending == kEvChunks) drain_events();
       RTPG_HIP(launch_render_chunk(rp, ctx->cam, sc, ps, sd, hg, mq, ml, guide, ctx->accum.p, s,
                                    timing ? ctx->events.data() + (ev_pending++) * ev_stride : nullptr));
@@ -687,6 +696,25 @@ extern "C" int rtpg_render(rtpg_context ctx, const rtpg_render_params* prm) {
     ctx->last_stats[0] = st[0];
     ctx->last_stats[1] = st[1];
     ctx->last_stats[2] = st[2];
+  });
+}
+
+extern "C" int rtpg_render_relvar(rtpg_context ctx, double* relvar) {
+  return guarded([&] {
+    require(ctx && relvar, "NULL argument");
+    require(ctx->moments.p != nullptr && ctx->moments_spp >= 2,
+            "needs a render with RTPG_FLAG_MOMENTS and at least 2 samples per pixel");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    const size_t npix = (size_t)ctx->width * ctx->height;
+    std::vector<float> m = ctx->moments.to_vector(2 * npix, ctx->stream);
+    const double n = (double)ctx->moments_spp;
+    double acc = 0.0;
+    for (size_t i = 0; i < npix; ++i) {
+      const double mean = m[2 * i] / n;
+      const double var = std::max(0.0, ((double)m[2 * i + 1] - m[2 * i] * mean) / (n - 1.0));
+      acc += var / (mean * mean + 1e-2);
+    }
+    *relvar = acc / (double)npix;
   });
 }
 

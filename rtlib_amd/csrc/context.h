@@ -48,6 +48,8 @@ struct rtpg_context_t {
   uint32_t width = 0, height = 0;
   rtpg::DBuf<float> accum;
   uint32_t accum_spp = 0;
+  rtpg::DBuf<float> moments;  // per pixel (sum, sum of squares) of the last RTPG_FLAG_MOMENTS render
+  uint32_t moments_spp = 0;
   rtpg::Camera cam{};
   // path arena
   rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC;

@@ -36,6 +36,7 @@ struct PathSoA {
   uint32_t* count;   // live paths per depth [max_depth + 2]
   uint32_t* scount;  // shadow rays per depth [max_depth + 2]
   float4* shadow;    // 3 x float4 per shadow ray
+  float* moments;    // per pixel: sum and sum of squares of the samples' RGB mean, or NULL
   uint32_t* deep;    // traversal spill stack: (entry, distance) pairs, 2 x (cap - kShortStack) x n words
   uint32_t* err;     // sticky error bits
   // guide vertices, [depth][pid]

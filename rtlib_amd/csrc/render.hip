@@ -702,6 +702,10 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
             gcell = grid_find_prv(hg, bucket, csum);
             vcell = gcell;
             if ((rp.flags & kFlagLocate) && gcell == kInvalidCell) vcell = grid_find_cur(hg, bucket, csum);
+          } else if (kGuide == 3 && record && sd.stree && records_path(ps, pid)) {
+            // the learned guide's training vertices also record into an
+            // SD-tree (its fallback), which does not guide this render
+            vcell = stree_lookup(sd, p);
           }
           // learned guide: the mixture the inference pass wrote for this queue slot
           const float4* mx = nullptr;
@@ -822,6 +826,7 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
             float gp;
             if (kGuide == 3) {
               gp = mix_pdf(mx, dir);
+              if (vcell != kInvalidCell) dir_to_canonical(dir, vcu, vcv);
             } else {
               dir_to_canonical(dir, vcu, vcv);
               if (kGuide == 1) {
@@ -874,9 +879,9 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
       size_t vi = (size_t)depth * ps.n + pid;
       ps.vB[vi] = make_float4(c.x, c.y, c.z, cos_out);
       ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
-      if (kGuide != 3) {
+      if (kGuide != 3 || sd.stree) {
         bool has_vertex = cont && vcell != kInvalidCell;
-        if (kGuide == 1 && sd.filter && has_vertex) {
+        if ((kGuide == 1 || kGuide == 3) && sd.filter && has_vertex) {
           // stochastic box filters: the record moves to a random point of the
           // reference's boxes (own stream); u < 0 marks a dropped flux
           uint32_t npix = rp.width * rp.height;
@@ -891,7 +896,8 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
         // integer store: the all-ones sentinel is a NaN pattern as a float
         reinterpret_cast<uint4*>(ps.vC)[vi] = make_uint4(fbits(vcu), fbits(vcv), has_vertex ? vcell : kInvalidCell,
                                                          vflags | (has_vertex ? kVtxValid : 0u));
-      } else if (!cont || (vflags & kVtxDelta)) {
+      }
+      if (kGuide == 3 && (!cont || (vflags & kVtxDelta))) {
         // no training vertex here (light, miss, delta or absorbed path)
         size_t st = (size_t)rp.max_depth * ps.mlp_rec;
         ps.mv[2 * st + (size_t)depth * ps.mlp_rec + pid / ps.mlp_stride] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
@@ -979,16 +985,23 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 __global__ void k_accum(PathSoA ps, uint32_t npix, uint32_t spp, float* accum) {
   uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= npix) return;
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, m1 = 0.0f, m2 = 0.0f;
   for (uint32_t s = 0; s < spp; ++s) {
     float4 l = ps.L[(size_t)s * npix + pix];
     sx += l.x;
     sy += l.y;
     sz += l.z;
+    float y = (l.x + l.y + l.z) / 3.0f;
+    m1 += y;
+    m2 += y * y;
   }
   accum[3 * pix + 0] += sx;
   accum[3 * pix + 1] += sy;
   accum[3 * pix + 2] += sz;
+  if (ps.moments) {
+    ps.moments[2 * pix] += m1;
+    ps.moments[2 * pix + 1] += m2;
+  }
 }
 
 // Display frame of the accumulated image (SimpleTrace.cu:91-101): x = sum /
@@ -1275,7 +1288,7 @@ __global__ void __launch_bounds__(kCommitBlock) k_commit(PathSoA ps, uint32_t ma
     // consecutive work-items take consecutive paths, so the vertex loads of a
     // wave at one depth are contiguous
     const uint32_t pid = first + k * kCommitBlock + threadIdx.x;
-    const bool in = pid < ps.n;
+    const bool in = pid < ps.n && records_path(ps, pid);
     uint32_t nh = in ? ps.nhit[pid] : 0u;
     nh = nh < (uint32_t)kMaxCommitVtx ? nh : (uint32_t)kMaxCommitVtx;
     uint32_t nv = nh > 0 ? nh - 1u : 0u;
@@ -1548,9 +1561,9 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
   hipLaunchKernelGGL(k_accum, dim3(blocks_for(npix, 256)), dim3(256), 0, s, ps, npix, rp.spp, accum);
   mark(3, 0, 1);
   mark(4, 0, 0);
-  if (record && (guide == 1 || guide == 2)) {
+  if (record && (guide == 1 || guide == 2 || (guide == 3 && sd.stree))) {
     const uint32_t nc = blocks_for(ps.n, kCommitBlock * kCommitPaths);
-    if (guide == 1) {
+    if (guide != 2) {
       hipLaunchKernelGGL(k_commit<1>, dim3(nc), dim3(kCommitBlock), 0, s, ps, rp.max_depth - 1u, sd, mq);
     } else {
       hipLaunchKernelGGL(k_commit<2>, dim3(nc), dim3(kCommitBlock), 0, s, ps, rp.max_depth - 1u, sd, mq);

@@ -118,6 +118,7 @@ enum RenderFlag : uint32_t {
   kFlagMis = 128u,
   kFlagRR = 256u,
   kFlagUseMlp = 512u,
+  kFlagMoments = 1024u,  // per-pixel sample moments (no reference counterpart)
 };
 
 struct Camera {

@@ -11,6 +11,7 @@ All work runs through librtpg.so on the GPU; there is no host fallback.
 from __future__ import annotations
 
 import ctypes as C
+import time
 import weakref
 
 import numpy as np
@@ -29,6 +30,7 @@ FLAG_LOCATE = 64
 FLAG_MIS = 128
 FLAG_RR = 256
 FLAG_USE_MLP = 512
+FLAG_MOMENTS = 1024  # per-pixel sample moments (Context.relvar)
 TEX_NONE = 0xFFFFFFFF
 TEX_CLAMP = 0
 TEX_WRAP = 1
@@ -197,6 +199,13 @@ class Context:
         rgba = np.zeros((self.height, self.width, 4), np.uint8)
         call("rtpg_render_debug", self.h, C.byref(p), frame_type, ptr(val), ptr(rgba, C.c_uint8))
         return val[..., :3].copy(), val[..., 3].view(np.uint32).copy(), rgba
+
+    def relvar(self):
+        """Per-sample relative variance of the last FLAG_MOMENTS render (>= 2
+        spp): mean over pixels of s^2 / (m^2 + 0.01) of y = (r + g + b) / 3."""
+        v = C.c_double()
+        call("rtpg_render_relvar", self.h, C.byref(v))
+        return float(v.value)
 
     def render_stats(self):
         s = np.zeros(3, np.uint64)
@@ -723,31 +732,105 @@ class MLPGuide:
 
 
 def render_mlp_guided(ctx: Context, mlp: MLPGuide, budget, base_flags=FLAG_NEE | FLAG_MIS | FLAG_RR, max_depth=8,
-                      rr_depth=3, seed=0, train_passes=8, pass_spp=1, steps_per_pass=4, max_paths=0, on_pass=None):
+                      rr_depth=3, seed=0, train_passes=8, pass_spp=1, steps_per_pass=4, max_paths=0, on_pass=None,
+                      fallback=None, fraction=0.3, judge=None):
     """Online schedule: train_passes passes of pass_spp samples record training
     samples (guided by the network once it has been trained at least once),
     each followed by steps_per_pass Adam steps; the rest of the budget is one
     final guided pass.  Every pass accumulates into the image.  Returns the
-    number of samples traced."""
+    number of samples traced.
+
+    fallback: an RTSTreeWrapper that the training passes record into as well
+    (reset with the reference's refinement rule before and built after every
+    pass), so that an SD-tree trained on the same paths is at hand.
+
+    judge: None, or a dict that enables the fallback decision and receives
+    its report.  Options: probe_spp (2), max_train_share (0.25).  Plain path
+    tracing renders a first probe pass; training stops (and plain path tracing
+    renders the rest) when the training passes would take more than
+    max_train_share of the budget's time at the plain rate.  After training,
+    probe passes of the learned guide and of the fallback SD-tree measure
+    their per-sample relative variance (Context.relvar) and cost, and the rest
+    of the budget goes to the estimator with the lowest variance x time (the
+    plain probe's included).  Probe passes count towards the budget and the
+    image; every pass is an unbiased estimate, so the mixture is too."""
     traced = 0
     trained = False
+    probe = int(judge.get("probe_spp", 2)) if judge is not None else 0
+    cands = {}
+    if fallback is not None:
+        fallback.Clear()
+
+    def timed_render(spp, flags, **kw):
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.render(spp=spp, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed, sample_offset=traced,
+                   max_paths=max_paths, **kw)
+        ctx.synchronize()
+        return time.perf_counter() - t
+
+    if judge is not None and budget >= probe + 1:
+        t = timed_render(probe, base_flags | FLAG_MOMENTS)
+        cands["plain"] = (ctx.relvar(), t / probe)
+        traced += probe
+    stop_training = False
     for k in range(train_passes):
-        if traced + pass_spp > budget:
+        if stop_training or traced + pass_spp + probe > budget:
             break
+        t0 = time.perf_counter()
+        if fallback is not None and k > 0:
+            fallback.Reset(k, pass_spp)
         flags = base_flags | FLAG_USE_MLP | (FLAG_GUIDE_SAMPLE if trained else 0)
         ctx.render(spp=pass_spp, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed,
-                   sample_offset=traced, mlp=mlp, max_paths=max_paths)
+                   sample_offset=traced, mlp=mlp, max_paths=max_paths, sdtree=fallback)
         traced += pass_spp
         loss, used = mlp.train_recorded(steps_per_pass)
+        if fallback is not None:
+            fallback.Build()
         trained = trained or used > 0
         if on_pass is not None:
             on_pass(k, pass_spp, traced, loss, used)
+        if judge is not None and k == 0 and "plain" in cands:
+            ctx.synchronize()
+            t_pass = time.perf_counter() - t0
+            share = float(judge.get("max_train_share", 0.25))
+            if t_pass * train_passes > share * cands["plain"][1] * budget:
+                stop_training = True
     rest = budget - traced
+    choice = "learned"
+    if judge is not None:
+        if stop_training or not trained:
+            choice = "plain"
+        else:
+            if rest >= 2 * probe:
+                t = timed_render(probe, base_flags | FLAG_USE_MLP | FLAG_FINAL | FLAG_GUIDE_SAMPLE | FLAG_MOMENTS,
+                                 mlp=mlp)
+                cands["learned"] = (ctx.relvar(), t / probe)
+                traced += probe
+            if fallback is not None and budget - traced >= 2 * probe:
+                t = timed_render(probe, base_flags | FLAG_USE_TREE | FLAG_FINAL | FLAG_GUIDE_SAMPLE | FLAG_MOMENTS,
+                                 sdtree=fallback, fraction=fraction)
+                cands["sdtree"] = (ctx.relvar(), t / probe)
+                traced += probe
+            choice = min(cands, key=lambda c: cands[c][0] * cands[c][1]) if cands else "learned"
+        judge["candidates"] = {c: {"relvar": v, "ms_per_spp": 1e3 * t} for c, (v, t) in cands.items()}
+        judge["choice"] = choice
+        judge["training_stopped"] = stop_training
+        rest = budget - traced
     if rest > 0:
-        flags = base_flags | FLAG_USE_MLP | FLAG_FINAL | (FLAG_GUIDE_SAMPLE if trained else 0)
+        if choice == "plain":
+            flags, kw = base_flags, {}
+        elif choice == "sdtree":
+            flags, kw = base_flags | FLAG_USE_TREE | FLAG_FINAL | FLAG_GUIDE_SAMPLE, {"sdtree": fallback,
+                                                                                      "fraction": fraction}
+        else:
+            flags = base_flags | FLAG_USE_MLP | FLAG_FINAL | (FLAG_GUIDE_SAMPLE if trained else 0)
+            kw = {"mlp": mlp}
         ctx.render(spp=rest, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed, sample_offset=traced,
-                   mlp=mlp, max_paths=max_paths)
+                   max_paths=max_paths, **kw)
         traced += rest
         if on_pass is not None:
             on_pass(-1, rest, traced, None, 0)
     return traced
+
+

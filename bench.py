@@ -333,6 +333,22 @@ def quality_legs(a, ctx, sc, tree, lo, hi, bounds, guided_step, timed, plain_rat
         return render_mlp_guided(ctx, g, budget or a.budget, base, max_depth=depth, seed=seed,
                                  train_passes=a.mlp_passes, steps_per_pass=a.mlp_steps)
 
+    # the learned guide with its fallback SD-tree (recorded in the same
+    # passes) and the judge that picks the estimator for the rest of the budget
+    fb_tree = RTSTreeWrapper(ctx, lo, hi, 20, frac)
+    fb_tree.SetSplit(mode=1, c=12000.0, max_mb=2000, subdiv_th=0.01)
+    fb_tree.SetFilter(spatial=a.spatial_filter, directional=a.directional_filter, seed=11)
+    choices = []
+
+    def mlp_judge_step(seed):
+        j = {}
+        mlp.reset(1)
+        ctx.clear()
+        n = render_mlp_guided(ctx, mlp, a.budget, base, max_depth=depth, seed=seed, train_passes=a.mlp_passes,
+                              steps_per_pass=a.mlp_steps, fallback=fb_tree, fraction=frac, judge=j)
+        choices.append(j["choice"])
+        return n
+
     def leg(name, fn, seeds, plain_flags=base, rate=plain_rate, seed0=1000):
         fn(seed0 - 1)  # warm
         rows = []
@@ -364,7 +380,12 @@ def quality_legs(a, ctx, sc, tree, lo, hi, bounds, guided_step, timed, plain_rat
     out["hashgrid_ris"] = leg("hash grid + RIS", lambda s: grid_step(s, flags=ris), a.seeds, plain_flags=ris,
                               rate=plain_ris_rate, seed0=4000)
     out["mlp"] = leg("learned guide (spatial cache)", lambda s: mlp_step(s), a.seeds, seed0=5000)
-    out["mlp"].update({"train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps, "cache_res": a.mlp_cache})
+    info = mlp.info()
+    out["mlp"].update({"train_passes": a.mlp_passes, "steps_per_pass": a.mlp_steps, "cache_res": a.mlp_cache,
+                       "samples_per_step": 1 << 18, "max_res": float(info["res"][-1])})
+    out["mlp_judge"] = leg("learned guide, judge + SD-tree fallback", mlp_judge_step, a.seeds, seed0=5500)
+    out["mlp_judge"]["choices"] = choices[1:]
+    fb_tree.Destroy()
     pv = MLPGuide(ctx, lo, hi, seed=1)
     out["mlp_per_vertex_inference"] = leg("learned guide (per-vertex inference)", lambda s: mlp_step(s, g=pv), 1,
                                           seed0=6000)
@@ -488,6 +509,21 @@ def other_configs(ctx, timed, frac):
     timed(mlp_step)
     t_mlp, _ = timed(mlp_step)
     img_mlp = ctx.image()
+    fb = RTSTreeWrapper(ctx, lo, hi, 20, frac)
+    fb.SetSplit(mode=1, c=12000.0)
+    judge = {}
+
+    def mlp_judge_step():
+        judge.clear()
+        mlp.reset(1)
+        ctx.clear()
+        return render_mlp_guided(ctx, mlp, 64, nee_mis_rr, max_depth=8, seed=25, train_passes=8, steps_per_pass=4,
+                                 fallback=fb, fraction=frac, judge=judge)
+
+    timed(mlp_judge_step)
+    t_judge, _ = timed(mlp_judge_step)
+    img_judge = ctx.image()
+    fb.Destroy()
     mlp.close()
     tree = RTSTreeWrapper(ctx, lo, hi, 20, frac)
     tree.SetSplit(mode=1, c=12000.0)
@@ -502,7 +538,7 @@ def other_configs(ctx, timed, frac):
     t_tree, _ = timed(tree_step)
     img_tree = ctx.image()
     res = {}
-    for name, tt, img in (("mlp", t_mlp, img_mlp), ("sdtree", t_tree, img_tree)):
+    for name, tt, img in (("mlp", t_mlp, img_mlp), ("mlp_judge", t_judge, img_judge), ("sdtree", t_tree, img_tree)):
         spp = max(1, int(round(tt * plain_rate / (256 * 256))))
         ctx.clear()
         ctx.render(spp=spp, flags=nee_mis_rr, max_depth=8, seed=23)
@@ -511,6 +547,7 @@ def other_configs(ctx, timed, frac):
     ctx.clear()
     ctx.render(spp=64, flags=nee_mis_rr, max_depth=8, seed=24)
     res["plain_at_budget"] = {"spp": 64, "ms": round(t * 1e3, 2), "relmse": round(rel_mse(ctx.image(), ref), 6)}
+    res["mlp_judge"]["choice"] = judge.get("choice")
     out["cornell_guided_equal_time"] = res
     log(f"cornell guided: {res}")
     # many-light RIS

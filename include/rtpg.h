@@ -179,13 +179,20 @@ int rtpg_render_debug(rtpg_context ctx, const rtpg_render_params* params, uint32
                       uint8_t* rgba);
 /* Per-sample relative variance of the last render made with RTPG_FLAG_MOMENTS
  * (at least 2 samples per pixel): the mean over pixels of s^2 / (m^2 + 0.01),
- * m and s^2 the mean and unbiased variance of the pixel's samples of
- * y = (r + g + b) / 3.  Times the render's cost per sample it ranks
+ * s^2 the unbiased variance of that render's samples of y = (r + g + b) / 3
+ * and m the pixel's y in the accumulated image (every render since the last
+ * clear: a steadier normaliser than the render's own few samples, whose mean
+ * would cap the bright outliers that dominate the relative error).  Times the render's cost per sample it ranks
  * estimators by efficiency: the learned guide's fallback decision
  * (rtlib_amd.tracer.render_mlp_guided) compares it between plain, SD-tree-
  * and learned-guide renders.  No reference counterpart (the reference judges
  * methods offline with RTLibExtOPX7Comp.cpp). */
 int rtpg_render_relvar(rtpg_context ctx, double* relvar);
+/* The raw moments of the last RTPG_FLAG_MOMENTS render: sums[2 i] and
+ * sums[2 i + 1] the sum and sum of squares of y over pixel i's samples (in
+ * accumulation-buffer order), *spp the render's samples per pixel.  Either
+ * may be NULL. */
+int rtpg_render_moments(rtpg_context ctx, float* sums, uint32_t* spp);
 /* Last render's counters: [0] paths, [1] radiance rays, [2] shadow rays. */
 int rtpg_render_stats(rtpg_context ctx, uint64_t stats[3]);
 /* Profiling of later renders (no reference counterpart; the reference only
@@ -379,13 +386,14 @@ int rtpg_mqtree_ctrl_query(rtpg_mqtree_ctrl c, uint32_t q[6]);
 typedef struct rtpg_mlp_config {
   uint32_t log2_table;   /* hash entries per level (19) */
   float base_res;        /* coarsest grid resolution (16) */
-  float max_res;         /* finest grid resolution (2048) */
+  float max_res;         /* finest grid resolution (256) */
   float lr;              /* Adam step size (1e-2) */
   float beta1, beta2;    /* Adam moments (0.9, 0.99) */
   float eps;             /* Adam epsilon (1e-15) */
   uint32_t batch;        /* samples per training step (2^18) */
   uint32_t capacity;     /* recorded samples kept between trainings (2^20) */
-  uint32_t record_paths; /* at most this many paths per render chunk record samples (2^16) */
+  uint32_t record_paths; /* at most this many paths per render chunk record samples (2^19: a
+                            1-spp pass of 1280 x 720 records more than 2^18 samples) */
   uint64_t seed;         /* initialisation */
   uint32_t guide_depth;  /* vertices guided and recorded: depth < guide_depth (0 = all) */
   uint32_t cache_res;    /* 0: the network is evaluated at every guided vertex; n > 0: the

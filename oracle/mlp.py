@@ -51,7 +51,7 @@ def split_weights(w):
 class Encoding:
     """Level resolutions and the cubic box of the position encoding."""
 
-    def __init__(self, lo, hi, log2_table=19, base_res=16.0, max_res=2048.0):
+    def __init__(self, lo, hi, log2_table=19, base_res=16.0, max_res=256.0):
         lo = np.asarray(lo, np.float32)
         hi = np.asarray(hi, np.float32)
         ext = np.float32(max(float(np.max(hi - lo)), 1e-6))

@@ -76,6 +76,7 @@ SIGNATURES = {
     "rtpg_render_debug": [c_vp, C.POINTER(RenderParams), C.c_uint32, c_fp, C.POINTER(C.c_uint8)],
     "rtpg_render_stats": [c_vp, c_u64p],
     "rtpg_render_relvar": [c_vp, C.POINTER(C.c_double)],
+    "rtpg_render_moments": [c_vp, c_vp, C.POINTER(C.c_uint32)],
     "rtpg_render_profile": [c_vp, C.c_uint32],
     "rtpg_adjoint": [c_vp, C.POINTER(RenderParams), c_fp, c_fp, c_fp],
     "rtpg_scene_set_emission": [c_vp, c_fp],

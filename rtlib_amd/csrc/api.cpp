@@ -705,16 +705,25 @@ extern "C" int rtpg_render_relvar(rtpg_context ctx, double* relvar) {
     require(ctx->moments.p != nullptr && ctx->moments_spp >= 2,
             "needs a render with RTPG_FLAG_MOMENTS and at least 2 samples per pixel");
     RTPG_HIP(hipSetDevice(ctx->device));
-    const size_t npix = (size_t)ctx->width * ctx->height;
-    std::vector<float> m = ctx->moments.to_vector(2 * npix, ctx->stream);
-    const double n = (double)ctx->moments_spp;
+    const uint32_t npix = ctx->width * ctx->height;
+    const uint32_t blocks = (npix + 255u) / 256u;
+    ctx->relvar_part.reserve(blocks);
+    RTPG_HIP(launch_relvar(ctx->moments.p, ctx->accum.p, npix, ctx->moments_spp, ctx->accum_spp, ctx->relvar_part.p,
+                           ctx->stream));
+    std::vector<double> ps = ctx->relvar_part.to_vector(blocks, ctx->stream);
     double acc = 0.0;
-    for (size_t i = 0; i < npix; ++i) {
-      const double mean = m[2 * i] / n;
-      const double var = std::max(0.0, ((double)m[2 * i + 1] - m[2 * i] * mean) / (n - 1.0));
-      acc += var / (mean * mean + 1e-2);
-    }
+    for (double v : ps) acc += v;
     *relvar = acc / (double)npix;
+  });
+}
+
+extern "C" int rtpg_render_moments(rtpg_context ctx, float* sums, uint32_t* spp) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    require(ctx->moments.p != nullptr, "no render with RTPG_FLAG_MOMENTS yet");
+    RTPG_HIP(hipSetDevice(ctx->device));
+    if (spp) *spp = ctx->moments_spp;
+    if (sums) ctx->moments.download(sums, 2 * (size_t)ctx->width * ctx->height, ctx->stream);
   });
 }
 

@@ -50,6 +50,7 @@ struct rtpg_context_t {
   uint32_t accum_spp = 0;
   rtpg::DBuf<float> moments;  // per pixel (sum, sum of squares) of the last RTPG_FLAG_MOMENTS render
   uint32_t moments_spp = 0;
+  rtpg::DBuf<double> relvar_part;  // per-work-group sums of rtpg_render_relvar
   rtpg::Camera cam{};
   // path arena
   rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC;

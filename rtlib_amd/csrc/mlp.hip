@@ -226,8 +226,9 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
       float pz = fminf(fmaxf((in.p.z - m.lo[2]) * m.inv_size, 0.0f), 1.0f);
       // levels below 1024 cells per axis: neighbouring samples (the window
       // is in spatial order) share cells, so each row of 16 samples sums its
-      // runs before the atomics (fp32 below kMlpPkFirst, packed bf16 pairs
-      // from there; the choice is uniform over a row: one level per row);
+      // runs before the atomics (fp32 through the work-group table for dense
+      // levels and those below kMlpPkFirst, packed bf16 pairs for the other
+      // hashed levels; the choice is uniform over a row: one level per row);
       // finer levels send every sample's contributions directly
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(kMlpTrainBlock) k_mlp_train(MlpView m, const f
           const int l = 8 * b + 2 * g + q;
           const float g0 = dh[b][2 * q], g1 = dh[b][2 * q + 1];
           if (m.res[l] < 1024.0f) {
-            if (l >= kMlpPkFirst) {
+            if (l >= kMlpPkFirst && !((m.dense >> l) & 1u)) {
               mlp_encode_level_grad_runs<true>(m, l, px, py, pz, g0, g1, valid);
             } else {
               mlp_encode_level_grad_runs<false>(m, l, px, py, pz, g0, g1, valid, tab);

@@ -39,14 +39,14 @@ rtpg_mlp_config default_config() {
   rtpg_mlp_config c{};
   c.log2_table = 19;
   c.base_res = 16.0f;
-  c.max_res = 2048.0f;
+  c.max_res = 256.0f;
   c.lr = 1e-2f;
   c.beta1 = 0.9f;
   c.beta2 = 0.99f;
   c.eps = 1e-15f;
   c.batch = 1u << 18;
   c.capacity = 1u << 20;
-  c.record_paths = 1u << 16;
+  c.record_paths = 1u << 19;
   c.seed = 0;
   c.guide_depth = 0;
   c.cache_res = 0;

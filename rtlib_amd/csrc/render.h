@@ -91,6 +91,10 @@ hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n
 void scratch_setup(int device);
 
 hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, uchar4* frame, hipStream_t s);
+// Per-work-group (256 pixels) sums of the relative variance of a moments render
+// (n samples per pixel) against the accumulated image (acc_spp samples).
+hipError_t launch_relvar(const float* mom, const float* accum, uint32_t npix, uint32_t n, uint32_t acc_spp,
+                         double* partial, hipStream_t s);
 
 hipError_t launch_debug(const RenderParams& rp, const Camera& cam, const SceneView& sc, const HashGridView& hg,
                         const MortonView& mq, uint32_t type, bool use_grid, float4* val, uchar4* frame,

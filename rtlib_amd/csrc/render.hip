@@ -1152,6 +1152,37 @@ hipError_t launch_tonemap(const float* accum, uint32_t npix, float inv_spp, ucha
   return hipGetLastError();
 }
 
+// Per-sample relative variance of a moments render, per work-group of 256
+// pixels: sum of s^2 / (m^2 + 0.01), s^2 the unbiased variance of the
+// render's n samples of y, m the pixel's y in the accumulated image.
+__global__ void k_relvar(const float* mom, const float* accum, uint32_t npix, float n, float inv_acc,
+                         double* partial) {
+  __shared__ double red[256];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < npix) {
+    double s1 = mom[2 * i], s2 = mom[2 * i + 1];
+    double var = fmax(0.0, (s2 - s1 * s1 / n) / (n - 1.0));
+    double ref = ((double)accum[3 * i] + accum[3 * i + 1] + accum[3 * i + 2]) * inv_acc;
+    v = var / (ref * ref + 1e-2);
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+hipError_t launch_relvar(const float* mom, const float* accum, uint32_t npix, uint32_t n, uint32_t acc_spp,
+                         double* partial, hipStream_t s) {
+  if (npix == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_relvar, dim3((npix + 255u) / 256u), dim3(256), 0, s, mom, accum, npix, (float)n,
+                     1.0f / (3.0f * (float)(acc_spp ? acc_spp : 1u)), partial);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ guide records
 // One kernel commits the guide vertices of a chunk (TraceVertexT::Commit,
 // OPX7UtilsPathGuiding.h:830-866; MortonTraceVertexT::Commit,

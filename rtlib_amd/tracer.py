@@ -202,10 +202,20 @@ class Context:
 
     def relvar(self):
         """Per-sample relative variance of the last FLAG_MOMENTS render (>= 2
-        spp): mean over pixels of s^2 / (m^2 + 0.01) of y = (r + g + b) / 3."""
+        spp): mean over pixels of s^2 / (m^2 + 0.01), s^2 the variance of that
+        render's samples of y = (r + g + b) / 3, m the pixel's y in the
+        accumulated image."""
         v = C.c_double()
         call("rtpg_render_relvar", self.h, C.byref(v))
         return float(v.value)
+
+    def moments(self):
+        """(sum, sum of squares) of y = (r + g + b) / 3 per pixel (H x W x 2)
+        and the samples per pixel of the last FLAG_MOMENTS render."""
+        m = np.zeros((self.height, self.width, 2), np.float32)
+        n = C.c_uint32()
+        call("rtpg_render_moments", self.h, ptr(m), C.byref(n))
+        return m, int(n.value)
 
     def render_stats(self):
         s = np.zeros(3, np.uint64)
@@ -620,8 +630,8 @@ class MLPGuide:
     omega (n x 3, sampled direction), weight (n, f Li |cos| / p_sample),
     pdf_bsdf (n)."""
 
-    def __init__(self, ctx: Context, aabb_min, aabb_max, log2_table=19, base_res=16.0, max_res=2048.0, lr=1e-2,
-                 beta1=0.9, beta2=0.99, eps=1e-15, batch=1 << 18, capacity=1 << 20, record_paths=1 << 16, seed=0,
+    def __init__(self, ctx: Context, aabb_min, aabb_max, log2_table=19, base_res=16.0, max_res=256.0, lr=1e-2,
+                 beta1=0.9, beta2=0.99, eps=1e-15, batch=1 << 18, capacity=1 << 20, record_paths=1 << 19, seed=0,
                  guide_depth=0, cache_res=0):
         cfg = _lib.MlpConfig(log2_table, base_res, max_res, lr, beta1, beta2, eps, batch, capacity, record_paths,
                              seed, guide_depth, cache_res)
@@ -745,35 +755,50 @@ def render_mlp_guided(ctx: Context, mlp: MLPGuide, budget, base_flags=FLAG_NEE |
     pass), so that an SD-tree trained on the same paths is at hand.
 
     judge: None, or a dict that enables the fallback decision and receives
-    its report.  Options: probe_spp (2), max_train_share (0.25).  Plain path
-    tracing renders a first probe pass; training stops (and plain path tracing
-    renders the rest) when the training passes would take more than
-    max_train_share of the budget's time at the plain rate.  After training,
-    probe passes of the learned guide and of the fallback SD-tree measure
-    their per-sample relative variance (Context.relvar) and cost, and the rest
-    of the budget goes to the estimator with the lowest variance x time (the
-    plain probe's included).  Probe passes count towards the budget and the
-    image; every pass is an unbiased estimate, so the mixture is too."""
+    its report.  Options: probe_spp (4), max_train_share (0.25), margin
+    (0.25).  Plain path tracing renders a first probe pass, which times it;
+    training stops (and plain path tracing renders the rest) when the
+    training passes would take more than max_train_share of the budget's time
+    at the plain rate: before the first pass by a lower bound of their time,
+    after it by the measured one.  After training, probe passes of plain path
+    tracing, the learned guide and the fallback SD-tree measure their
+    per-sample relative variance (Context.relvar) and cost; the rest of the
+    budget goes to the estimator with the lowest variance x time when that is
+    below the learned guide's by more than the margin, else to the learned
+    guide.  Probe passes count towards the budget and the image; every pass
+    is an unbiased estimate, so the mixture is too."""
     traced = 0
     trained = False
-    probe = int(judge.get("probe_spp", 2)) if judge is not None else 0
+    probe = int(judge.get("probe_spp", 4)) if judge is not None else 0
     cands = {}
     if fallback is not None:
         fallback.Clear()
 
-    def timed_render(spp, flags, **kw):
+    def probe_render(flags, **kw):
+        """A FLAG_MOMENTS pass of probe spp: (per-sample relative variance
+        against the image accumulated so far, cost per spp in seconds)."""
         ctx.synchronize()
         t = time.perf_counter()
-        ctx.render(spp=spp, flags=flags, max_depth=max_depth, rr_depth=rr_depth, seed=seed, sample_offset=traced,
-                   max_paths=max_paths, **kw)
+        ctx.render(spp=probe, flags=flags | FLAG_MOMENTS, max_depth=max_depth, rr_depth=rr_depth, seed=seed,
+                   sample_offset=traced, max_paths=max_paths, **kw)
         ctx.synchronize()
-        return time.perf_counter() - t
+        cost = (time.perf_counter() - t) / probe
+        return ctx.relvar(), cost
 
     if judge is not None and budget >= probe + 1:
-        t = timed_render(probe, base_flags | FLAG_MOMENTS)
-        cands["plain"] = (ctx.relvar(), t / probe)
+        cands["plain"] = probe_render(base_flags)
         traced += probe
     stop_training = False
+    if "plain" in cands:
+        # lower bound of the training passes' time: the recording passes at
+        # the plain rate plus, per Adam step, one pass over the fp32 table and
+        # weights (parameter, two moments and gradient read and written, the
+        # bf16 copy written) at the 8 TB/s HBM peak
+        info = mlp.info()
+        adam_bytes = 34.0 * (info["table_floats"] + info["weights"])
+        t_min = train_passes * (pass_spp * cands["plain"][1] + steps_per_pass * adam_bytes / 8e12)
+        share = float(judge.get("max_train_share", 0.25))
+        stop_training = t_min > share * cands["plain"][1] * budget
     for k in range(train_passes):
         if stop_training or traced + pass_spp + probe > budget:
             break
@@ -799,21 +824,33 @@ def render_mlp_guided(ctx: Context, mlp: MLPGuide, budget, base_flags=FLAG_NEE |
     rest = budget - traced
     choice = "learned"
     if judge is not None:
+        # the probes that rank the estimators run back to back, each scored
+        # against the image accumulated by then (all of them past the
+        # training passes, so the normalisers are close and steadier than a
+        # probe's own few samples, whose mean would cap the bright outliers
+        # that dominate the relative error)
+        n_probes = 3 if fallback is not None else 2
+        if not (stop_training or not trained) and rest >= (n_probes + 1) * probe:
+            cands["plain"] = probe_render(base_flags)
+            traced += probe
+            cands["learned"] = probe_render(base_flags | FLAG_USE_MLP | FLAG_FINAL | FLAG_GUIDE_SAMPLE, mlp=mlp)
+            traced += probe
+            if fallback is not None:
+                cands["sdtree"] = probe_render(base_flags | FLAG_USE_TREE | FLAG_FINAL | FLAG_GUIDE_SAMPLE,
+                                               sdtree=fallback, fraction=fraction)
+                traced += probe
+        report = {c: {"relvar": v, "ms_per_spp": 1e3 * t} for c, (v, t) in cands.items()}
+        eff = {c: r["relvar"] * r["ms_per_spp"] for c, r in report.items()}
         if stop_training or not trained:
             choice = "plain"
-        else:
-            if rest >= 2 * probe:
-                t = timed_render(probe, base_flags | FLAG_USE_MLP | FLAG_FINAL | FLAG_GUIDE_SAMPLE | FLAG_MOMENTS,
-                                 mlp=mlp)
-                cands["learned"] = (ctx.relvar(), t / probe)
-                traced += probe
-            if fallback is not None and budget - traced >= 2 * probe:
-                t = timed_render(probe, base_flags | FLAG_USE_TREE | FLAG_FINAL | FLAG_GUIDE_SAMPLE | FLAG_MOMENTS,
-                                 sdtree=fallback, fraction=fraction)
-                cands["sdtree"] = (ctx.relvar(), t / probe)
-                traced += probe
-            choice = min(cands, key=lambda c: cands[c][0] * cands[c][1]) if cands else "learned"
-        judge["candidates"] = {c: {"relvar": v, "ms_per_spp": 1e3 * t} for c, (v, t) in cands.items()}
+        elif "learned" in eff:
+            # the learned guide stays unless a fallback is measured clearly
+            # more efficient (a few probe samples per pixel estimate the
+            # variance loosely)
+            best = min(eff, key=eff.get)
+            margin = float(judge.get("margin", 0.25))
+            choice = best if eff[best] * (1.0 + margin) < eff["learned"] else "learned"
+        judge["candidates"] = report
         judge["choice"] = choice
         judge["training_stopped"] = stop_training
         rest = budget - traced

@@ -122,6 +122,7 @@ def test_moments_give_the_per_sample_variance(ctx):
     # squares of y = (r + g + b) / 3 over its two samples; the same samples
     # come from two 1-spp renders at sample offsets 0 and 1 (one stream per
     # pixel sample), whose images give the unbiased variance (ya - yb)^2 / 2.
+    # The normaliser is the pixel's y in the accumulated image.
     from rtlib_amd.tracer import FLAG_MOMENTS
 
     sc = scenes.cornell(48, 40)
@@ -137,8 +138,12 @@ def test_moments_give_the_per_sample_variance(ctx):
     ctx.render(spp=2, flags=BASE | FLAG_MOMENTS, max_depth=8, seed=9)
     got = ctx.relvar()
     assert want > 0.0 and abs(got - want) <= 1e-4 * want, (got, want)
-    ctx.render(spp=1, flags=BASE, max_depth=8, seed=9)  # moments stay those of the last MOMENTS render
-    assert ctx.relvar() == got
+    # a later render without the flag keeps the moments and changes the normaliser
+    ctx.render(spp=1, flags=BASE, max_depth=8, seed=9, sample_offset=2)
+    m3 = ctx.image().astype(np.float64).mean(axis=2)
+    want3 = float(np.mean(0.5 * (ys[0] - ys[1]) ** 2 / (m3 * m3 + 1e-2)))
+    got3 = ctx.relvar()
+    assert abs(got3 - want3) <= 1e-4 * want3, (got3, want3)
 
 
 def test_training_passes_record_into_the_fallback_sdtree(ctx):
@@ -185,8 +190,9 @@ def test_judge_falls_back_and_keeps_the_estimate_unbiased(ctx):
     # The judge's decisions: training stops after the first pass when it
     # cannot pay off (a zero training share), the rest of the budget then goes
     # to plain path tracing; with training allowed, the learned guide and the
-    # fallback SD-tree are probed and the estimator of least variance x time
-    # renders the rest.  Every image is an unbiased estimate of the reference.
+    # fallback SD-tree are probed, and the estimator of least variance x time
+    # renders the rest if it beats the learned guide by the margin (25%).
+    # Every image is an unbiased estimate of the reference.
     from rtlib_amd.tracer import RTSTreeWrapper
 
     sc = scenes.guided_room(width=160, height=90, grid=(128, 64))
@@ -216,7 +222,9 @@ def test_judge_falls_back_and_keeps_the_estimate_unbiased(ctx):
     c = j1["candidates"]
     assert set(c) == {"plain", "learned", "sdtree"}
     assert all(v["relvar"] > 0.0 and v["ms_per_spp"] > 0.0 for v in c.values())
-    assert j1["choice"] == min(c, key=lambda k: c[k]["relvar"] * c[k]["ms_per_spp"])
+    eff = {k: c[k]["relvar"] * c[k]["ms_per_spp"] for k in c}
+    best = min(eff, key=eff.get)
+    assert j1["choice"] == (best if eff[best] * 1.25 < eff["learned"] else "learned")
     for img in (img0, img1):
         assert abs(img.mean() - ref.mean()) / ref.mean() < 0.03
     tree.Destroy()

@@ -1197,11 +1197,24 @@ hipError_t launch_relvar(const float* mom, const float* accum, uint32_t npix, ui
 // roots do not serialise on HBM atomics; a key that finds no LDS slot in 16
 // probes goes to HBM directly.
 constexpr int kMaxCommitVtx = 16;
-constexpr uint32_t kCommitBlock = 256;
-constexpr uint32_t kCommitPaths = 32;  // paths per work-item
-constexpr uint32_t kLeafTab = 4096;
-constexpr uint32_t kCellTab = 1024;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+// Work-group shape per guide: work-items, paths per work-item, LDS slots for
+// leaf values and for DTree weights.  The SD-tree's leaves are many and
+// spread (a DTree each per spatial leaf): 16 waves over 8192 paths share a
+// 16k-slot table (144 KB of LDS, one work-group per CU), which misses far
+// less often than four waves with 4k slots (measured: 9.0 vs 12.2 ms per
+// 64-spp recording pass of the room).  The hash grid's Morton leaves fit the
+// smaller shape.
+template <int kKind>
+struct CommitShape;
+template <>
+struct CommitShape<1> {
+  static constexpr uint32_t block = 1024, paths = 8, leaf = 16384, cell = 2048;
+};
+template <>
+struct CommitShape<2> {
+  static constexpr uint32_t block = 256, paths = 32, leaf = 4096, cell = 1;
+};
 
 // Insert-or-add into an LDS open-addressing table; false when 16 probes miss
 // (the caller then adds to global memory directly).
@@ -1296,11 +1309,14 @@ __device__ __forceinline__ bool wave_combine(bool active, uint32_t key, float& v
 // kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285, which tests
 // no value).
 template <int kKind>
-__global__ void __launch_bounds__(kCommitBlock) k_commit(PathSoA ps, uint32_t max_vtx, SdTreeView sd, MortonView mq) {
+__global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps, uint32_t max_vtx, SdTreeView sd,
+                                                                     MortonView mq) {
+  constexpr uint32_t kCommitBlock = CommitShape<kKind>::block, kCommitPaths = CommitShape<kKind>::paths;
+  constexpr uint32_t kLeafTab = CommitShape<kKind>::leaf, kCellTab = CommitShape<kKind>::cell;
   __shared__ uint32_t lkey[kLeafTab];
   __shared__ float lval[kLeafTab];
-  __shared__ uint32_t ckey[kKind == 1 ? kCellTab : 1];
-  __shared__ uint32_t cval[kKind == 1 ? kCellTab : 1];
+  __shared__ uint32_t ckey[kCellTab];
+  __shared__ uint32_t cval[kCellTab];
   const uint32_t first = blockIdx.x * kCommitBlock * kCommitPaths;
   if (first >= ps.n) return;  // uniform over the work-group
   for (uint32_t i = threadIdx.x; i < kLeafTab; i += kCommitBlock) {
@@ -1593,11 +1609,14 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
   mark(3, 0, 1);
   mark(4, 0, 0);
   if (record && (guide == 1 || guide == 2 || (guide == 3 && sd.stree))) {
-    const uint32_t nc = blocks_for(ps.n, kCommitBlock * kCommitPaths);
     if (guide != 2) {
-      hipLaunchKernelGGL(k_commit<1>, dim3(nc), dim3(kCommitBlock), 0, s, ps, rp.max_depth - 1u, sd, mq);
+      using Sh = CommitShape<1>;
+      hipLaunchKernelGGL(k_commit<1>, dim3(blocks_for(ps.n, Sh::block * Sh::paths)), dim3(Sh::block), 0, s, ps,
+                         rp.max_depth - 1u, sd, mq);
     } else {
-      hipLaunchKernelGGL(k_commit<2>, dim3(nc), dim3(kCommitBlock), 0, s, ps, rp.max_depth - 1u, sd, mq);
+      using Sh = CommitShape<2>;
+      hipLaunchKernelGGL(k_commit<2>, dim3(blocks_for(ps.n, Sh::block * Sh::paths)), dim3(Sh::block), 0, s, ps,
+                         rp.max_depth - 1u, sd, mq);
     }
   }
   if (record && guide == 3) {

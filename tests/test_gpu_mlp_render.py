@@ -27,24 +27,22 @@ def test_mlp_guided_room_unbiased_and_lower_error(ctx):
     for k in range(8):
         ctx.render(spp=256, flags=BASE, max_depth=10, seed=999, sample_offset=256 * k)
     ref = ctx.image()
-    # relMSE has heavy tails (rare bright paths): the errors are averaged over
-    # two independent renders of each method (over 8 seeds the guided / plain
-    # ratio is about 0.79 with a standard deviation of 0.035)
-    e_plain, e_mlp, means = [], [], []
-    for plain_seed, seed in ((2, 5), (3, 6)):
-        ctx.clear()
-        ctx.render(spp=64, flags=BASE, max_depth=10, seed=plain_seed)
-        e_plain.append(_rel_mse(ctx.image(), ref))
-        mlp = MLPGuide(ctx, lo, hi, seed=1)
-        ctx.clear()
-        n = render_mlp_guided(ctx, mlp, 64, BASE, max_depth=10, seed=seed, train_passes=8, steps_per_pass=4)
-        guided = ctx.image()
-        assert n == 64 and mlp.info()["steps"] == 32
-        e_mlp.append(_rel_mse(guided, ref))
-        means.append(guided.mean())
-        mlp.close()
-    assert np.mean(e_mlp) < 0.9 * np.mean(e_plain), (e_mlp, e_plain)
-    assert abs(np.mean(means) - ref.mean()) / ref.mean() < 0.02
+    # one render of each method at 64 spp: over 8 seeds the learned guide's
+    # relMSE is 0.754 +- 0.017 of plain path tracing's (worst 0.783), so 0.85
+    # leaves more than five standard deviations
+    ctx.clear()
+    ctx.render(spp=64, flags=BASE, max_depth=10, seed=2)
+    e_plain = _rel_mse(ctx.image(), ref)
+    mlp = MLPGuide(ctx, lo, hi, seed=1)
+    ctx.clear()
+    n = render_mlp_guided(ctx, mlp, 64, BASE, max_depth=10, seed=5, train_passes=8, steps_per_pass=4)
+    guided = ctx.image()
+    assert n == 64 and mlp.info()["steps"] == 32
+    e_mlp = _rel_mse(guided, ref)
+    mlp.close()
+    print(f"learned guide relMSE {e_mlp:.5f} vs plain {e_plain:.5f} (ratio {e_mlp / e_plain:.3f})", flush=True)
+    assert e_mlp < 0.85 * e_plain, (e_mlp, e_plain)
+    assert abs(guided.mean() - ref.mean()) / ref.mean() < 0.02
 
 
 def test_mlp_recording_without_sampling_is_plain_path_tracing(ctx):

@@ -44,6 +44,42 @@ __global__ void __launch_bounds__(64) k_morton_build(float* w, uint32_t level, u
   }
 }
 
+// The same for level <= 3 (at most 16 parents per step), 16 lanes per cell:
+// lane p adds parent p of the deepest step from its four leaves (the 16 lanes
+// read 64 consecutive floats), and each coarser step takes the four children
+// from the lanes that hold them; the additions keep the order above (parent,
+// then children 0..3).  Four cells per wave instead of one.
+__global__ void __launch_bounds__(256) k_morton_build16(float* w, uint32_t level, uint32_t per_cell,
+                                                         uint32_t cells) {
+  const uint32_t lane = threadIdx.x & 15u;
+  const uint32_t cell = (blockIdx.x * 256u + threadIdx.x) >> 4;
+  const bool in = cell < cells;  // uniform over the 16 lanes of a cell
+  float* base = w + (size_t)per_cell * (in ? cell : 0u);
+  float cur = 0.0f;
+  for (uint32_t src_lvl = level; src_lvl >= 1u; --src_lvl) {
+    const uint32_t src = mq_level_offset(src_lvl), dst = mq_level_offset(src_lvl - 1u);
+    const uint32_t parents = 1u << (2u * (src_lvl - 1u));
+    float c[4];
+    if (src_lvl == level) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[k] = (in && lane < parents) ? base[src + 4u * lane + (uint32_t)k] : 0.0f;
+    } else {
+      // lanes 4p .. 4p + 3 hold the children of parent p (4p + 3 <= 15)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[k] = __shfl(cur, (int)((4u * lane + (uint32_t)k) & 15u), 16);
+    }
+    if (in && lane < parents) {
+      float v = base[dst + lane];
+      v += c[0];
+      v += c[1];
+      v += c[2];
+      v += c[3];
+      base[dst + lane] = v;
+      cur = v;
+    }
+  }
+}
+
 // MortonQuadTreeT::Record leaf-only branch (OPX7UtilsMorton.h:262-285) for a batch.
 __global__ void k_morton_record(MortonView m, const uint32_t* cells, const float* canon, const float* val,
                                 uint32_t n) {
@@ -89,7 +125,11 @@ hipError_t launch_grid_find(HashGridView h, const float* pos, uint32_t n, bool i
 }
 hipError_t launch_morton_build(float* w, uint32_t level, uint32_t per_cell, uint32_t cells, hipStream_t s) {
   if (cells == 0 || level == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_morton_build, dim3(cells), dim3(64), 0, s, w, level, per_cell, cells);
+  if (level <= 3u) {
+    hipLaunchKernelGGL(k_morton_build16, dim3(nblk(cells, 16u)), dim3(256), 0, s, w, level, per_cell, cells);
+  } else {
+    hipLaunchKernelGGL(k_morton_build, dim3(cells), dim3(64), 0, s, w, level, per_cell, cells);
+  }
   return hipGetLastError();
 }
 hipError_t launch_morton_record(MortonView m, const uint32_t* cells, const float* canon, const float* val,

@@ -113,6 +113,8 @@ def main(argv=None):
     ap.add_argument("--pmc-step", action="store_true", help="run exactly one guided step of the workload (seed 1, "
                     "the first timed step's) and exit: the command that the counter summaries under profiles/ "
                     "are collected from")
+    ap.add_argument("--pmc-grid", action="store_true", help="run one hash-grid-guided render of the workload "
+                    "(seed 1) and exit: the command the hash-grid kernel trace in profiles/ is taken from")
     ap.add_argument("--pmc-mlp", action="store_true", help="run one learned-guide render of the workload and exit "
                     "(for the matrix-core counters of its training and inference kernels)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -159,6 +161,17 @@ def main(argv=None):
         return render_guided(ctx, ctrl, tree, "tree", flags, max_depth=depth, seed=seed, fraction=frac,
                              on_pass=on_pass)
 
+    if a.pmc_grid:
+        from rtlib_amd.tracer import (DoubleBufferedHashGrid3Buffer, RTMortonQuadTreeController,
+                                      RTMortonQuadTreeWrapper)
+        grid = DoubleBufferedHashGrid3Buffer(ctx, lo, hi, bounds=bounds, slots=1 << 20, key_mode=1)
+        mq = RTMortonQuadTreeWrapper(ctx, 1 << 20, 3, frac)
+        ctrl = RTMortonQuadTreeController(mq, grid, a.budget, 3, 0, 0.5, 1)
+        ctx.clear()
+        render_guided(ctx, ctrl, mq, "grid", base, max_depth=depth, seed=1, fraction=frac, grid=grid)
+        ctx.synchronize()
+        log("one hash-grid-guided render done")
+        return 0
     if a.pmc_step:
         guided_step(1)
         ctx.synchronize()

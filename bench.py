@@ -48,6 +48,12 @@ TRI_BYTES = 36        # three float3 vertices per tested triangle
 RAY_IN_BYTES = 32     # origin + direction
 HIT_OUT_BYTES = 16    # t, u, v, triangle
 QUEUE_BYTES = 4       # queue index read
+SHADOW_REC_BYTES = 48  # shadow ray record: origin + tmax, direction + path, contribution
+SHADE_VTX_BYTES = 200  # hit 16 + indices 12 + vertices 36 + material 16 + state in 44 + out 44 + next ray 32
+GUIDE_VTX_BYTES = 48   # recorded guide vertex (three float4), written by shade and read by commit
+COMMIT_ATOMIC_BYTES = 8  # the DTree weight and one leaf slot (the record descent's nodes are not counted)
+ACCUM_PIX_BYTES = 24   # accumulation buffer read-modify-write per pixel and launch
+PATH_L_BYTES = 16      # per-sample radiance read by the accumulation
 PMC_FILE = ROOT / "profiles" / "guided_room_counters.json"
 
 
@@ -224,13 +230,17 @@ def main(argv=None):
     # random streams, so the same paths and the same traversal work
     ctx.set_profile(time_kernels=False, count_work=True)
     work = np.zeros(4)
-    rays_c = 0
+    rays_c = shadow_c = rec_rays = 0
 
     def acc_work(state, spp, traced):
-        nonlocal rays_c
+        nonlocal rays_c, shadow_c, rec_rays
         _, wk, _ = ctx.kernel_stats()
         work[:] += np.array([wk["extend_nodes"], wk["extend_tris"], wk["shadow_nodes"], wk["shadow_tris"]])
-        rays_c += ctx.render_stats()["rays"]
+        st = ctx.render_stats()
+        rays_c += st["rays"]
+        shadow_c += st["shadow_rays"]
+        if state != 2:  # recording passes (every state but the final Sample one)
+            rec_rays += st["rays"]
 
     guided_step(1, on_pass=acc_work)
     ctx.set_profile(False, False)
@@ -265,6 +275,12 @@ def main(argv=None):
     # ---- roofline of the dominant kernel (k_extend)
     ext_bytes = rays_c * (RAY_IN_BYTES + HIT_OUT_BYTES + QUEUE_BYTES) + work[0] * NODE_BYTES + work[1] * TRI_BYTES
     roofline = extend_roofline(W, H, per_kernel, launches, ext_bytes, names, plain_kernel_ms)
+    algo = {"extend": ext_bytes,
+            "shade": rays_c * SHADE_VTX_BYTES + shadow_c * SHADOW_REC_BYTES + rec_rays * GUIDE_VTX_BYTES,
+            "shadow": shadow_c * SHADOW_REC_BYTES + work[2] * NODE_BYTES + work[3] * TRI_BYTES,
+            "accumulate": a.budget * npix * PATH_L_BYTES + int(launches[3]) * npix * ACCUM_PIX_BYTES,
+            "commit": rec_rays * (GUIDE_VTX_BYTES + COMMIT_ATOMIC_BYTES)}
+    roofline.update(per_kernel_roofline(W, H, names, per_kernel, launches, algo, t_step))
 
     # ---- the other configurations, measured on the same context
     configs = None if (a.small or a.no_configs) else other_configs(ctx, timed, frac)
@@ -473,6 +489,41 @@ def extend_roofline(W, H, per_kernel, launches, ext_bytes, names, plain_kernel_m
                     "traffic_over_touched": round(traffic / max(ext_bytes / max(n_ext, 1), 1), 3)}
         r["counters_source"] = f"profiles/{PMC_FILE.name}"
     return r
+
+
+def per_kernel_roofline(W, H, names, per_kernel, launches, algo, t_step):
+    """Every kernel class of the guided step: the rate of the algorithmic
+    bytes it touches (from the work-count pass: rays, shadow rays, visited
+    nodes, tested triangles, recorded vertices; most hit L2 and the MALL, so
+    this rate is no HBM fraction) over its event-timed ms per step, and, where
+    the counter summary holds the kernel, its HBM traffic per step
+    (FETCH_SIZE and WRITE_SIZE per dispatch x dispatches) as a fraction of
+    the HBM peak; the frame aggregate divides the sums by the step's wall
+    time."""
+    pmc = load_counters(W, H)
+    key = {"extend": "k_extend", "shade": "k_shade", "shadow": "k_shadow", "commit": "k_commit"}
+    out, tot_algo, tot_hbm = {}, 0.0, 0.0
+    for i, n in enumerate(names):
+        ms = float(per_kernel[i])
+        b = float(algo[n])
+        e = {"ms_per_step": round(ms, 2), "algorithmic_bytes_per_step": round(b),
+             "algorithmic_gbs": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+        tot_algo += b
+        if pmc is not None and key.get(n) in pmc["kernels"]:
+            k = pmc["kernels"][key[n]]
+            per_disp = k["FETCH_SIZE"] * 1024 * pmc["fetch_scale"] + k["WRITE_SIZE"] * 1024 * pmc["write_scale"]
+            hbm = per_disp * int(launches[i])
+            tot_hbm += hbm
+            e["hbm_bytes_per_step"] = round(hbm)
+            e["hbm_frac"] = round(hbm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None
+            if "active_lanes_per_valu" in k:
+                e["active_lanes_per_valu"] = k["active_lanes_per_valu"]
+        out[n] = e
+    frame = {"ms_per_step": round(t_step * 1e3, 2), "algorithmic_bytes_per_step": round(tot_algo),
+             "algorithmic_gbs": round(tot_algo / t_step / 1e9, 1),
+             "hbm_bytes_per_step": round(tot_hbm) if tot_hbm else None,
+             "hbm_frac": round(tot_hbm / t_step / 1e9 / HBM_PEAK_GBS, 4) if tot_hbm else None}
+    return {"per_kernel": out, "frame": frame}
 
 
 def other_configs(ctx, timed, frac):

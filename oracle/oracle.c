@@ -947,29 +947,35 @@ void orc_sdtree_pdf(const uint32_t* s_base, const float* s_sum, const float* s_s
 }
 
 /* ======================================================================
- * geometry: Moller-Trumbore, same operation order as the HIP kernel
+ * geometry: Moller-Trumbore, two-sided, operation for operation as the HIP
+ * kernel (render.hip isect_tri): the last product of every cross and dot
+ * product fused (fmaf, correctly rounded here as on the GPU), barycentric
+ * tests on u det and v det against det, one division for a passing triangle.
+ * The reference's triangle test is OptiX's (closed): parity with it is
+ * unpinned; brute force over this test pins the traversal.
  * ==================================================================== */
 int orc_isect_tri(const float* o, const float* d, const float* a, const float* e1, const float* e2, float tmin,
                   float* tbest, float* ou, float* ov) {
-  float px = d[1] * e2[2] - d[2] * e2[1];
-  float py = d[2] * e2[0] - d[0] * e2[2];
-  float pz = d[0] * e2[1] - d[1] * e2[0];
-  float det = e1[0] * px + e1[1] * py + e1[2] * pz;
+  float px = fmaf(d[1], e2[2], -(d[2] * e2[1]));
+  float py = fmaf(d[2], e2[0], -(d[0] * e2[2]));
+  float pz = fmaf(d[0], e2[1], -(d[1] * e2[0]));
+  float det = fmaf(e1[0], px, fmaf(e1[1], py, e1[2] * pz));
   if (det == 0.0f) return 0;
-  float inv = 1.0f / det;
   float sx = o[0] - a[0], sy = o[1] - a[1], sz = o[2] - a[2];
-  float u = (sx * px + sy * py + sz * pz) * inv;
-  if (!(u >= 0.0f && u <= 1.0f)) return 0;
-  float qx = sy * e1[2] - sz * e1[1];
-  float qy = sz * e1[0] - sx * e1[2];
-  float qz = sx * e1[1] - sy * e1[0];
-  float v = (d[0] * qx + d[1] * qy + d[2] * qz) * inv;
-  if (!(v >= 0.0f && u + v <= 1.0f)) return 0;
-  float t = (e2[0] * qx + e2[1] * qy + e2[2] * qz) * inv;
+  float ud = fmaf(sx, px, fmaf(sy, py, sz * pz));
+  if (det > 0.0f ? !(ud >= 0.0f && ud <= det) : !(ud <= 0.0f && ud >= det)) return 0;
+  float qx = fmaf(sy, e1[2], -(sz * e1[1]));
+  float qy = fmaf(sz, e1[0], -(sx * e1[2]));
+  float qz = fmaf(sx, e1[1], -(sy * e1[0]));
+  float vd = fmaf(d[0], qx, fmaf(d[1], qy, d[2] * qz));
+  float uvd = ud + vd;
+  if (det > 0.0f ? !(vd >= 0.0f && uvd <= det) : !(vd <= 0.0f && uvd >= det)) return 0;
+  float inv = 1.0f / det;
+  float t = fmaf(e2[0], qx, fmaf(e2[1], qy, e2[2] * qz)) * inv;
   if (!(t > tmin && t < *tbest)) return 0;
   *tbest = t;
-  *ou = u;
-  *ov = v;
+  *ou = ud * inv;
+  *ov = vd * inv;
   return 1;
 }
 

@@ -48,8 +48,11 @@ __device__ __forceinline__ bool records_path(const PathSoA& ps, uint32_t pid) {
   return ps.mlp_stride ? pid % ps.mlp_stride == ps.mlp_phase : pid < ps.rec_limit;
 }
 
-// Moller-Trumbore without FMA contraction so hits are bit-identical to the
-// CPU brute-force oracle.  Two-sided.
+// Moller-Trumbore, two-sided, in a form the CPU oracle repeats operation for
+// operation, so hits are bit-identical: the cross and dot products fuse their
+// last products (fmaf rounds once, on both sides alike), the barycentric
+// tests compare the unnormalised u det and v det against det, and the one
+// division is made only for a triangle that passes them.
 __device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tmin, float& tbest,
                                           float& ou, float& ov) {
 #pragma clang fp contract(off)
@@ -57,25 +60,28 @@ __device__ __forceinline__ bool isect_tri(const TriRec* tp, V3 o, V3 d, float tm
   float4 a = q[0], b = q[1], c = q[2];
   float e1x = b.x, e1y = b.y, e1z = b.z;
   float e2x = c.x, e2y = c.y, e2z = c.z;
-  float px = d.y * e2z - d.z * e2y;
-  float py = d.z * e2x - d.x * e2z;
-  float pz = d.x * e2y - d.y * e2x;
-  float det = e1x * px + e1y * py + e1z * pz;
+  float px = fmaf(d.y, e2z, -(d.z * e2y));
+  float py = fmaf(d.z, e2x, -(d.x * e2z));
+  float pz = fmaf(d.x, e2y, -(d.y * e2x));
+  float det = fmaf(e1x, px, fmaf(e1y, py, e1z * pz));
   if (det == 0.0f) return false;
-  float inv = 1.0f / det;
   float sx = o.x - a.x, sy = o.y - a.y, sz = o.z - a.z;
-  float u = (sx * px + sy * py + sz * pz) * inv;
-  if (!(u >= 0.0f && u <= 1.0f)) return false;
-  float qx = sy * e1z - sz * e1y;
-  float qy = sz * e1x - sx * e1z;
-  float qz = sx * e1y - sy * e1x;
-  float v = (d.x * qx + d.y * qy + d.z * qz) * inv;
-  if (!(v >= 0.0f && u + v <= 1.0f)) return false;
-  float t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  float ud = fmaf(sx, px, fmaf(sy, py, sz * pz));
+  // u = ud / det in [0, 1]
+  if (det > 0.0f ? !(ud >= 0.0f && ud <= det) : !(ud <= 0.0f && ud >= det)) return false;
+  float qx = fmaf(sy, e1z, -(sz * e1y));
+  float qy = fmaf(sz, e1x, -(sx * e1z));
+  float qz = fmaf(sx, e1y, -(sy * e1x));
+  float vd = fmaf(d.x, qx, fmaf(d.y, qy, d.z * qz));
+  float uvd = ud + vd;
+  // v >= 0 and u + v <= 1
+  if (det > 0.0f ? !(vd >= 0.0f && uvd <= det) : !(vd <= 0.0f && uvd >= det)) return false;
+  float inv = 1.0f / det;
+  float t = fmaf(e2x, qx, fmaf(e2y, qy, e2z * qz)) * inv;
   if (!(t > tmin && t < tbest)) return false;
   tbest = t;
-  ou = u;
-  ov = v;
+  ou = ud * inv;
+  ov = vd * inv;
   return true;
 }
 

@@ -33,7 +33,21 @@ constexpr int kLdsW3 = kLdsW2 + kMlpHidden * kLdsP1;
 constexpr int kLdsWeights = kLdsW3 + kMlpOut * kLdsP1;  // 16256 bf16
 constexpr int kHeadPitch = 49;                           // floats per sample row of the head scratch
 
+// 19 wait states, with nothing scheduled across them.
+__device__ __forceinline__ void mfma_drain() {
+  __builtin_amdgcn_sched_barrier(0);
+  asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 3");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// The lone 16-wide block of a layer accumulates onto the 32-wide form's
+// result.  Issued right behind that v_mfma_f32_16x16x32_bf16 (as the compiler
+// schedules it, with no wait between the two), the 16x16x16 form read a
+// stale accumulator now and then: the network's outputs for identical inputs
+// and weights differed from run to run in whole waves.  The wait before it
+// makes them identical.
 __device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+  mfma_drain();
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 

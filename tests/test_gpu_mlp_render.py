@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from rtlib_amd import scenes
-from rtlib_amd.tracer import (FLAG_GUIDE_SAMPLE, FLAG_MIS, FLAG_NEE, FLAG_RR, FLAG_USE_MLP, Context, MLPGuide,
+from rtlib_amd.tracer import (FLAG_FINAL, FLAG_GUIDE_SAMPLE, FLAG_MIS, FLAG_NEE, FLAG_RR, FLAG_USE_MLP, Context, MLPGuide,
                               render_mlp_guided)
 
 pytestmark = pytest.mark.gpu
@@ -43,6 +43,28 @@ def test_mlp_guided_room_unbiased_and_lower_error(ctx):
     print(f"learned guide relMSE {e_mlp:.5f} vs plain {e_plain:.5f} (ratio {e_mlp / e_plain:.3f})", flush=True)
     assert e_mlp < 0.85 * e_plain, (e_mlp, e_plain)
     assert abs(guided.mean() - ref.mean()) / ref.mean() < 0.02
+
+
+def test_mlp_guided_render_is_deterministic(ctx):
+    # A trained network and fixed random streams give the same image every
+    # time: the inference pass's matrix-core tiles must not depend on timing
+    # (the per-vertex inference of a whole depth-4 render, twice).
+    sc = scenes.guided_room(width=160, height=90, grid=(128, 64))
+    ctx.upload_scene(sc)
+    info = ctx.scene_info()
+    lo, hi = info["aabb_min"] - 0.005, info["aabb_max"] + 0.005
+    mlp = MLPGuide(ctx, lo, hi, seed=3)
+    ctx.clear()
+    render_mlp_guided(ctx, mlp, 8, BASE, max_depth=6, seed=5, train_passes=4, steps_per_pass=4)
+    imgs = []
+    for _ in range(3):
+        ctx.clear()
+        ctx.render(spp=4, flags=BASE | FLAG_USE_MLP | FLAG_GUIDE_SAMPLE | FLAG_FINAL, max_depth=4, seed=7,
+                   sample_offset=100, mlp=mlp)
+        imgs.append(ctx.image())
+    mlp.close()
+    for k in (1, 2):
+        assert np.array_equal(imgs[0], imgs[k]), float(np.abs(imgs[0] - imgs[k]).max())
 
 
 def test_mlp_recording_without_sampling_is_plain_path_tracing(ctx):

@@ -392,9 +392,13 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
   c->p_ray_o.reserve(n);
   c->p_ray_d.reserve(n);
   c->p_T.reserve(n);
+  c->p_rng.reserve(n);
+  c->p_ray_o1.reserve(n);
+  c->p_ray_d1.reserve(n);
+  c->p_T1.reserve(n);
+  c->p_rng1.reserve(n);
   c->p_L.reserve(n);
   c->p_hit.reserve(n);
-  c->p_rng.reserve(n);
   c->p_nhit.reserve(n);
   c->p_q0.reserve(n);
   c->p_q1.reserve(n);
@@ -424,12 +428,16 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
   ensure_arena(ctx, n, max_depth, guide, mlp_rec, mix_slots);
   PathSoA ps{};
   ps.n = n;
-  ps.ray_o = ctx->p_ray_o.p;
-  ps.ray_d = ctx->p_ray_d.p;
-  ps.T = ctx->p_T.p;
+  ps.ray_o[0] = ctx->p_ray_o.p;
+  ps.ray_o[1] = ctx->p_ray_o1.p;
+  ps.ray_d[0] = ctx->p_ray_d.p;
+  ps.ray_d[1] = ctx->p_ray_d1.p;
+  ps.T[0] = ctx->p_T.p;
+  ps.T[1] = ctx->p_T1.p;
+  ps.rng[0] = ctx->p_rng.p;
+  ps.rng[1] = ctx->p_rng1.p;
   ps.L = ctx->p_L.p;
   ps.hit = ctx->p_hit.p;
-  ps.rng = ctx->p_rng.p;
   ps.nhit = ctx->p_nhit.p;
   ps.queue[0] = ctx->p_q0.p;
   ps.queue[1] = ctx->p_q1.p;

@@ -54,9 +54,10 @@ struct rtpg_context_t {
   rtpg::Camera cam{};
   // path arena
   rtpg::DBuf<float4> p_ray_o, p_ray_d, p_T, p_L, p_hit, p_shadow, p_vA, p_vB, p_vC;
+  rtpg::DBuf<float4> p_ray_o1, p_ray_d1, p_T1;  // second slot buffers of the path state
   rtpg::DBuf<float4> p_mix, p_mv;  // learned guide: mixture per queue slot; training vertices (5 arrays)
   rtpg::DBuf<float> p_mixc;
-  rtpg::DBuf<uint2> p_rng;
+  rtpg::DBuf<uint2> p_rng, p_rng1;
   rtpg::DBuf<uint32_t> p_nhit, p_q0, p_q1, p_count, p_deep, p_err;
   rtpg::DBuf<unsigned long long> p_stats;
   uint64_t last_stats[3] = {0, 0, 0};

@@ -25,19 +25,23 @@ constexpr uint32_t kVtxBadBsdf = 4u;
 // Path pid = pixel + sample * npix.
 struct PathSoA {
   uint32_t n;
-  float4* ray_o;     // origin, w = pdf of the sampled direction (MIS)
-  float4* ray_d;     // direction, w = meta word
-  float4* T;         // throughput
-  float4* L;         // radiance
-  float4* hit;       // t, u, v, leaf-order triangle (bits; 0xFFFFFFFF = miss)
-  uint2* rng;        // PCG32 state
-  uint32_t* nhit;    // hits processed
-  uint32_t* queue[2];
+  // The state of a live path moves with it: it is stored at the path's slot
+  // in the queue of its depth (buffer depth & 1) and shading writes it to the
+  // slot the path takes in the next queue, so every bounce kernel reads and
+  // writes its rays contiguously however few paths are left.
+  float4* ray_o[2];  // origin, w = pdf of the sampled direction (MIS)
+  float4* ray_d[2];  // direction, w = meta word
+  float4* T[2];      // throughput
+  uint2* rng[2];     // PCG32 state
+  float4* hit;       // per slot of the current depth: t, u, v, leaf-order triangle (bits; 0xFFFFFFFF = miss)
+  uint32_t* queue[2];  // path index (pid) per slot
+  float4* L;         // radiance, per path (pid)
+  uint32_t* nhit;    // hits processed, per path
   uint32_t* count;   // live paths per depth [max_depth + 2]
   uint32_t* scount;  // shadow rays per depth [max_depth + 2]
   float4* shadow;    // 3 x float4 per shadow ray
   float* moments;    // per pixel: sum and sum of squares of the samples' RGB mean, or NULL
-  uint32_t* deep;    // traversal spill stack: (entry, distance) pairs, 2 x (cap - kShortStack) x n words
+  uint32_t* deep;    // traversal spill stack per ray slot: (entry, distance) pairs, 2 x (cap - kShortStack) x n words
   uint32_t* err;     // sticky error bits
   // guide vertices, [depth][pid]
   float4* vA;        // throughput after the bounce, woPdf

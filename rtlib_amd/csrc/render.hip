@@ -327,11 +327,12 @@ __global__ void __launch_bounds__(kBlock) k_camera(RenderParams rp, Camera cam, 
   float dy = 2.0f * ((float)y + jy) / (float)rp.height - 1.0f;
   V3 U{cam.u[0], cam.u[1], cam.u[2]}, Vv{cam.v[0], cam.v[1], cam.v[2]}, W{cam.w[0], cam.w[1], cam.w[2]};
   V3 d = normalize(dx * U + dy * Vv + W);
-  ps.ray_o[pid] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], 0.0f);
-  ps.ray_d[pid] = make_float4(d.x, d.y, d.z, bitsf(kMetaDelta));  // depth 0, counts emission
-  ps.T[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  // depth 0: the path's slot is its index
+  ps.ray_o[0][pid] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], 0.0f);
+  ps.ray_d[0][pid] = make_float4(d.x, d.y, d.z, bitsf(kMetaDelta));  // depth 0, counts emission
+  ps.T[0][pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   ps.L[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  ps.rng[pid] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+  ps.rng[0][pid] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
   ps.queue[0][pid] = pid;
 }
 
@@ -358,14 +359,13 @@ k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
   uint32_t n_act = ps.count[depth];
   uint32_t cnt[2] = {0u, 0u};
   if (i < n_act) {
-    uint32_t pid = ps.queue[depth & 1][i];
-    float4 a = ps.ray_o[pid], b = ps.ray_d[pid];
-    StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
+    float4 a = ps.ray_o[depth & 1][i], b = ps.ray_d[depth & 1][i];
+    StackRef st = make_stack(lds, ps.deep, ps.n, i, sc.max_stack);
     float t, u, v;
     int32_t tri;
     traverse<false, kCount>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, kRayTmax, st, ps.err, t, u, v, tri,
                             cnt);
-    ps.hit[pid] = make_float4(t, u, v, bitsf((uint32_t)tri));
+    ps.hit[i] = make_float4(t, u, v, bitsf((uint32_t)tri));
   }
   if (kCount && ((blockIdx.x * kBlock) < n_act)) wave_count(ps.work, cnt[0], cnt[1]);
 }
@@ -552,8 +552,8 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
   const bool record = (kGuide != 0) && !(rp.flags & kFlagFinal);
   if (active) {
     pid = ps.queue[depth & 1][i];
-    float4 ro = ps.ray_o[pid], rd = ps.ray_d[pid], T4 = ps.T[pid], hit = ps.hit[pid];
-    uint2 rs = ps.rng[pid];
+    float4 ro = ps.ray_o[depth & 1][i], rd = ps.ray_d[depth & 1][i], T4 = ps.T[depth & 1][i], hit = ps.hit[i];
+    uint2 rs = ps.rng[depth & 1][i];
     uint32_t npix = rp.width * rp.height;
     rng.state = ((uint64_t)rs.y << 32) | rs.x;
     rng.inc = pcg_stream_inc(pid % npix);
@@ -877,11 +877,14 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
         }
       }
     }
-    // Path radiance and, for guiding, the per-hit contribution.
-    float4 L4 = ps.L[pid];
-    ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
-    ps.nhit[pid] = depth + 1u;
+    // Path radiance (read and written only when this hit adds to it) and, for
+    // guiding, the per-hit contribution.
+    if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+      float4 L4 = ps.L[pid];
+      ps.L[pid] = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, 0.0f);
+    }
     if (record && records_path(ps, pid)) {
+      ps.nhit[pid] = depth + 1u;
       size_t vi = (size_t)depth * ps.n + pid;
       ps.vB[vi] = make_float4(c.x, c.y, c.z, cos_out);
       ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
@@ -909,13 +912,6 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
         ps.mv[2 * st + (size_t)depth * ps.mlp_rec + pid / ps.mlp_stride] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
       }
     }
-    if (cont) {
-      V3 ndir = nd;
-      ps.ray_o[pid] = make_float4(p.x, p.y, p.z, wo_pdf);
-      ps.ray_d[pid] = make_float4(ndir.x, ndir.y, ndir.z, bitsf(meta_next));
-      ps.T[pid] = make_float4(Tn.x, Tn.y, Tn.z, 0.0f);
-    }
-    ps.rng[pid] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
   }
   uint32_t slot, sslot;
   block_append2(&ps.count[depth + 1u], cont, &ps.scount[depth], shadow, slot, sslot);
@@ -924,7 +920,15 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
     acc_flush(acc_alb, ps.grad_alb);
     acc_flush(acc_le, ps.grad_le);
   }
-  if (cont) ps.queue[(depth + 1u) & 1][slot] = pid;
+  if (cont) {
+    // the path's state moves to its slot in the next depth's queue
+    const uint32_t nb = (depth + 1u) & 1u;
+    ps.queue[nb][slot] = pid;
+    ps.ray_o[nb][slot] = make_float4(p.x, p.y, p.z, wo_pdf);
+    ps.ray_d[nb][slot] = make_float4(nd.x, nd.y, nd.z, bitsf(meta_next));
+    ps.T[nb][slot] = make_float4(Tn.x, Tn.y, Tn.z, 0.0f);
+    ps.rng[nb][slot] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+  }
   if (shadow) {
     float4* e = ps.shadow + 3u * (size_t)sslot;
     e[0] = make_float4(p.x, p.y, p.z, sdist);
@@ -952,7 +956,7 @@ k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
   if (i < n_sh) {
     float4 a = e[0], b = e[1];
     pid = fbits(b.w);
-    StackRef st = make_stack(lds, ps.deep, ps.n, pid, sc.max_stack);
+    StackRef st = make_stack(lds, ps.deep, ps.n, i, sc.max_stack);
     float t, u, v;
     int32_t tri;
     occ = traverse<true, kCount>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, a.w, st, ps.err, t, u, v, tri,
@@ -1416,11 +1420,10 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_infer(SceneView sc, PathSoA
     bool valid = i < n_act;
     MlpInput in{V3{0, 0, 0}, V3{0, 0, 1}, V3{0, 0, 1}, 0.0f};
     if (valid) {
-      uint32_t pid = ps.queue[depth & 1][i];
-      float4 hit = ps.hit[pid];
+      float4 hit = ps.hit[i];
       int32_t tri = (int32_t)fbits(hit.w);
       if (tri >= 0) {
-        float4 ro = ps.ray_o[pid], rd = ps.ray_d[pid];
+        float4 ro = ps.ray_o[depth & 1][i], rd = ps.ray_d[depth & 1][i];
         V3 o{ro.x, ro.y, ro.z}, d{rd.x, rd.y, rd.z};
         const float4* tq = reinterpret_cast<const float4*>(sc.tris + tri);
         V3 ng = normalize(cross(load3(tq[1]), load3(tq[2])));

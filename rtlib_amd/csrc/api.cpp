@@ -403,7 +403,7 @@ static void ensure_arena(rtpg_context_t* c, uint32_t n, uint32_t max_depth, bool
   c->p_q0.reserve(n);
   c->p_q1.reserve(n);
   c->p_shadow.reserve(3 * (size_t)n);
-  c->p_count.reserve(2 * (size_t)(max_depth + 2));
+  c->p_count.reserve(2 * (size_t)(max_depth + 2) * kQueueRegions);
   // spill stack: (entry, distance) pairs beyond the kShortStack LDS entries
   if (c->max_stack > kShortStack) c->p_deep.reserve(2 * (size_t)(c->max_stack - kShortStack) * n);
   if (guide) {
@@ -441,8 +441,14 @@ static PathSoA make_soa(rtpg_context_t* ctx, uint32_t n, uint32_t max_depth, boo
   ps.nhit = ctx->p_nhit.p;
   ps.queue[0] = ctx->p_q0.p;
   ps.queue[1] = ctx->p_q1.p;
+  // queue regions: slots per region a multiple of 1024 (every work-group of a
+  // queue kernel lies in one region), at most kQueueRegions of them
+  const uint32_t per = (n + kQueueRegions - 1u) / kQueueRegions;
+  ps.qreg = std::max(1024u, (per + 1023u) & ~1023u);
+  ps.nreg = (n + ps.qreg - 1u) / ps.qreg;
+  const uint32_t ncnt = (max_depth + 2u) * kQueueRegions;
   ps.count = ctx->p_count.p;
-  ps.scount = ctx->p_count.p + (max_depth + 2u);
+  ps.scount = ctx->p_count.p + ncnt;
   ps.shadow = ctx->p_shadow.p;
   ps.deep = ctx->max_stack > kShortStack ? ctx->p_deep.p : nullptr;
   ps.err = ctx->p_err.p;

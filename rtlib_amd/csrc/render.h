@@ -11,6 +11,13 @@ namespace rtpg {
 constexpr float kRayTmin = 1e-4f;   // SimpleTrace.cu:76
 // Traversal stack entries kept in LDS per lane; deeper entries spill to HBM.
 constexpr uint32_t kShortStack = 8;
+// The queues of live paths and shadow rays are split into
+// regions of PathSoA::qreg slots (a multiple of 1024), each with its own
+// counter per depth (count[depth * kQueueRegions + region]): a work-group
+// appends to the region that holds its own slots.  Appends to one counter
+// serialise (about 11 ns each on MI355X); with one counter per queue the
+// one-atomic-per-work-group appends of a 32M-path bounce took 0.7-1.4 ms.
+constexpr uint32_t kQueueRegions = 256;
 constexpr float kRayTmax = 1e20f;
 
 // Path meta word (ray_d.w): depth in bits 0..7, flags above.
@@ -35,10 +42,12 @@ struct PathSoA {
   uint2* rng[2];     // PCG32 state
   float4* hit;       // per slot of the current depth: t, u, v, leaf-order triangle (bits; 0xFFFFFFFF = miss)
   uint32_t* queue[2];  // path index (pid) per slot
+  uint32_t qreg;       // slots per queue region
+  uint32_t nreg;       // regions in use: ceil(n / qreg)
   float4* L;         // radiance, per path (pid)
   uint32_t* nhit;    // hits processed, per path
-  uint32_t* count;   // live paths per depth [max_depth + 2]
-  uint32_t* scount;  // shadow rays per depth [max_depth + 2]
+  uint32_t* count;   // live paths per depth and queue region [(max_depth + 2) x kQueueRegions]
+  uint32_t* scount;  // shadow rays per depth and queue region [(max_depth + 2) x kQueueRegions]
   float4* shadow;    // 3 x float4 per shadow ray
   float* moments;    // per pixel: sum and sum of squares of the samples' RGB mean, or NULL
   uint32_t* deep;    // traversal spill stack per ray slot: (entry, distance) pairs, 2 x (cap - kShortStack) x n words

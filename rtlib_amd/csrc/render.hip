@@ -5,9 +5,11 @@
 // (SimpleTrace.cu:45-103, 214-425, 625-630; SimpleGuide.cu:48-156, 287-511;
 // SimpleGuide2.cu:48-153, 285-523).  Paths live in structure-of-arrays HBM
 // buffers; between bounces the surviving paths are compacted into a queue with
-// one atomic per work-group (ballot + popcount, combined in LDS), so the work
+// one atomic per work-group (ballot + popcount, combined in LDS) on the
+// counter of the queue region that holds the work-group's slots, so the work
 // of each bounce is the live paths only; the grid of every bounce is sized
-// for the whole chunk and work-groups past the live count exit at once.
+// for the whole chunk and work-groups past their region's live count exit at
+// once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +42,20 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
 }
 
 __device__ __forceinline__ V3 load3(float4 v) { return V3{v.x, v.y, v.z}; }
+
+// The queue region of a work-group whose first slot is b0: its first slot
+// (base) and its live entries at this depth in counter array cnt.
+struct QueueRegion {
+  uint32_t base, live, index;
+};
+__device__ __forceinline__ QueueRegion queue_region(const PathSoA& ps, const uint32_t* cnt, uint32_t depth,
+                                                    uint32_t b0) {
+  QueueRegion q;
+  q.index = b0 / ps.qreg;
+  q.base = q.index * ps.qreg;
+  q.live = cnt[depth * kQueueRegions + q.index];
+  return q;
+}
 
 // Paths whose guide vertices are recorded: pid < rec_limit, or with the
 // learned guide (mlp_stride > 0) every mlp_stride-th path from mlp_phase
@@ -355,10 +371,11 @@ template <bool kCount>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
   __shared__ uint2 lds[kShortStack][kBlock];
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t n_act = ps.count[depth];
+  const uint32_t b0 = blockIdx.x * kBlock, i = b0 + threadIdx.x;
+  const QueueRegion q = queue_region(ps, ps.count, depth, b0);
+  if (b0 - q.base >= q.live) return;  // uniform over the work-group
   uint32_t cnt[2] = {0u, 0u};
-  if (i < n_act) {
+  if (i - q.base < q.live) {
     float4 a = ps.ray_o[depth & 1][i], b = ps.ray_d[depth & 1][i];
     StackRef st = make_stack(lds, ps.deep, ps.n, i, sc.max_stack);
     float t, u, v;
@@ -367,7 +384,7 @@ k_extend(SceneView sc, PathSoA ps, uint32_t depth) {
                             cnt);
     ps.hit[i] = make_float4(t, u, v, bitsf((uint32_t)tri));
   }
-  if (kCount && ((blockIdx.x * kBlock) < n_act)) wave_count(ps.work, cnt[0], cnt[1]);
+  if (kCount) wave_count(ps.work, cnt[0], cnt[1]);
 }
 
 __device__ __forceinline__ uint32_t pick_light(const SceneView& sc, float u) {
@@ -534,14 +551,15 @@ __attribute__((amdgpu_waves_per_eu(shade_min_waves<kGuide, kAdj, kRis, kTex>()))
 k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView hg, MortonView mq, uint32_t depth) {
   __shared__ LdsAcc3<kAdj ? 1024u : 1u> acc_alb;
   __shared__ LdsAcc3<kAdj ? 256u : 1u> acc_le;
+  const uint32_t b0 = blockIdx.x * kShadeBlock, i = b0 + threadIdx.x;
+  const QueueRegion q = queue_region(ps, ps.count, depth, b0);
+  if (b0 - q.base >= q.live) return;  // uniform over the work-group
   if (kAdj) {
     acc_init(acc_alb);
     acc_init(acc_le);
     __syncthreads();
   }
-  uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
-  uint32_t n_act = ps.count[depth];
-  bool active = i < n_act;
+  bool active = i - q.base < q.live;
   bool cont = false, shadow = false;
   uint32_t pid = 0;
   V3 p{0, 0, 0}, nd{0, 0, 0}, sdir{0, 0, 0}, scon{0, 0, 0}, Tn{0, 0, 0};
@@ -914,7 +932,10 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
     }
   }
   uint32_t slot, sslot;
-  block_append2(&ps.count[depth + 1u], cont, &ps.scount[depth], shadow, slot, sslot);
+  block_append2(&ps.count[(depth + 1u) * kQueueRegions + q.index], cont, &ps.scount[depth * kQueueRegions + q.index],
+                shadow, slot, sslot);
+  slot += q.base;
+  sslot += q.base;
   if (kAdj) {
     __syncthreads();
     acc_flush(acc_alb, ps.grad_alb);
@@ -943,17 +964,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
 k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
   __shared__ uint2 lds[kShortStack][kBlock];
   __shared__ LdsAcc3<kAdj ? 256u : 1u> acc_le;
+  const uint32_t b0 = blockIdx.x * kBlock, i = b0 + threadIdx.x;
+  const QueueRegion q = queue_region(ps, ps.scount, depth, b0);
+  if (b0 - q.base >= q.live) return;  // uniform over the work-group
   if (kAdj) {
     acc_init(acc_le);
     __syncthreads();
   }
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t n_sh = ps.scount[depth];
   uint32_t cnt[2] = {0u, 0u};
   bool occ = true;
   uint32_t pid = 0;
   const float4* e = ps.shadow + 3u * (size_t)i;
-  if (i < n_sh) {
+  if (i - q.base < q.live) {
     float4 a = e[0], b = e[1];
     pid = fbits(b.w);
     StackRef st = make_stack(lds, ps.deep, ps.n, i, sc.max_stack);
@@ -962,7 +984,7 @@ k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
     occ = traverse<true, kCount>(sc, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, kRayTmin, a.w, st, ps.err, t, u, v, tri,
                                  cnt);
   }
-  if (kCount && ((blockIdx.x * kBlock) < n_sh)) wave_count(ps.work + 2, cnt[0], cnt[1]);
+  if (kCount) wave_count(ps.work + 2, cnt[0], cnt[1]);
   if (!occ) {
     float4 c = e[2];  // contribution, loaded after the traversal to keep it out of its registers
     float4 L4 = ps.L[pid];
@@ -984,7 +1006,11 @@ k_shadow(SceneView sc, PathSoA ps, uint32_t depth, uint32_t record) {
   }
 }
 
-__global__ void k_set_count(uint32_t* c, uint32_t n) { c[0] = n; }
+// Depth 0: region k holds the camera paths of its slots.
+__global__ void k_set_count(uint32_t* c, uint32_t n, uint32_t qreg, uint32_t nreg) {
+  uint32_t k = threadIdx.x;
+  if (k < nreg) c[k] = min(qreg, n - k * qreg);
+}
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1409,15 +1435,16 @@ constexpr uint32_t kInferGrid = 2048;
 __global__ void __launch_bounds__(kInferBlock) k_mlp_infer(SceneView sc, PathSoA ps, MlpView m, uint32_t depth) {
   __shared__ uint16_t lw[kLdsWeights];
   __shared__ float hs[4][16 * kHeadPitch];
-  const uint32_t n_act = ps.count[depth];
-  if (blockIdx.x * 64u >= n_act) return;  // uniform over the work-group
+  const uint32_t nq = ps.nreg * ps.qreg;
   mlp_load_weights(lw, m.w);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15;
   float* row = hs[wave] + r * kHeadPitch;
-  for (uint32_t tile = blockIdx.x; tile * 64u < n_act; tile += gridDim.x) {
+  for (uint32_t tile = blockIdx.x; tile * 64u < nq; tile += gridDim.x) {
+    const QueueRegion q = queue_region(ps, ps.count, depth, tile * 64u);
+    if (tile * 64u - q.base >= q.live) continue;  // uniform over the work-group
     uint32_t i = tile * 64u + 16u * (uint32_t)wave + (uint32_t)r;
-    bool valid = i < n_act;
+    bool valid = i - q.base < q.live;
     MlpInput in{V3{0, 0, 0}, V3{0, 0, 1}, V3{0, 0, 1}, 0.0f};
     if (valid) {
       float4 hit = ps.hit[i];
@@ -1509,24 +1536,42 @@ __global__ void __launch_bounds__(kBlock) k_mlp_emit(PathSoA ps, uint32_t max_de
 }
 
 // Ray counters of a chunk: paths, radiance rays (live paths per depth) and
-// shadow rays, accumulated in 64 bits.
-__global__ void k_stats_add(const uint32_t* count, const uint32_t* scount, uint32_t depth, uint32_t n,
-                            unsigned long long* stats) {
-  if (threadIdx.x != 0) return;
+// shadow rays, summed over the queue regions and accumulated in 64 bits.
+__global__ void __launch_bounds__(kQueueRegions) k_stats_add(const uint32_t* count, const uint32_t* scount,
+                                                              uint32_t depth, uint32_t n,
+                                                              unsigned long long* stats) {
+  __shared__ unsigned long long part[2][kQueueRegions / 64];
+  const uint32_t k = threadIdx.x;
   unsigned long long r = 0, sh = 0;
   for (uint32_t d = 0; d < depth; ++d) {
-    r += count[d];
-    sh += scount[d];
+    r += count[d * kQueueRegions + k];
+    sh += scount[d * kQueueRegions + k];
   }
-  stats[0] += n;
-  stats[1] += r;
-  stats[2] += sh;
+  for (int off = 32; off >= 1; off >>= 1) {
+    r += __shfl_xor(r, off);
+    sh += __shfl_xor(sh, off);
+  }
+  if (lane_id() == 0) {
+    part[0][k >> 6] = r;
+    part[1][k >> 6] = sh;
+  }
+  __syncthreads();
+  if (k == 0) {
+    unsigned long long tr = 0, ts = 0;
+    for (uint32_t w = 0; w < kQueueRegions / 64; ++w) {
+      tr += part[0][w];
+      ts += part[1][w];
+    }
+    stats[0] += n;
+    stats[1] += tr;
+    stats[2] += ts;
+  }
 }
 
 // ------------------------------------------------------------------ host launchers
 hipError_t launch_stats_add(const uint32_t* count, const uint32_t* scount, uint32_t depth, uint32_t n,
                             unsigned long long* stats, hipStream_t s) {
-  hipLaunchKernelGGL(k_stats_add, dim3(1), dim3(64), 0, s, count, scount, depth, n, stats);
+  hipLaunchKernelGGL(k_stats_add, dim3(1), dim3(kQueueRegions), 0, s, count, scount, depth, n, stats);
   return hipGetLastError();
 }
 
@@ -1543,12 +1588,14 @@ hipError_t launch_trace_rays(const SceneView& sc, const float4* rays, uint32_t n
 hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const SceneView& sc, PathSoA ps,
                                const SdTreeView& sd, const HashGridView& hg, const MortonView& mq, const MlpView& ml,
                                int guide, float* accum, hipStream_t s, hipEvent_t* ev) {
-  uint32_t nb = blocks_for(ps.n, kBlock);
-  uint32_t ncnt = 2u * (rp.max_depth + 2u);
+  // queue kernels cover the slots of every region in use
+  const uint32_t nq = ps.nreg * ps.qreg;
+  uint32_t nb = blocks_for(nq, kBlock);
+  uint32_t ncnt = 2u * (rp.max_depth + 2u) * kQueueRegions;
   hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(ncnt, 64)), dim3(64), 0, s, ps.count, ncnt);
-  hipLaunchKernelGGL(k_camera, dim3(nb), dim3(kBlock), 0, s, rp, cam, ps);
-  // count[0] = n: written by a one-thread kernel to stay stream ordered.
-  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, ps.count, ps.n);
+  hipLaunchKernelGGL(k_camera, dim3(blocks_for(ps.n, kBlock)), dim3(kBlock), 0, s, rp, cam, ps);
+  // depth-0 counts of the regions: written by a kernel to stay stream ordered
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(kQueueRegions), 0, s, ps.count, ps.n, ps.qreg, ps.nreg);
   const bool record = guide != 0 && !(rp.flags & kFlagFinal);
   const bool count = ps.work != nullptr;
   // ev: per kernel class k in [0, kTimedKernels): pairs of events (begin, end)
@@ -1567,7 +1614,7 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     mark(1, d, 0);
     const bool adjoint = ps.grad_alb != nullptr;
     const bool ris = (rp.flags & kFlagRis) != 0;
-    const uint32_t nbs = blocks_for(ps.n, kShadeBlock);
+    const uint32_t nbs = blocks_for(nq, kShadeBlock);
     const bool tex = sc.tattr != nullptr || sc.vnorm != nullptr;
 #define RTPG_SHADE_T(G, A, R)                                                                                       \
   if (tex) {                                                                                                        \
@@ -1592,7 +1639,7 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     } else if (guide == 3) {
       // the learned guide's inference pass is timed with the shade class
       if ((rp.flags & kFlagGuideSample) && d < ps.mlp_guide_depth && !ps.mcache) {
-        uint32_t ni = std::min(blocks_for(ps.n, 64u), kInferGrid);
+        uint32_t ni = std::min(blocks_for(nq, 64u), kInferGrid);
         hipLaunchKernelGGL(k_mlp_infer, dim3(ni), dim3(kInferBlock), 0, s, sc, ps, ml, d);
       }
       RTPG_SHADE(3, false)

@@ -1305,45 +1305,6 @@ __device__ __forceinline__ uint32_t morton_key(const MortonView& mq, uint32_t ce
 // positive finite value not dropped by the directional box filter, cu < 0);
 // kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285, which tests
 // no value).
-// kind 1: SD-tree (DTreeT::RecordIrradiance :419-427 -> DTreeNodeT::Record
-// :113-125: weight += 1, flux into the first leaf slot on the descent, for a
-// positive finite value not dropped by the directional box filter, cu < 0);
-// kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285, which tests
-// no value).
-// Combines the lanes of the wave that hold the same key, for the first
-// `rounds` distinct keys in lane order: each group's first lane gets the
-// group's count (n) and sum (v) and returns true, the others false; the lanes
-// of keys beyond those rounds keep their own (n = 1) and return true.  Lanes
-// of neighbouring pixels at one depth mostly share a DTree and its near-root
-// leaves, whose LDS atomics would otherwise serialise on one address.  All
-// lanes of the wave must call it.
-__device__ __forceinline__ bool wave_combine(bool active, uint32_t key, float& v, uint32_t& n, int rounds) {
-  uint64_t pending = __ballot(active);
-  const uint32_t lane = lane_id();
-  bool lead = active;
-  n = 1u;
-  for (int r = 0; r < rounds && pending; ++r) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t k = __shfl(key, leader);
-    const bool mine = active && key == k;
-    const uint64_t same = __ballot(mine);
-    float part = mine ? v : 0.0f;
-    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
-    if (mine) {
-      lead = lane == (uint32_t)leader;
-      v = part;
-      n = (uint32_t)__popcll(same);
-    }
-    pending &= ~same;
-  }
-  return lead;
-}
-
-// kind 1: SD-tree (DTreeT::RecordIrradiance :419-427 -> DTreeNodeT::Record
-// :113-125: weight += 1, flux into the first leaf slot on the descent, for a
-// positive finite value not dropped by the directional box filter, cu < 0);
-// kind 2: Morton quadtree leaf (MortonQuadTreeT::Record :262-285, which tests
-// no value).
 template <int kKind>
 __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps, uint32_t max_vtx, SdTreeView sd,
                                                                      MortonView mq) {
@@ -1476,17 +1437,29 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_infer(SceneView sc, PathSoA
 // vertex d from its sampled direction is the sum of the contributions of the
 // later hits divided by the throughput after the bounce (per channel, guarded
 // as in the SD-tree record); the sample weight is avg_rgb(f |cos| / p_sample *
-// Li).  Appended with one atomic per wave; samples beyond the capacity are
-// dropped.
-__global__ void __launch_bounds__(kBlock) k_mlp_emit(PathSoA ps, uint32_t max_depth, uint32_t max_vtx) {
-  uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+// Li).  Appended with one atomic per work-group and depth (the samples of a
+// work-group's 16 waves combined in LDS: appends to one counter serialise);
+// samples beyond the capacity are dropped.
+constexpr uint32_t kEmitBlock = 1024;
+__global__ void __launch_bounds__(kEmitBlock) k_mlp_emit(PathSoA ps, uint32_t max_depth, uint32_t max_vtx) {
+  constexpr uint32_t kWaves = kEmitBlock / 64;
+  __shared__ uint32_t s_n[kWaves + 1];
+  __shared__ uint32_t s_max;
+  uint32_t slot = blockIdx.x * kEmitBlock + threadIdx.x;
   uint32_t pid = slot * ps.mlp_stride + ps.mlp_phase;
   bool in = slot < ps.mlp_rec && pid < ps.n;
   uint32_t nh = in ? ps.nhit[pid] : 0u;
   uint32_t nv = nh > 0 ? nh - 1u : 0u;
   nv = nv < max_vtx ? nv : max_vtx;
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  // the deepest path of the work-group bounds its (uniform) depth loop
   uint32_t nh_max = nh;
   for (int off = 32; off >= 1; off >>= 1) nh_max = max(nh_max, (uint32_t)__shfl_xor((int)nh_max, off));
+  if (threadIdx.x == 0) s_max = 0u;
+  __syncthreads();
+  if (lane == 0) atomicMax(&s_max, nh_max);
+  __syncthreads();
+  nh_max = s_max;
   const size_t st = (size_t)max_depth * ps.mlp_rec;
   float rx = 0.0f, ry = 0.0f, rz = 0.0f;
   for (int d = (int)nh_max - 2; d >= 0; --d) {
@@ -1516,18 +1489,25 @@ __global__ void __launch_bounds__(kBlock) k_mlp_emit(PathSoA ps, uint32_t max_de
         }
       }
     }
+    // work-group append: per-wave counts, one atomic
     uint64_t mask = __ballot(ok);
-    if (!mask) continue;
-    uint32_t lane = lane_id();
-    uint32_t base = 0;
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    if (lane == leader) base = atomicAdd(ps.mlp_count, (uint32_t)__popcll(mask));
-    base = (uint32_t)__shfl((int)base, (int)leader);
-    if (!ok) continue;
-    uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (slot >= ps.mlp_cap) continue;
+    if (lane == 0) s_n[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (uint32_t k = 0; k < kWaves; ++k) {
+        uint32_t c = s_n[k];
+        s_n[k] = t;
+        t += c;
+      }
+      s_n[kWaves] = t ? atomicAdd(ps.mlp_count, t) : 0u;
+    }
+    __syncthreads();
+    const uint32_t o_slot = s_n[kWaves] + s_n[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();  // s_n is rewritten by the next depth
+    if (!ok || o_slot >= ps.mlp_cap) continue;
     float4 ma = ps.mv[mi], mb = ps.mv[st + mi], md = ps.mv[3 * st + mi];
-    float4* o = ps.mlp_out + 4u * (size_t)slot;
+    float4* o = ps.mlp_out + 4u * (size_t)o_slot;
     o[0] = ma;
     o[1] = mb;
     o[2] = make_float4(mc.x, mc.y, mc.z, w);
@@ -1676,8 +1656,8 @@ hipError_t launch_render_chunk(const RenderParams& rp, const Camera& cam, const 
     }
   }
   if (record && guide == 3) {
-    uint32_t ne = blocks_for(ps.mlp_rec, kBlock);
-    hipLaunchKernelGGL(k_mlp_emit, dim3(ne), dim3(kBlock), 0, s, ps, rp.max_depth, rp.max_depth - 1u);
+    uint32_t ne = blocks_for(ps.mlp_rec, kEmitBlock);
+    hipLaunchKernelGGL(k_mlp_emit, dim3(ne), dim3(kEmitBlock), 0, s, ps, rp.max_depth, rp.max_depth - 1u);
   }
   mark(4, 0, 1);
   return hipGetLastError();

@@ -162,7 +162,7 @@ struct SdTreeView {
   const uint32_t* leaf_depth;
 };
 
-constexpr uint32_t kPdfJumpBits = 4;
+constexpr uint32_t kPdfJumpBits = 5;
 constexpr uint32_t kPdfJumpCells = 1u << (2 * kPdfJumpBits);
 constexpr uint32_t kJumpDone = 0xFFFFFFFFu;
 

@@ -482,6 +482,10 @@ def extend_roofline(W, H, per_kernel, launches, ext_bytes, names, plain_kernel_m
         r["frac"] = round(valu / avg_s / 1e9 / VALU_PEAK_GWIS, 4)
         r["valu_per_launch"] = valu
         r["valu_per_wave"] = round(valu / e["SQ_WAVES"], 1)
+        if "ta_busy_frac" in e:
+            # the texture addresser that issues its node and triangle loads:
+            # busy fraction over the counter run's dispatches (co-bound)
+            r["ta_busy_frac"] = e["ta_busy_frac"]
         traffic = e["FETCH_SIZE"] * 1024 * pmc["fetch_scale"] + e["WRITE_SIZE"] * 1024 * pmc["write_scale"]
         r["traffic"] = round(traffic)
         r["hbm"] = {"gbs": round(traffic / avg_s / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
@@ -518,6 +522,8 @@ def per_kernel_roofline(W, H, names, per_kernel, launches, algo, t_step):
             e["hbm_frac"] = round(hbm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None
             if "active_lanes_per_valu" in k:
                 e["active_lanes_per_valu"] = k["active_lanes_per_valu"]
+            if "ta_busy_frac" in k:
+                e["ta_busy_frac"] = k["ta_busy_frac"]
         out[n] = e
     frame = {"ms_per_step": round(t_step * 1e3, 2), "algorithmic_bytes_per_step": round(tot_algo),
              "algorithmic_gbs": round(tot_algo / t_step / 1e9, 1),

@@ -144,115 +144,179 @@ __device__ __forceinline__ bool dtree_box_jitter(const SdTreeView& g, uint32_t c
   return u >= 0.0f && u < 1.0f && v >= 0.0f && v < 1.0f;
 }
 
-// DTreeNodeT::Pdf (:278-316) times 1/(4 pi) as in DTreeT::Pdf (:484-489):
-// density on the sphere of the sampling tree of DTree `cell`.
-__device__ __forceinline__ float dtree_pdf(const SdTreeView& g, uint32_t cell, float px, float py) {
-#pragma clang fp contract(off)
-  float result = 1.0f;
-  uint32_t node = 0, base;
-  if (g.samp_jump) {
-    // the first kPdfJumpBits levels (and the mean-0 case) from the jump
-    // table: their child indices are the top bits of (u, v), and
-    // kPdfJumpBits doublings of u minus those bits leave exactly
-    // u 2^kPdfJumpBits - cell (as in stree_lookup)
-    const float sn = (float)(1u << kPdfJumpBits);
-    float gx = px * sn, gy = py * sn;
-    float cx = fminf(fmaxf(floorf(gx), 0.0f), sn - 1.0f);
-    float cy = fminf(fmaxf(floorf(gy), 0.0f), sn - 1.0f);
-    px = gx - cx;
-    py = gy - cy;
-    uint4 e = g.samp_jump[(size_t)cell * kPdfJumpCells + ((uint32_t)cy << kPdfJumpBits) + (uint32_t)cx];
-    result = bitsf(e.x);
-    if (e.y == kJumpDone) return result * kInv2Pi / 2.0f;
-    node = e.y;
-    base = e.z;
+// One density walk of a DTree's sampling tree (DTreeNodeT::Pdf :278-316,
+// times 1/(4 pi) as in DTreeT::Pdf :484-489): the state kept between the
+// loads of its descent, so that two descents can share their load rounds.
+struct DPdfWalk {
+  uint32_t base, node;
+  float result, px, py;
+  int idx;
+  bool live;
+};
+
+// Start of a walk from the first word of its DTree: with the jump table, the
+// entry j read at (px, py) (first kPdfJumpBits levels and the mean-0 case:
+// their child indices are the top bits of (u, v), and kPdfJumpBits doublings
+// of u minus those bits leave exactly u 2^kPdfJumpBits - cell, as in
+// stree_lookup; dpdf_jump_slot has already rescaled px, py); without it, the
+// DTree's info word.
+RTPG_HD DPdfWalk dpdf_begin(bool jump, uint4 j, float px, float py) {
+  DPdfWalk w;
+  w.node = 0u;
+  w.px = px;
+  w.py = py;
+  w.idx = 0;
+  if (jump) {
+    w.result = bitsf(j.x);
+    w.live = j.y != kJumpDone;
+    w.node = w.live ? j.y : 0u;
+    w.base = j.z;
   } else {
-    float4 inf = g.info[cell];
-    if (dtree_mean(inf.y, inf.z) <= 0.0f) return 1.0f * kInv2Pi / 2.0f;
-    base = fbits(inf.x);
+    w.result = 1.0f;
+    w.live = !(dtree_mean(bitsf(j.y), bitsf(j.z)) <= 0.0f);
+    w.base = j.x;
   }
-  int idx = dtree_child_idx(px, py);
-  for (;;) {
-    uint4 r0 = g.samp_node[2u * (base + node)], r1 = g.samp_node[2u * (base + node) + 1u];
-    float4 s = make_float4(bitsf(r0.x), bitsf(r0.y), bitsf(r0.z), bitsf(r0.w));
-    float total = s.x + s.y + s.z + s.w;
-    float si = f4c(s, idx);
-    if (si <= 0.0f) {
-      result = 0.0f;
-      break;
-    }
-    result *= 4.0f * si / total;
-    uint32_t ch = dnode_child(make_uint2(r1.x, r1.y), idx);
-    if (ch == 0) break;
-    node = ch;
-    idx = dtree_child_idx(px, py);
-  }
-  return result * kInv2Pi / 2.0f;
+  if (w.live) w.idx = dtree_child_idx(w.px, w.py);
+  return w;
 }
 
-// DTreeNodeT::Sample (:228-277) with the clamp of DTreeT::Sample (:477-482):
-// canonical direction drawn from the sampling tree of DTree `cell`.  pdf: the
-// sphere density of the drawn point as DTreeNodeT::SampleAndPdf (:178-227)
-// accumulates it along the same descent (4 s_i / sum per level, 1 / (4 pi)).
-__device__ __forceinline__ void dtree_sample(const SdTreeView& g, uint32_t cell, Pcg32& rng,
-                                             float& ox, float& oy, float& pdf) {
+// Index of the jump entry of (px, py) in DTree `cell`; rescales px, py.
+RTPG_HD size_t dpdf_jump_slot(uint32_t cell, float& px, float& py) {
 #pragma clang fp contract(off)
-  float4 inf = g.info[cell];
-  if (dtree_mean(inf.y, inf.z) <= 0.0f) {
-    ox = rnd(rng);
-    oy = rnd(rng);
-    pdf = 1.0f * kInv2Pi / 2.0f;
+  const float sn = (float)(1u << kPdfJumpBits);
+  float gx = px * sn, gy = py * sn;
+  float cx = fminf(fmaxf(floorf(gx), 0.0f), sn - 1.0f);
+  float cy = fminf(fmaxf(floorf(gy), 0.0f), sn - 1.0f);
+  px = gx - cx;
+  py = gy - cy;
+  return (size_t)cell * kPdfJumpCells + ((uint32_t)cy << kPdfJumpBits) + (uint32_t)cx;
+}
+
+// One level of a walk, given its node's sums r0 and child words r1.
+RTPG_HD void dpdf_step(DPdfWalk& w, uint4 r0, uint2 r1) {
+#pragma clang fp contract(off)
+  float4 s = make_float4(bitsf(r0.x), bitsf(r0.y), bitsf(r0.z), bitsf(r0.w));
+  float total = s.x + s.y + s.z + s.w;
+  float si = f4c(s, w.idx);
+  if (si <= 0.0f) {
+    w.result = 0.0f;
+    w.live = false;
     return;
   }
-  float result = 1.0f;
-  uint32_t base = fbits(inf.x);
-  uint32_t node = 0;
-  float rx = 0.0f, ry = 0.0f, size = 1.0f;
-  for (;;) {
-    uint4 r0 = g.samp_node[2u * (base + node)], r1 = g.samp_node[2u * (base + node) + 1u];
-    float4 s = make_float4(bitsf(r0.x), bitsf(r0.y), bitsf(r0.z), bitsf(r0.w));
-    int idx = 0;
-    float partial = s.x + s.z;
-    float total = s.x + s.y + s.z + s.w;
-    float boundary = partial / total;
-    float ox0 = 0.0f, oy0 = 0.0f;
-    float sample = rnd(rng);
-    if (sample < boundary) {
-      sample = sample / boundary;
-      boundary = s.x / partial;
-    } else {
-      partial = total - partial;
-      ox0 = 0.5f;
-      sample = (sample - boundary) / (1.0f - boundary);
-      boundary = s.y / partial;
-      idx |= 1;
-    }
-    if (sample >= boundary) {
-      oy0 = 0.5f;
-      idx |= 2;
-    }
-    float t0 = size * ox0;
-    float t1 = size * oy0;
-    rx = rx + t0;
-    ry = ry + t1;
-    size *= 0.5f;
-    float si = f4c(s, idx);
-    result = si > 0.0f ? result * (4.0f * si / total) : 0.0f;
-    uint32_t ch = dnode_child(make_uint2(r1.x, r1.y), idx);
-    if (ch == 0 || si <= 0.0f) {
-      float a = rnd(rng);
-      float b = rnd(rng);
-      float t2 = size * a;
-      float t3 = size * b;
-      rx = rx + t2;
-      ry = ry + t3;
-      break;
-    }
-    node = ch;
+  w.result *= 4.0f * si / total;
+  uint32_t ch = dnode_child(r1, w.idx);
+  if (ch == 0) {
+    w.live = false;
+    return;
   }
-  ox = fminf(fmaxf(rx, 0.0f), 1.0f);
-  oy = fminf(fmaxf(ry, 0.0f), 1.0f);
-  pdf = result * kInv2Pi / 2.0f;
+  w.node = ch;
+  w.idx = dtree_child_idx(w.px, w.py);
+}
+
+// Two descents of DTree `cell`'s sampling tree in one loop: every round
+// issues the node loads of both before either is used, so a shading point
+// waits for the longer descent rather than for the sum of the two.
+//  - chain a, with a_sample set: DTreeNodeT::Sample (:228-277) with the clamp
+//    of DTreeT::Sample (:477-482): (ax, ay) <- a canonical direction, apdf <-
+//    the sphere density of the drawn point as DTreeNodeT::SampleAndPdf
+//    (:178-227) accumulates it along the same descent (4 s_i / sum per level,
+//    1 / (4 pi)); one random number per level and two at the end, in order;
+//  - chain a, without a_sample: apdf <- the density at (ax, ay);
+//  - chain b, when b_on: bpdf <- the density at (bx, by).
+// Each chain performs the operations of the descent it stands for in the
+// same order, so its result does not depend on the other chain.
+__device__ __forceinline__ void dtree_pair(const SdTreeView& g, uint32_t cell, bool a_sample, Pcg32& rng,
+                                           float& ax, float& ay, float& apdf, bool b_on, float bx, float by,
+                                           float& bpdf) {
+#pragma clang fp contract(off)
+  const bool jump = g.samp_jump != nullptr;
+  const uint4* info = reinterpret_cast<const uint4*>(g.info + cell);
+  float apx = ax, apy = ay;
+  const uint4* pa = (a_sample || !jump) ? info : g.samp_jump + dpdf_jump_slot(cell, apx, apy);
+  const uint4* pb = jump ? g.samp_jump + dpdf_jump_slot(cell, bx, by) : info;
+  uint4 ja = *pa;
+  uint4 jb = b_on ? *pb : make_uint4(0u, kJumpDone, 0u, 0u);
+  DPdfWalk b = dpdf_begin(jump, jb, bx, by);
+  b.live = b.live && b_on;
+  // chain a: a density walk, or the sampling descent's state
+  DPdfWalk a = dpdf_begin(a_sample ? false : jump, ja, apx, apy);
+  float rx = 0.0f, ry = 0.0f, size = 1.0f;
+  bool uniform = false;
+  if (a_sample) {
+    a.idx = 0;
+    a.result = 1.0f;
+    if (!a.live) {
+      // an empty DTree samples the square uniformly
+      ax = rnd(rng);
+      ay = rnd(rng);
+      uniform = true;
+    }
+  }
+  while (a.live || b.live) {
+    uint4 a0 = make_uint4(0u, 0u, 0u, 0u), b0 = a0;
+    uint2 a1 = make_uint2(0u, 0u), b1 = a1;
+    if (a.live) {
+      a0 = g.samp_node[2u * (a.base + a.node)];
+      a1 = *reinterpret_cast<const uint2*>(g.samp_node + 2u * (a.base + a.node) + 1u);
+    }
+    if (b.live) {
+      b0 = g.samp_node[2u * (b.base + b.node)];
+      b1 = *reinterpret_cast<const uint2*>(g.samp_node + 2u * (b.base + b.node) + 1u);
+    }
+    if (b.live) dpdf_step(b, b0, b1);
+    if (a.live) {
+      if (!a_sample) {
+        dpdf_step(a, a0, a1);
+      } else {
+        float4 s = make_float4(bitsf(a0.x), bitsf(a0.y), bitsf(a0.z), bitsf(a0.w));
+        int idx = 0;
+        float partial = s.x + s.z;
+        float total = s.x + s.y + s.z + s.w;
+        float boundary = partial / total;
+        float ox0 = 0.0f, oy0 = 0.0f;
+        float sample = rnd(rng);
+        if (sample < boundary) {
+          sample = sample / boundary;
+          boundary = s.x / partial;
+        } else {
+          partial = total - partial;
+          ox0 = 0.5f;
+          sample = (sample - boundary) / (1.0f - boundary);
+          boundary = s.y / partial;
+          idx |= 1;
+        }
+        if (sample >= boundary) {
+          oy0 = 0.5f;
+          idx |= 2;
+        }
+        float t0 = size * ox0;
+        float t1 = size * oy0;
+        rx = rx + t0;
+        ry = ry + t1;
+        size *= 0.5f;
+        float si = f4c(s, idx);
+        a.result = si > 0.0f ? a.result * (4.0f * si / total) : 0.0f;
+        uint32_t ch = dnode_child(a1, idx);
+        if (ch == 0 || si <= 0.0f) {
+          float u = rnd(rng);
+          float v = rnd(rng);
+          float t2 = size * u;
+          float t3 = size * v;
+          rx = rx + t2;
+          ry = ry + t3;
+          a.live = false;
+        } else {
+          a.node = ch;
+        }
+      }
+    }
+  }
+  if (a_sample && !uniform) {
+    ax = fminf(fmaxf(rx, 0.0f), 1.0f);
+    ay = fminf(fmaxf(ry, 0.0f), 1.0f);
+  }
+  apdf = a.result * kInv2Pi / 2.0f;
+  bpdf = b.result * kInv2Pi / 2.0f;
 }
 
 // ---------------------------------------------------------------- hash grid
@@ -331,38 +395,62 @@ __device__ __forceinline__ uint32_t grid_find_cur(const HashGridView& h, uint32_
 // Leaf offset of level L in the pyramid: (4^L - 1) / 3.
 RTPG_HD uint32_t mq_level_offset(uint32_t level) { return ((1u << (2u * level)) - 1u) / 3u; }
 
-// MortonQuadTreeT::Pdf (OPX7UtilsMorton.h:401-414) times 1/(4 pi) (wrapper :515-523).
-__device__ __forceinline__ float mq_pdf(const MortonView& m, uint32_t cell, float px, float py) {
-  const float* w = m.samp + (size_t)m.per_cell * cell;
-  float root = w[0];
-  if (root <= 0.0f) return 1.0f * 0.25f * kInvPi;
-  uint32_t nn = 1u << m.level;
+// A point of [0,1) from an index (the top 24 bits of i * mult), for the
+// self-checks of the sampling entry points.
+RTPG_HD float check_coord(uint32_t i, uint32_t mult) { return (float)((i * mult) >> 8) * 0x1p-24f; }
+
+// Leaf bin of canonical (px, py) in a level-`level` Morton pyramid.
+RTPG_HD uint32_t mq_leaf(uint32_t level, float px, float py) {
+  uint32_t nn = 1u << level;
   uint32_t x = (uint32_t)(px * (float)nn);
   uint32_t y = (uint32_t)(py * (float)nn);
   x = x < nn ? x : nn - 1u;
   y = y < nn ? y : nn - 1u;
-  uint32_t code = morton2(x, y);
-  float leaf = w[mq_level_offset(m.level) + code];
-  return leaf / root * (float)(nn * nn) * 0.25f * kInvPi;
+  return mq_level_offset(level) + morton2(x, y);
 }
 
-// MortonQuadTreeT::SampleAndPdf (:286-344); returns canonical coordinates and
-// the density on the sphere.
-__device__ __forceinline__ void mq_sample(const MortonView& m, uint32_t cell, Pcg32& rng,
-                                          float& ox, float& oy, float& pdf) {
+// Two uses of cell `cell`'s pyramid with their loads issued together (the
+// root, both leaves and the first sampling level in one round):
+//  - chain a, with a_sample set: MortonQuadTreeT::SampleAndPdf (:286-344),
+//    (ax, ay) <- canonical coordinates, apdf <- the density on the sphere of
+//    the sampled bin; one random number per level and two at the end;
+//  - chain a, without a_sample: MortonQuadTreeT::Pdf (:401-414) at (ax, ay)
+//    times 1/(4 pi) (wrapper :515-523);
+//  - chain b, when b_on: the same density at (bx, by).
+__device__ __forceinline__ void mq_pair(const MortonView& m, uint32_t cell, bool a_sample, Pcg32& rng, float& ax,
+                                        float& ay, float& apdf, bool b_on, float bx, float by, float& bpdf) {
 #pragma clang fp contract(off)
   const float* w = m.samp + (size_t)m.per_cell * cell;
-  float total = w[0];
+  const uint32_t nn = 1u << m.level;
+  const float root = w[0];
+  const float lb = b_on ? w[mq_leaf(m.level, bx, by)] : 0.0f;
+  const float la = a_sample ? 0.0f : w[mq_leaf(m.level, ax, ay)];
+  float s1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (a_sample && m.level > 0u) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s1[k] = w[1 + k];
+  }
+  bpdf = root <= 0.0f ? 1.0f * 0.25f * kInvPi : lb / root * (float)(nn * nn) * 0.25f * kInvPi;
+  if (!a_sample) {
+    apdf = root <= 0.0f ? 1.0f * 0.25f * kInvPi : la / root * (float)(nn * nn) * 0.25f * kInvPi;
+    return;
+  }
+  float total = root;
   if (total <= 0.0f) {
-    ox = rnd(rng);
-    oy = rnd(rng);
-    pdf = 1.0f * 0.25f * kInvPi;
+    ax = rnd(rng);
+    ay = rnd(rng);
+    apdf = 1.0f * 0.25f * kInvPi;
     return;
   }
   uint32_t size = 4, offset = 1, code = 0;
   for (uint32_t i = 0; i < m.level; ++i) {
-    float w0 = w[offset + (code << 2) + 0], w1 = w[offset + (code << 2) + 1];
-    float w2 = w[offset + (code << 2) + 2], w3 = w[offset + (code << 2) + 3];
+    float w0 = s1[0], w1 = s1[1], w2 = s1[2], w3 = s1[3];
+    if (i > 0) {
+      w0 = w[offset + (code << 2) + 0];
+      w1 = w[offset + (code << 2) + 1];
+      w2 = w[offset + (code << 2) + 2];
+      w3 = w[offset + (code << 2) + 3];
+    }
     uint32_t local = 0;
     float partial = w0 + w2;
     float boundary = partial / total;
@@ -383,12 +471,12 @@ __device__ __forceinline__ void mq_sample(const MortonView& m, uint32_t cell, Pc
     size *= 4u;
   }
   uint32_t xi = compact1by1_16(code), yi = compact1by1_16(code >> 1);
-  float nn = (float)(1u << m.level);
-  float a = rnd(rng);
-  float b = rnd(rng);
-  ox = ((float)xi + a) / nn;
-  oy = ((float)yi + b) / nn;
-  pdf = total / w[0] * (nn * nn) * 0.25f * kInvPi;
+  float fn = (float)nn;
+  float u = rnd(rng);
+  float v = rnd(rng);
+  ax = ((float)xi + u) / fn;
+  ay = ((float)yi + v) / fn;
+  apdf = total / root * (fn * fn) * 0.25f * kInvPi;
 }
 
 }  // namespace rtpg

@@ -99,7 +99,12 @@ __global__ void k_morton_sample(MortonView m, const uint32_t* cells, uint32_t n,
   if (i >= n) return;
   Pcg32 rng = pcg32_seed(seed, i);
   float u, v, pdf;
-  mq_sample(m, cells[i], rng, u, v, pdf);
+  // chain b evaluates a point spread over the square beside the sampling
+  // chain; a lone evaluation must give the same bits, else the pdf is NaN
+  float bx = check_coord(i, 2654435769u), by = check_coord(i, 0x85EBCA6Bu), pb, pb1, unused;
+  mq_pair(m, cells[i], true, rng, u, v, pdf, true, bx, by, pb);
+  mq_pair(m, cells[i], false, rng, bx, by, pb1, false, 0.0f, 0.0f, unused);
+  if (__float_as_uint(pb) != __float_as_uint(pb1)) pdf = __uint_as_float(0x7FC00000u);
   out[3 * i] = u;
   out[3 * i + 1] = v;
   out[3 * i + 2] = pdf;
@@ -108,7 +113,10 @@ __global__ void k_morton_sample(MortonView m, const uint32_t* cells, uint32_t n,
 __global__ void k_morton_pdf(MortonView m, const uint32_t* cells, const float* canon, uint32_t n, float* out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = mq_pdf(m, cells[i], canon[2 * i], canon[2 * i + 1]);
+  Pcg32 rng = pcg32_seed(0u, i);  // drawn from by neither chain
+  float u = canon[2 * i], v = canon[2 * i + 1], pa, pb;
+  mq_pair(m, cells[i], false, rng, u, v, pa, true, canon[2 * i], canon[2 * i + 1], pb);
+  out[i] = __float_as_uint(pa) == __float_as_uint(pb) ? pa : __uint_as_float(0x7FC00000u);
 }
 
 __global__ void k_fill_zero(float* p, size_t n) {

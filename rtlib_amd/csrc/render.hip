@@ -747,7 +747,12 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
           const bool guided = kGuide == 3 ? mx != nullptr
                                           : (kGuide != 0) && (rp.flags & kFlagGuideSample) && gcell != kInvalidCell;
           const float alpha = kGuide == 3 ? csel : rp.fraction;
-          // ---- next-event estimation (SimpleTrace.cu:302-339)
+          // ---- next-event estimation (SimpleTrace.cu:302-339); nee_state 1:
+          // a light sample without MIS, 2: with MIS, completed after the
+          // direction sampling
+          uint32_t nee_state = 0u;
+          float nee_pl = 0.0f, nee_bp = 0.0f, nee_cos = 0.0f;
+          V3 nee_f{0.0f, 0.0f, 0.0f};
           if (nee) {
             if (ris) {
               float wsum = 0.0f, fa_y = 0.0f, dist_y = 0.0f;
@@ -792,73 +797,71 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
               float cos_l = -dot(dir, ls.n);
               float cos_s = dot(dir, n);
               if (cos_l > 0.0f && cos_s > 0.0f) {
+                // the MIS weight waits for the guide's density of this
+                // direction, which is evaluated together with the sampling
+                // below
                 BsdfEval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
-                float pl = ls.pdf_sel / ls.area * dist2 / cos_l;
-                float w = 1.0f;
-                if (mis) {
-                  float bp = be.pdf;
-                  if (guided) {
-                    float gp;
-                    if (kGuide == 3) {
-                      gp = mix_pdf(mx, dir);
-                    } else {
-                      float cu, cv;
-                      dir_to_canonical(dir, cu, cv);
-                      gp = kGuide == 1 ? dtree_pdf(sd, gcell, cu, cv) : mq_pdf(mq, gcell, cu, cv);
-                    }
-                    bp = (1.0f - alpha) * bp + alpha * gp;
-                  }
-                  w = power_heuristic(pl, bp);
-                }
-                scon = T * be.f * ls.Le * (cos_s * w / pl);
+                nee_pl = ls.pdf_sel / ls.area * dist2 / cos_l;
+                nee_bp = be.pdf;
+                nee_cos = cos_s;
+                nee_f = T * be.f * ls.Le;
+                nee_state = mis ? 2u : 1u;
                 sdir = dir;
                 sdist = dist - 1e-4f;
                 slight = ls.idx;
-                shadow = max3(scon) > 0.0f;
               }
             }
           }
           // ---- direction sampling: BSDF or guide mixture (SimpleGuide.cu:359-377)
           V3 dir;
-          // density of a direction drawn from the SD-tree, from the sampling descent
-          float tree_pdf = -1.0f;
-          if (guided) {
-            float us = rnd(rng);
-            if (us < alpha) {
-              if (kGuide == 3) {
-                dir = mix_sample(mx, rng);
+          // the guide's density of the continuation direction and of the
+          // light-sample direction (needed when NEE weights by MIS)
+          float gp = 0.0f, gp_nee = 0.0f;
+          const bool need_nee = guided && nee_state == 2u;
+          if (kGuide == 1 || kGuide == 2) {
+            const bool gsample = guided && rnd(rng) < alpha;
+            if (!gsample) dir = bsdf_sample(type, albedo, param, spec, n, wo, rng);
+            if (guided) {
+              // a direction drawn from the tree, or the BSDF direction's
+              // density, in the same descent loop as the light direction's
+              // density (SD-tree), or the same load round (hash grid:
+              // MortonQuadTreeWrapperT::SampleAndPdf, OPX7UtilsMorton.h:486-500,
+              // gives the density of the sampled bin)
+              float ncu = 0.0f, ncv = 0.0f;
+              if (need_nee) dir_to_canonical(sdir, ncu, ncv);
+              if (!gsample) dir_to_canonical(dir, vcu, vcv);
+              float gu = vcu, gv = vcv;
+              if (kGuide == 1) {
+                dtree_pair(sd, gcell, gsample, rng, gu, gv, gp, need_nee, ncu, ncv, gp_nee);
               } else {
-                float gu, gv;
-                if (kGuide == 1) {
-                  dtree_sample(sd, gcell, rng, gu, gv, tree_pdf);
-                } else {
-                  // MortonQuadTreeWrapperT::SampleAndPdf (OPX7UtilsMorton.h:486-500):
-                  // the density of the sampled bin
-                  mq_sample(mq, gcell, rng, gu, gv, tree_pdf);
-                }
-                dir = normalize(canonical_to_dir(gu, gv));
+                mq_pair(mq, gcell, gsample, rng, gu, gv, gp, need_nee, ncu, ncv, gp_nee);
               }
-            } else {
-              dir = bsdf_sample(type, albedo, param, spec, n, wo, rng);
+              if (gsample) {
+                dir = normalize(canonical_to_dir(gu, gv));
+                dir_to_canonical(dir, vcu, vcv);
+              }
             }
+          } else if (guided) {
+            dir = rnd(rng) < alpha ? mix_sample(mx, rng) : bsdf_sample(type, albedo, param, spec, n, wo, rng);
+            gp = mix_pdf(mx, dir);
+            if (need_nee) gp_nee = mix_pdf(mx, sdir);
+            if (vcell != kInvalidCell) dir_to_canonical(dir, vcu, vcv);
           } else {
             dir = bsdf_sample(type, albedo, param, spec, n, wo, rng);
+          }
+          if (nee_state != 0u) {
+            float w = 1.0f;
+            if (nee_state == 2u) {
+              float bp = nee_bp;
+              if (guided) bp = (1.0f - alpha) * bp + alpha * gp_nee;
+              w = power_heuristic(nee_pl, bp);
+            }
+            scon = nee_f * (nee_cos * w / nee_pl);
+            shadow = max3(scon) > 0.0f;
           }
           BsdfEval be = bsdf_eval(type, albedo, param, spec, n, wo, dir);
           wo_pdf = be.pdf;
           if (guided) {
-            float gp;
-            if (kGuide == 3) {
-              gp = mix_pdf(mx, dir);
-              if (vcell != kInvalidCell) dir_to_canonical(dir, vcu, vcv);
-            } else {
-              dir_to_canonical(dir, vcu, vcv);
-              if (kGuide == 1) {
-                gp = tree_pdf >= 0.0f ? tree_pdf : dtree_pdf(sd, gcell, vcu, vcv);
-              } else {
-                gp = tree_pdf >= 0.0f ? tree_pdf : mq_pdf(mq, gcell, vcu, vcv);
-              }
-            }
             wo_pdf = (1.0f - alpha) * be.pdf + alpha * fmaxf(gp, 0.0f);
           } else if (record && vcell != kInvalidCell) {
             dir_to_canonical(dir, vcu, vcv);
@@ -1270,7 +1273,7 @@ __device__ __forceinline__ bool lds_table_add(uint32_t* keys, T* vals, uint32_t 
 
 // Flux slot of a record (DTreeNodeT::Record :113-125): the first levels of the
 // descent from the record jump table (cell = top bits of u, v; the remaining
-// coordinates are exact, as in dtree_pdf), the rest node by node.
+// coordinates are exact, as in dpdf_jump_slot), the rest node by node.
 __device__ __forceinline__ uint32_t record_key(const SdTreeView& sd, uint32_t cell, float cu, float cv) {
   const float sn = (float)(1u << kPdfJumpBits);
   float gx = cu * sn, gy = cv * sn;

@@ -80,7 +80,7 @@ __global__ void k_dtree_interleave(DTreeArrays a) {
 }
 
 // Pdf jump table of the sampling trees: work-item (DTree j, cell c) runs the
-// first kPdfJumpBits levels of dtree_pdf's descent for the directions of cell
+// first kPdfJumpBits levels of the density walk (dpdf_step) for the directions of cell
 // c (child index bits = the cell's coordinate bits, most significant first),
 // multiplying the density factors in the same order, and records where the
 // descent continues (kJumpDone when it ended within those levels).
@@ -509,9 +509,13 @@ __global__ void k_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n,
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Pcg32 rng = pcg32_seed(seed, i);
-  float u, w;
-  float pdf;
-  dtree_sample(v, cells[i], rng, u, w, pdf);
+  float u, w, pdf;
+  // chain b evaluates a point spread over the square beside the sampling
+  // chain; a lone evaluation must give the same bits, else u is NaN
+  float bx = check_coord(i, 2654435769u), by = check_coord(i, 0x85EBCA6Bu), pb, pb1, unused;
+  dtree_pair(v, cells[i], true, rng, u, w, pdf, true, bx, by, pb);
+  dtree_pair(v, cells[i], false, rng, bx, by, pb1, false, 0.0f, 0.0f, unused);
+  if (__float_as_uint(pb) != __float_as_uint(pb1)) u = __uint_as_float(0x7FC00000u);
   out[2 * i] = u;
   out[2 * i + 1] = w;
 }
@@ -519,7 +523,10 @@ __global__ void k_sdtree_sample(SdTreeView v, const uint32_t* cells, uint32_t n,
 __global__ void k_sdtree_pdf(SdTreeView v, const uint32_t* cells, const float* canon, uint32_t n, float* out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = dtree_pdf(v, cells[i], canon[2 * i], canon[2 * i + 1]);
+  Pcg32 rng = pcg32_seed(0u, i);  // drawn from by neither chain
+  float u = canon[2 * i], w = canon[2 * i + 1], pa, pb;
+  dtree_pair(v, cells[i], false, rng, u, w, pa, true, canon[2 * i], canon[2 * i + 1], pb);
+  out[i] = __float_as_uint(pa) == __float_as_uint(pb) ? pa : __uint_as_float(0x7FC00000u);
 }
 
 __global__ void k_sdtree_lookup(SdTreeView v, const float* pos, uint32_t n, uint32_t* out) {

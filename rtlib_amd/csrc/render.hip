@@ -27,20 +27,6 @@ constexpr uint32_t kBlock = 128;
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Wave-aggregated queue append: one atomic per wave, returns this lane's slot.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-  uint64_t mask = __ballot(pred);
-  uint32_t lane = lane_id();
-  uint32_t base = 0;
-  if (mask) {
-    int leader = __ffsll((long long)mask) - 1;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-  }
-  uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64u - lane)));
-  return base + (uint32_t)__popcll(below);
-}
-
 __device__ __forceinline__ V3 load3(float4 v) { return V3{v.x, v.y, v.z}; }
 
 // The queue region of a work-group whose first slot is b0: its first slot
@@ -537,12 +523,13 @@ __device__ __forceinline__ void block_append2(uint32_t* ca, bool pa, uint32_t* c
 
 // kRis instantiates the RIS light selection; without it the kernel needs
 // fewer registers and runs at a higher occupancy.
-// SD-tree shading is held to 80 VGPRs (6 waves per SIMD, as plain shading).
+// SD-tree and hash-grid shading are held to 80 VGPRs (6 waves per SIMD, as
+// plain shading; the hash grid's 82 gave 5 and took 8% longer).
 // kTex instantiates texture and vertex-normal reads (scenes with attributes
 // only: the code costs registers everywhere else).
 template <int kGuide, bool kAdj, bool kRis, bool kTex>
 constexpr unsigned shade_min_waves() {
-  return (kGuide == 1 && !kAdj && !kRis && !kTex) ? 6u : 1u;
+  return ((kGuide == 1 || kGuide == 2) && !kAdj && !kRis && !kTex) ? 6u : 1u;
 }
 
 template <int kGuide, bool kAdj = false, bool kRis = false, bool kTex = false>

@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4
 # cycles (16 lanes), at the 2.4 GHz peak engine clock
 VALU_PEAK_GWIS = 256 * 4 * 2.4 / 4  # 614.4 G wave-instructions/s
-NODE_BYTES = 112      # what the traversal reads of a 128-byte BVH4 node: four child boxes + child entries
+NODE_BYTES = 80       # a BVH4 node: four child boxes (binary16 planes), child entries, corner
 TRI_BYTES = 36        # three float3 vertices per tested triangle
 RAY_IN_BYTES = 32     # origin + direction
 HIT_OUT_BYTES = 16    # t, u, v, triangle

@@ -34,6 +34,32 @@ struct Box {
   bool valid() const { return lo[0] <= hi[0]; }
 };
 
+// binary16 bits of v >= 0 rounded down (up = false) or up, then one more
+// binary16 step in the same direction.  frexp splits v exactly, and the
+// mantissa scaled to 10 fraction bits is exact in double.
+uint16_t half_outward(double v, bool up) {
+  uint32_t b;
+  if (v <= 0.0) {
+    b = 0u;
+  } else if (v >= 65504.0) {
+    b = up && v > 65504.0 ? kHalfInf : 0x7BFFu;
+  } else {
+    int e;
+    double m = std::frexp(v, &e);  // v = m 2^e, m in [0.5, 1)
+    int E = e - 1;                 // v = (2m) 2^E, 2m in [1, 2)
+    if (E < -14) {
+      double q = std::ldexp(v, 24);  // subnormal: multiples of 2^-24
+      b = (uint32_t)(up ? std::ceil(q) : std::floor(q));
+    } else {
+      double f = (2.0 * m - 1.0) * 1024.0;
+      uint32_t fi = (uint32_t)(up ? std::ceil(f) : std::floor(f));
+      b = ((uint32_t)(E + 15) << 10) + fi;  // fi = 1024 carries into the exponent
+    }
+  }
+  if (up) return (uint16_t)(b >= kHalfInf ? kHalfInf : b + 1u);
+  return (uint16_t)(b == 0u ? 0x8001u : b - 1u);  // 0x8001: -2^-24
+}
+
 struct PrimRef {
   Box box;
   float c[3];
@@ -198,24 +224,28 @@ struct Collapser {
       }
     }
     BvhNode4& n = res.nodes[me];
+    bool used[4];
+    float org[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
     for (int i = 0; i < 4; ++i) {
       // an empty leaf (the tiny-scene path) is an empty slot
-      const bool used = i < nk && !(child_ref[i] < 0 && child_cnt[i] == 0);
-      n.pad[i] = 0;
-      if (used) {
+      used[i] = i < nk && !(child_ref[i] < 0 && child_cnt[i] == 0);
+      if (used[i])
+        for (int k = 0; k < 3; ++k) org[k] = std::min(org[k], bin[kids[i]].box.lo[k]);
+    }
+    for (int k = 0; k < 3; ++k) n.org[k] = org[k] == FLT_MAX ? 0.0f : org[k];
+    n.pad = 0;
+    uint16_t* planes[3] = {n.x, n.y, n.z};
+    for (int i = 0; i < 4; ++i) {
+      if (used[i]) {
         const Box& bx = bin[kids[i]].box;
-        n.lo_x[i] = bx.lo[0];
-        n.hi_x[i] = bx.hi[0];
-        n.lo_y[i] = bx.lo[1];
-        n.hi_y[i] = bx.hi[1];
-        n.lo_z[i] = bx.lo[2];
-        n.hi_z[i] = bx.hi[2];
+        for (int k = 0; k < 3; ++k) {
+          planes[k][i] = half_outward((double)bx.lo[k] - (double)n.org[k], false);
+          planes[k][4 + i] = half_outward((double)bx.hi[k] - (double)n.org[k], true);
+        }
         n.entry[i] = child_ref[i] >= 0 ? (uint32_t)child_ref[i]
                                        : (kLeafEntry | ((child_cnt[i] - 1u) << 27) | (uint32_t)(~child_ref[i]));
       } else {
-        n.lo_x[i] = n.hi_x[i] = kEmptyBox;
-        n.lo_y[i] = n.hi_y[i] = kEmptyBox;
-        n.lo_z[i] = n.hi_z[i] = kEmptyBox;
+        for (int k = 0; k < 3; ++k) planes[k][i] = planes[k][4 + i] = kHalfInf;
         n.entry[i] = 0;
       }
     }

@@ -27,6 +27,8 @@ constexpr uint32_t kBlock = 128;
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
 __device__ __forceinline__ V3 load3(float4 v) { return V3{v.x, v.y, v.z}; }
 
 // The queue region of a work-group whose first slot is b0: its first slot
@@ -180,20 +182,23 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
     // leaf or has nothing left, so that the leaf phase runs with full waves
     while (cur != kNoEntry && !(cur & kLeafEntry)) {
       if (kCount) ++wcnt[0];
-      const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
-      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+      // the planes are binary16 offsets from the node's corner (bvh.h): the
+      // corner's ray distance once per node, then one v_fma_mix_f32 per plane
+      const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + cur);
+      const half8 px = __builtin_bit_cast(half8, np[0]), py = __builtin_bit_cast(half8, np[1]);
+      const half8 pz = __builtin_bit_cast(half8, np[2]);
+      const uint4 ch = np[3];
+      const float4 org = reinterpret_cast<const float4*>(np)[4];
+      const float cx = fmaf(org.x, inv.x, -oi.x), cy = fmaf(org.y, inv.y, -oi.y), cz = fmaf(org.z, inv.z, -oi.z);
       float tk[4];
       uint32_t nk[4];
       int n_hit = 0;
       uint32_t hmask = 0;  // children whose box the ray enters
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        float lox = f4c(lx, i), hix = f4c(hx, i), loy = f4c(ly, i), hiy = f4c(hy, i);
-        float loz = f4c(lz, i), hiz = f4c(hz, i);
-        float t0x = lox * inv.x - oi.x, t1x = hix * inv.x - oi.x;
-        float t0y = loy * inv.y - oi.y, t1y = hiy * inv.y - oi.y;
-        float t0z = loz * inv.z - oi.z, t1z = hiz * inv.z - oi.z;
+        float t0x = fmaf((float)px[i], inv.x, cx), t1x = fmaf((float)px[4 + i], inv.x, cx);
+        float t0y = fmaf((float)py[i], inv.y, cy), t1y = fmaf((float)py[4 + i], inv.y, cy);
+        float t0z = fmaf((float)pz[i], inv.z, cz), t1z = fmaf((float)pz[4 + i], inv.z, cz);
         float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
         bool hit = tn <= tf * 1.00000024f;

@@ -27,7 +27,7 @@ constexpr uint32_t kBlock = 128;
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ V3 load3(float4 v) { return V3{v.x, v.y, v.z}; }
 
@@ -155,6 +155,8 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
              fabsf(d.z) > eps ? d.z : copysignf(eps, d.z)};
   V3 inv = V3{1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z};
   V3 oi = o * inv;
+  // per axis, whether the ray enters a box through its upper plane
+  const bool rx = inv.x < 0.0f, ry = inv.y < 0.0f, rz = inv.z < 0.0f;
   float tbest = tmax;
   int32_t best = -1;
   float bu = 0.0f, bv = 0.0f;
@@ -184,11 +186,19 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
       if (kCount) ++wcnt[0];
       // the planes are binary16 offsets from the node's corner (bvh.h): the
       // corner's ray distance once per node, then one v_fma_mix_f32 per plane
+      // Entry and exit planes per axis are the lower and upper ones, swapped
+      // where the ray runs downwards (fma is monotonic in the plane, so this
+      // equals a min and a max of the two distances).
       const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + cur);
-      const half8 px = __builtin_bit_cast(half8, np[0]), py = __builtin_bit_cast(half8, np[1]);
-      const half8 pz = __builtin_bit_cast(half8, np[2]);
+      const uint4 qx = np[0], qy = np[1], qz = np[2];
       const uint4 ch = np[3];
       const float4 org = reinterpret_cast<const float4*>(np)[4];
+      const half4 nx = __builtin_bit_cast(half4, rx ? make_uint2(qx.z, qx.w) : make_uint2(qx.x, qx.y));
+      const half4 fx = __builtin_bit_cast(half4, rx ? make_uint2(qx.x, qx.y) : make_uint2(qx.z, qx.w));
+      const half4 ny = __builtin_bit_cast(half4, ry ? make_uint2(qy.z, qy.w) : make_uint2(qy.x, qy.y));
+      const half4 fy = __builtin_bit_cast(half4, ry ? make_uint2(qy.x, qy.y) : make_uint2(qy.z, qy.w));
+      const half4 nz = __builtin_bit_cast(half4, rz ? make_uint2(qz.z, qz.w) : make_uint2(qz.x, qz.y));
+      const half4 fz = __builtin_bit_cast(half4, rz ? make_uint2(qz.x, qz.y) : make_uint2(qz.z, qz.w));
       const float cx = fmaf(org.x, inv.x, -oi.x), cy = fmaf(org.y, inv.y, -oi.y), cz = fmaf(org.z, inv.z, -oi.z);
       float tk[4];
       uint32_t nk[4];
@@ -196,11 +206,11 @@ __device__ bool traverse(const SceneView& sc, V3 o, V3 d, float tmin, float tmax
       uint32_t hmask = 0;  // children whose box the ray enters
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        float t0x = fmaf((float)px[i], inv.x, cx), t1x = fmaf((float)px[4 + i], inv.x, cx);
-        float t0y = fmaf((float)py[i], inv.y, cy), t1y = fmaf((float)py[4 + i], inv.y, cy);
-        float t0z = fmaf((float)pz[i], inv.z, cz), t1z = fmaf((float)pz[4 + i], inv.z, cz);
-        float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
+        float t0x = fmaf((float)nx[i], inv.x, cx), t1x = fmaf((float)fx[i], inv.x, cx);
+        float t0y = fmaf((float)ny[i], inv.y, cy), t1y = fmaf((float)fy[i], inv.y, cy);
+        float t0z = fmaf((float)nz[i], inv.z, cz), t1z = fmaf((float)fz[i], inv.z, cz);
+        float tn = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
+        float tf = fminf(fminf(t1x, t1y), fminf(t1z, tbest));
         bool hit = tn <= tf * 1.00000024f;
         tk[i] = hit ? tn : 3.0e38f;
         nk[i] = i == 0 ? ch.x : (i == 1 ? ch.y : (i == 2 ? ch.z : ch.w));

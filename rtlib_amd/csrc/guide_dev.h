@@ -354,25 +354,27 @@ RTPG_HD void grid_hash(const HashGridView& h, uint32_t ix, uint32_t iy, uint32_t
 }
 
 // GetPrvCellIndex (:298-319): read-only probe of the previous pass's table.
-// The slots are scanned in order up to the first empty one; the whole
-// 128-byte block is read at once (eight 16-byte loads issued together), and
-// the scan runs on bit masks: the result is the first slot holding the
-// checksum if no empty slot comes before it.
+// The slots are scanned in order up to the first empty one, on bit masks:
+// the result is the first slot holding the checksum if no empty slot comes
+// before it.  The first 8 slots of the 128-byte block are read first (two
+// 16-byte loads); the other 24 only when those hold neither the checksum nor
+// an empty slot (a bucket holds about one cell on average).
 RTPG_HD uint32_t grid_find_prv(const HashGridView& h, uint32_t bucket, uint32_t checksum) {
   const uint4* blk = reinterpret_cast<const uint4*>(h.prv + 32u * bucket);
-  uint4 v[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = blk[q];
   uint32_t empty = 0u, hit = 0u;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const uint32_t c[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      empty |= (c[j] == 0u ? 1u : 0u) << (4 * q + j);
-      hit |= (c[j] == checksum ? 1u : 0u) << (4 * q + j);
+  auto scan = [&](int q0, int q1) {
+    uint4 v[6];
+    for (int q = q0; q < q1; ++q) v[q - q0] = blk[q];
+    for (int q = q0; q < q1; ++q) {
+      const uint32_t c[4] = {v[q - q0].x, v[q - q0].y, v[q - q0].z, v[q - q0].w};
+      for (int j = 0; j < 4; ++j) {
+        empty |= (c[j] == 0u ? 1u : 0u) << (4 * q + j);
+        hit |= (c[j] == checksum ? 1u : 0u) << (4 * q + j);
+      }
     }
-  }
+  };
+  scan(0, 2);
+  if ((empty | hit) == 0u) scan(2, 8);
   if (hit == 0u) return kInvalidCell;
   const uint32_t first_hit = (uint32_t)__builtin_ctz(hit);
   const uint32_t first_empty = empty ? (uint32_t)__builtin_ctz(empty) : 32u;

@@ -110,7 +110,7 @@ def main(argv=None):
                     "learned-guide configurations")
     ap.add_argument("--mlp-passes", type=int, default=8)
     ap.add_argument("--mlp-steps", type=int, default=4)
-    ap.add_argument("--mlp-cache", type=int, default=48, help="spatial-cache cells per axis of the learned guide "
+    ap.add_argument("--mlp-cache", type=int, default=32, help="spatial-cache cells per axis of the learned guide "
                     "(0: network inference at every guided vertex)")
     ap.add_argument("--large-budget", type=int, default=1024, help="second budget for the equal-time comparison "
                     "of the SD-tree and the learned guide (0: skip)")

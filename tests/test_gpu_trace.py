@@ -53,6 +53,25 @@ def test_triangle_soup(ctx, orc):
     _check(ctx, orc, sc, _rays(rng, 4000, (-6, -6, -6), (6, 6, 6)))
 
 
+def _soup(rng, nt, center, spread, size):
+    c = center + rng.uniform(-spread, spread, (nt, 1, 3))
+    verts = (c + rng.normal(scale=size, size=(nt, 3, 3))).reshape(-1, 3).astype(np.float32)
+    return {"verts": verts, "idx": np.arange(3 * nt, dtype=np.uint32).reshape(-1, 3),
+            "albedo": np.full((nt, 4), 0.5, np.float32), "mat": np.zeros(nt, np.uint32),
+            "emission": np.zeros((nt, 3), np.float32), "spec": None, "camera": None}
+
+
+@pytest.mark.parametrize("center,spread,size", [(0.0, 2.0e5, 2.0e4), (1.0e3, 0.05, 0.004)])
+def test_node_planes_at_extreme_scales(ctx, orc, center, spread, size):
+    # the nodes hold binary16 plane offsets from their corner: offsets beyond
+    # 65504 (first scene) become +inf upper planes, and tiny boxes far from
+    # the origin (second) rely on the offsets being relative to the corner
+    rng = np.random.default_rng(4)
+    sc = _soup(rng, 3000, center, spread, size)
+    r = 1.2 * spread
+    _check(ctx, orc, sc, _rays(rng, 4000, (center - r,) * 3, (center + r,) * 3))
+
+
 def test_heightfield_room_deep_bvh(ctx, orc):
     rng = np.random.default_rng(2)
     sc = scenes.guided_room(grid=(256, 128))

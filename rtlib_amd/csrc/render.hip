@@ -1317,8 +1317,12 @@ __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps
   constexpr uint32_t kLeafTab = CommitShape<kKind>::leaf, kCellTab = CommitShape<kKind>::cell;
   __shared__ uint32_t lkey[kLeafTab];
   __shared__ float lval[kLeafTab];
-  __shared__ uint32_t ckey[kCellTab];
-  __shared__ uint32_t cval[kCellTab];
+  // DTree weights: with at most 2 kCellTab DTrees a count per DTree (no
+  // probing), else an open-addressing table of kCellTab keys and counts
+  __shared__ uint32_t cwords[2 * kCellTab];
+  uint32_t* const ckey = cwords;
+  uint32_t* const cval = cwords + kCellTab;
+  const bool cdense = kKind == 1 && sd.n_dtrees <= 2u * kCellTab;
   const uint32_t first = blockIdx.x * kCommitBlock * kCommitPaths;
   if (first >= ps.n) return;  // uniform over the work-group
   for (uint32_t i = threadIdx.x; i < kLeafTab; i += kCommitBlock) {
@@ -1327,7 +1331,7 @@ __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps
   }
   if (kKind == 1) {
     for (uint32_t i = threadIdx.x; i < kCellTab; i += kCommitBlock) {
-      ckey[i] = kEmptyKey;
+      ckey[i] = cdense ? 0u : kEmptyKey;
       cval[i] = 0u;
     }
   }
@@ -1371,7 +1375,11 @@ __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps
       const float value = ((lx + ly + lz) / 3.0f) / wo_pdf;
       const float cu = vc.x, cv = vc.y;
       if (kKind == 1) {
-        if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) unsafeAtomicAdd(sd.build_sw + cell, 1.0f);
+        if (cdense) {
+          atomicAdd(&cwords[cell], 1u);
+        } else if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) {
+          unsafeAtomicAdd(sd.build_sw + cell, 1.0f);
+        }
         if (!(isfinite(value) && value > 0.0f && cu >= 0.0f)) continue;
       }
       const uint32_t key = kKind == 1 ? record_key(sd, cell, cu, cv) : morton_key(mq, cell, cu, cv);
@@ -1383,8 +1391,14 @@ __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps
     if (lkey[i] != kEmptyKey) unsafeAtomicAdd(flat + lkey[i], lval[i]);
   }
   if (kKind == 1) {
-    for (uint32_t i = threadIdx.x; i < kCellTab; i += kCommitBlock) {
-      if (ckey[i] != kEmptyKey) unsafeAtomicAdd(sd.build_sw + ckey[i], (float)cval[i]);
+    if (cdense) {
+      for (uint32_t i = threadIdx.x; i < sd.n_dtrees; i += kCommitBlock) {
+        if (cwords[i] != 0u) unsafeAtomicAdd(sd.build_sw + i, (float)cwords[i]);
+      }
+    } else {
+      for (uint32_t i = threadIdx.x; i < kCellTab; i += kCommitBlock) {
+        if (ckey[i] != kEmptyKey) unsafeAtomicAdd(sd.build_sw + ckey[i], (float)cval[i]);
+      }
     }
   }
 }

@@ -1349,12 +1349,15 @@ __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps
     uint32_t nh_max = nh;
     for (int off = 32; off >= 1; off >>= 1) nh_max = max(nh_max, (uint32_t)__shfl_xor((int)nh_max, off));
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    // vB record of hit d + 1 (its contribution); read once per hit, it then
+    // gives the cosine of vertex d + 1 and the contribution at d
+    float4 b = nh >= 2u ? ps.vB[(size_t)(nh - 1u) * ps.n + pid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int d = (int)nh_max - 2; d >= 0; --d) {
       if ((uint32_t)d + 1u < nh) {
-        float4 b = ps.vB[(size_t)(d + 1) * ps.n + pid];
         rx += b.x;
         ry += b.y;
         rz += b.z;
+        b = ps.vB[(size_t)d * ps.n + pid];
       }
       if ((uint32_t)d >= nv) continue;
       const size_t vi = (size_t)d * ps.n + pid;
@@ -1368,7 +1371,7 @@ __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps
       float lx = va.x * wo_pdf > 1e-4f ? rx / va.x : 0.0f;
       float ly = va.y * wo_pdf > 1e-4f ? ry / va.y : 0.0f;
       float lz = va.z * wo_pdf > 1e-4f ? rz / va.z : 0.0f;
-      const float cosv = fabsf(ps.vB[vi].w);
+      const float cosv = fabsf(b.w);  // b: this vertex's record
       lx *= cosv;
       ly *= cosv;
       lz *= cosv;

@@ -910,8 +910,10 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
       ps.nhit[pid] = depth + 1u;
       size_t vi = (size_t)depth * ps.n + pid;
       ps.vB[vi] = make_float4(c.x, c.y, c.z, cos_out);
-      ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
-      if (kGuide != 3 || sd.stree) {
+      // a path's last hit is never committed (only its contribution, vB, is
+      // read): its throughput and guide records are not written
+      if (cont) ps.vA[vi] = make_float4(Tn.x, Tn.y, Tn.z, wo_pdf);
+      if (cont && (kGuide != 3 || sd.stree)) {
         bool has_vertex = cont && vcell != kInvalidCell;
         if ((kGuide == 1 || kGuide == 3) && sd.filter && has_vertex) {
           // stochastic box filters: the record moves to a random point of the

@@ -1250,21 +1250,26 @@ constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
 // smaller shape.
 template <int kKind>
 struct CommitShape;
+// probes: LDS slots tried before a record goes to global memory (the hash
+// grid's small table is often full, where long probe runs only cost time:
+// 4 probes instead of 16 measured -7% on its commit).
 template <>
 struct CommitShape<1> {
   static constexpr uint32_t block = 1024, paths = 8, leaf = 16384, cell = 2048;
+  static constexpr int probes = 16;
 };
 template <>
 struct CommitShape<2> {
   static constexpr uint32_t block = 256, paths = 32, leaf = 4096, cell = 1;
+  static constexpr int probes = 4;
 };
 
-// Insert-or-add into an LDS open-addressing table; false when 16 probes miss
-// (the caller then adds to global memory directly).
-template <class T>
+// Insert-or-add into an LDS open-addressing table; false when kProbes probes
+// miss (the caller then adds to global memory directly).
+template <int kProbes, class T>
 __device__ __forceinline__ bool lds_table_add(uint32_t* keys, T* vals, uint32_t mask, uint32_t key, T v) {
   uint32_t h = (key * 0x9E3779B1u) & mask;
-  for (int probe = 0; probe < 16; ++probe) {
+  for (int probe = 0; probe < kProbes; ++probe) {
     uint32_t prev = atomicCAS(&keys[h], kEmptyKey, key);
     if (prev == kEmptyKey || prev == key) {
       atomicAdd(&vals[h], v);
@@ -1382,13 +1387,13 @@ __global__ void __launch_bounds__(CommitShape<kKind>::block) k_commit(PathSoA ps
       if (kKind == 1) {
         if (cdense) {
           atomicAdd(&cwords[cell], 1u);
-        } else if (!lds_table_add<uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) {
+        } else if (!lds_table_add<CommitShape<kKind>::probes, uint32_t>(ckey, cval, kCellTab - 1u, cell, 1u)) {
           unsafeAtomicAdd(sd.build_sw + cell, 1.0f);
         }
         if (!(isfinite(value) && value > 0.0f && cu >= 0.0f)) continue;
       }
       const uint32_t key = kKind == 1 ? record_key(sd, cell, cu, cv) : morton_key(mq, cell, cu, cv);
-      if (!lds_table_add<float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
+      if (!lds_table_add<CommitShape<kKind>::probes, float>(lkey, lval, kLeafTab - 1u, key, value)) unsafeAtomicAdd(flat + key, value);
     }
   }
   __syncthreads();

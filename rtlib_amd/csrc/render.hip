@@ -840,7 +840,9 @@ k_shade(RenderParams rp, SceneView sc, PathSoA ps, SdTreeView sd, HashGridView h
               }
               if (gsample) {
                 dir = normalize(canonical_to_dir(gu, gv));
-                dir_to_canonical(dir, vcu, vcv);
+                // the recorded coordinates are those of the normalised
+                // direction (only a recording pass reads them)
+                if (record) dir_to_canonical(dir, vcu, vcv);
               }
             }
           } else if (guided) {

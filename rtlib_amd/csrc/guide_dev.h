@@ -401,6 +401,9 @@ RTPG_HD uint32_t mq_level_offset(uint32_t level) { return ((1u << (2u * level)) 
 // self-checks of the sampling entry points.
 RTPG_HD float check_coord(uint32_t i, uint32_t mult) { return (float)((i * mult) >> 8) * 0x1p-24f; }
 
+// Four floats at a 4-byte-aligned address (one 16-byte load).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
 // Leaf bin of canonical (px, py) in a level-`level` Morton pyramid.
 RTPG_HD uint32_t mq_leaf(uint32_t level, float px, float py) {
   uint32_t nn = 1u << level;
@@ -427,11 +430,10 @@ __device__ __forceinline__ void mq_pair(const MortonView& m, uint32_t cell, bool
   const float root = w[0];
   const float lb = b_on ? w[mq_leaf(m.level, bx, by)] : 0.0f;
   const float la = a_sample ? 0.0f : w[mq_leaf(m.level, ax, ay)];
-  float s1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  if (a_sample && m.level > 0u) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) s1[k] = w[1 + k];
-  }
+  // the four values of a pyramid node are consecutive floats: one 16-byte
+  // load (4-byte aligned, as a cell's pyramid is 85 floats)
+  f4u s1 = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (a_sample && m.level > 0u) s1 = *reinterpret_cast<const f4u*>(w + 1);
   bpdf = root <= 0.0f ? 1.0f * 0.25f * kInvPi : lb / root * (float)(nn * nn) * 0.25f * kInvPi;
   if (!a_sample) {
     apdf = root <= 0.0f ? 1.0f * 0.25f * kInvPi : la / root * (float)(nn * nn) * 0.25f * kInvPi;
@@ -446,13 +448,8 @@ __device__ __forceinline__ void mq_pair(const MortonView& m, uint32_t cell, bool
   }
   uint32_t size = 4, offset = 1, code = 0;
   for (uint32_t i = 0; i < m.level; ++i) {
-    float w0 = s1[0], w1 = s1[1], w2 = s1[2], w3 = s1[3];
-    if (i > 0) {
-      w0 = w[offset + (code << 2) + 0];
-      w1 = w[offset + (code << 2) + 1];
-      w2 = w[offset + (code << 2) + 2];
-      w3 = w[offset + (code << 2) + 3];
-    }
+    const f4u q = i > 0 ? *reinterpret_cast<const f4u*>(w + offset + (code << 2)) : s1;
+    const float w0 = q.x, w1 = q.y, w2 = q.z, w3 = q.w;
     uint32_t local = 0;
     float partial = w0 + w2;
     float boundary = partial / total;
